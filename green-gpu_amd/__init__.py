@@ -1,0 +1,322 @@
+"""Python binding of libgreen_gpu_amd.so (the gfx950 HF/GW library).
+
+The binding mirrors the reference plugin's kernel objects
+(src/green/gpu/hf_gpu_kernel.h, gw_gpu_kernel.h): an HF kernel whose
+``solve(dm)`` returns the Fock contribution and a GW kernel whose
+``solve(G, Sigma)`` overwrites Sigma(tau).  Every call goes through the C ABI
+declared in include/green_gpu_amd.h; there is no Python or CPU fallback - if
+the shared library is missing, importing the kernels raises.
+
+Tensor layouts are the reference's host layouts (complex128, row-major):
+G, Sigma (nts, ns, ink, nao, nao); dm, S_k, Fock (ns, ink, nao, nao);
+stored integrals (num_kpair_stored, NQ, nao, nao).
+"""
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgreen_gpu_amd.so")
+
+PRECISION = {"fp64": 0, "fp32": 1, "bf16x3": 2}
+SOLVER = {"cholesky": 0, "lu": 1}
+
+
+class GgpuError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load the shared library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} not built; run green-gpu_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        vp, i32, u64, f64, sz = C.c_void_p, C.c_int, C.c_uint64, C.c_double, C.c_size_t
+        pd = C.POINTER(C.c_double)
+        L.ggpu_last_error.restype = C.c_char_p
+        L.ggpu_device_count.argtypes = [C.POINTER(C.c_int)]
+        L.ggpu_device_memory.argtypes = [C.POINTER(sz), C.POINTER(sz)]
+        L.ggpu_hf_plan_create.argtypes = [C.POINTER(vp), i32, i32, i32, C.POINTER(SymmetryC)]
+        L.ggpu_hf_load_integrals.argtypes = [vp, pd]
+        L.ggpu_hf_synth_integrals.argtypes = [vp, u64, f64]
+        L.ggpu_hf_exchange.argtypes = [vp, pd, pd]
+        L.ggpu_hf_solve.argtypes = [vp, pd, pd, f64, pd]
+        L.ggpu_hf_plan_destroy.argtypes = [vp]
+        L.ggpu_hf_plan_destroy.restype = None
+        L.ggpu_gw_plan_create.argtypes = [C.POINTER(vp), i32, i32, i32, i32, i32, C.POINTER(SymmetryC),
+                                          pd, pd, i32, i32, sz]
+        L.ggpu_gw_load_integrals.argtypes = [vp, pd]
+        L.ggpu_gw_synth_integrals.argtypes = [vp, u64, f64]
+        L.ggpu_gw_iterate.argtypes = [vp, pd, pd]
+        L.ggpu_gw_iterate_device.argtypes = [vp, vp, vp]
+        L.ggpu_gw_polarization.argtypes = [vp, pd, i32, i32, pd]
+        L.ggpu_gw_last_timings.argtypes = [vp, pd]
+        L.ggpu_gw_plan_destroy.argtypes = [vp]
+        L.ggpu_gw_plan_destroy.restype = None
+        _lib = L
+    return _lib
+
+
+EXPORTED = ["ggpu_last_error", "ggpu_device_count", "ggpu_device_memory",
+            "ggpu_hf_plan_create", "ggpu_hf_load_integrals", "ggpu_hf_synth_integrals",
+            "ggpu_hf_exchange", "ggpu_hf_solve", "ggpu_hf_plan_destroy",
+            "ggpu_gw_plan_create", "ggpu_gw_load_integrals", "ggpu_gw_synth_integrals",
+            "ggpu_gw_iterate", "ggpu_gw_iterate_device", "ggpu_gw_polarization",
+            "ggpu_gw_last_timings", "ggpu_gw_plan_destroy"]
+
+
+def _check(rc):
+    if rc != 0:
+        raise GgpuError(rc, lib().ggpu_last_error().decode())
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _c128(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    if shape is not None and a.shape != tuple(shape):
+        raise ValueError(f"expected shape {tuple(shape)}, got {a.shape}")
+    return a
+
+
+# ---------------------------------------------------------------------------
+# Symmetry tables
+# ---------------------------------------------------------------------------
+class SymmetryC(C.Structure):
+    _fields_ = [("nk", C.c_int32), ("ink", C.c_int32), ("nq", C.c_int32), ("inq", C.c_int32)] + [
+        (n, C.c_void_p) for n in (
+            "k_full_to_reduced", "k_reduced_to_full", "k_tr_conj", "k_star_offsets", "k_star_members",
+            "q_full_to_reduced", "q_reduced_to_full", "q_tr_conj", "q_star_offsets", "q_star_members",
+            "k1_from_k2q", "k2_from_k1q", "U_k", "U_q", "conj_kpair", "trans_kpair", "irre_pos")
+    ] + [("num_kpair_stored", C.c_int64)]
+
+
+@dataclass
+class Symmetry:
+    """Plain symmetry tables (the fields of green-symmetry the path uses)."""
+    nk: int
+    ink: int
+    nq: int
+    inq: int
+    k_full_to_reduced: np.ndarray
+    k_reduced_to_full: np.ndarray
+    k_tr_conj: np.ndarray
+    k_stars: list
+    q_full_to_reduced: np.ndarray
+    q_reduced_to_full: np.ndarray
+    q_tr_conj: np.ndarray
+    q_stars: list
+    k1_from_k2q: np.ndarray
+    k2_from_k1q: np.ndarray
+    U_k: object = None
+    U_q: object = None
+    conj_kpair: object = None
+    trans_kpair: object = None
+    irre_pos: object = None
+    num_kpair_stored: int = 0
+    _keep: list = field(default_factory=list, repr=False)
+
+    @classmethod
+    def like(cls, other):
+        """Copy the tables of any object exposing the same attribute names."""
+        names = [f for f in cls.__dataclass_fields__ if not f.startswith("_")]
+        return cls(**{n: getattr(other, n, None) for n in names})
+
+    def to_c(self):
+        keep = []
+
+        def i64(a):
+            a = np.ascontiguousarray(a, dtype=np.int64)
+            keep.append(a)
+            return a.ctypes.data
+
+        def c128(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.complex128)
+            keep.append(a)
+            return a.ctypes.data
+
+        ko = np.concatenate([[0], np.cumsum([len(s) for s in self.k_stars])]).astype(np.int64)
+        qo = np.concatenate([[0], np.cumsum([len(s) for s in self.q_stars])]).astype(np.int64)
+        c = SymmetryC()
+        c.nk, c.ink, c.nq, c.inq = int(self.nk), int(self.ink), int(self.nq), int(self.inq)
+        c.k_full_to_reduced = i64(self.k_full_to_reduced)
+        c.k_reduced_to_full = i64(self.k_reduced_to_full)
+        c.k_tr_conj = i64(self.k_tr_conj)
+        c.k_star_offsets = i64(ko)
+        c.k_star_members = i64(np.concatenate([np.asarray(s) for s in self.k_stars]))
+        c.q_full_to_reduced = i64(self.q_full_to_reduced)
+        c.q_reduced_to_full = i64(self.q_reduced_to_full)
+        c.q_tr_conj = i64(self.q_tr_conj)
+        c.q_star_offsets = i64(qo)
+        c.q_star_members = i64(np.concatenate([np.asarray(s) for s in self.q_stars]))
+        c.k1_from_k2q = i64(self.k1_from_k2q)
+        c.k2_from_k1q = i64(self.k2_from_k1q)
+        c.U_k = c128(self.U_k)
+        c.U_q = c128(self.U_q)
+        if self.irre_pos is not None:
+            c.conj_kpair = i64(self.conj_kpair)
+            c.trans_kpair = i64(self.trans_kpair)
+            c.irre_pos = i64(self.irre_pos)
+            c.num_kpair_stored = int(self.num_kpair_stored)
+        else:
+            c.num_kpair_stored = int(self.nk * (self.nk + 1) // 2)
+        self._keep = keep
+        return c
+
+
+def mesh_symmetry_none(mesh):
+    """Tables of a Monkhorst-Pack mesh (n1, n2, n3) with no symmetry
+    reduction: k = (i1 n2 + i2) n3 + i3, k1 = k2 + q and k2 = k1 - q
+    component-wise modulo the mesh, U = I, every pair k1 >= k2 stored."""
+    n1, n2, n3 = (int(x) for x in mesh)
+    nk = n1 * n2 * n3
+    i = np.arange(nk)
+    c = np.stack([i // (n2 * n3), (i // n3) % n2, i % n3], axis=1)
+    n = np.array([n1, n2, n3])
+
+    def idx(v):
+        v = v % n
+        return (v[..., 0] * n2 + v[..., 1]) * n3 + v[..., 2]
+
+    k1 = idx(c[:, None, :] + c[None, :, :])
+    k2 = idx(c[:, None, :] - c[None, :, :])
+    ar = np.arange(nk, dtype=np.int64)
+    return Symmetry(nk=nk, ink=nk, nq=nk, inq=nk,
+                    k_full_to_reduced=ar, k_reduced_to_full=ar, k_tr_conj=np.zeros(nk, np.int64),
+                    k_stars=[[k] for k in range(nk)],
+                    q_full_to_reduced=ar, q_reduced_to_full=ar, q_tr_conj=np.zeros(nk, np.int64),
+                    q_stars=[[q] for q in range(nk)],
+                    k1_from_k2q=k1.astype(np.int64), k2_from_k1q=k2.astype(np.int64),
+                    num_kpair_stored=nk * (nk + 1) // 2)
+
+
+def device_count():
+    n = C.c_int(0)
+    _check(lib().ggpu_device_count(C.byref(n)))
+    return n.value
+
+
+def device_memory():
+    f, t = C.c_size_t(0), C.c_size_t(0)
+    _check(lib().ggpu_device_memory(C.byref(f), C.byref(t)))
+    return f.value, t.value
+
+
+# ---------------------------------------------------------------------------
+# Kernels
+# ---------------------------------------------------------------------------
+class HfKernel:
+    """HF kernel on the GPU (reference: scalar_hf_gpu_kernel, hf_gpu_kernel.h:67-84)."""
+
+    def __init__(self, nao, ns, NQ, sym, madelung=0.0, S_k=None):
+        self.nao, self.ns, self.NQ = int(nao), int(ns), int(NQ)
+        self.sym = sym if isinstance(sym, Symmetry) else Symmetry.like(sym)
+        self.madelung = float(madelung)
+        self.S_k = None if S_k is None else _c128(S_k, (self.ns, self.sym.ink, self.nao, self.nao))
+        h = C.c_void_p()
+        self._csym = self.sym.to_c()
+        _check(lib().ggpu_hf_plan_create(C.byref(h), self.nao, self.ns, self.NQ, C.byref(self._csym)))
+        self._h = h
+
+    def load_integrals(self, V_stored):
+        V = _c128(V_stored)
+        _check(lib().ggpu_hf_load_integrals(self._h, _dptr(V)))
+
+    def synth_integrals(self, seed, sigma=0.1):
+        _check(lib().ggpu_hf_synth_integrals(self._h, int(seed), float(sigma)))
+
+    def exchange(self, dm):
+        dm = _c128(dm, (self.ns, self.sym.ink, self.nao, self.nao))
+        K = np.empty_like(dm)
+        _check(lib().ggpu_hf_exchange(self._h, _dptr(dm), _dptr(K)))
+        return K
+
+    def solve(self, dm):
+        """J + K - Ewald for the IBZ density matrix (hf_gpu_kernel.cpp:66-104)."""
+        dm = _c128(dm, (self.ns, self.sym.ink, self.nao, self.nao))
+        S = self.S_k if self.S_k is not None else np.zeros_like(dm)
+        F = np.empty_like(dm)
+        _check(lib().ggpu_hf_solve(self._h, _dptr(dm), _dptr(S), self.madelung, _dptr(F)))
+        return F
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ggpu_hf_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+class GwKernel:
+    """GW kernel on the GPU (reference: scalar_gw_gpu_kernel, gw_gpu_kernel.h:139-172)."""
+
+    def __init__(self, nao, ns, NQ, T_wt, T_tw, sym, precision="fp64", solver="lu", work_bytes=0):
+        T_wt = _c128(T_wt)
+        T_tw = _c128(T_tw)
+        self.nao, self.ns, self.NQ = int(nao), int(ns), int(NQ)
+        self.nw_b, self.nts = T_wt.shape[0], T_tw.shape[0]
+        self.sym = sym if isinstance(sym, Symmetry) else Symmetry.like(sym)
+        h = C.c_void_p()
+        self._csym = self.sym.to_c()
+        _check(lib().ggpu_gw_plan_create(C.byref(h), self.nao, self.ns, self.NQ, self.nts, self.nw_b,
+                                         C.byref(self._csym), _dptr(T_wt), _dptr(T_tw),
+                                         PRECISION[precision], SOLVER[solver.lower()], int(work_bytes)))
+        self._h = h
+
+    @property
+    def shape(self):
+        return (self.nts, self.ns, self.sym.ink, self.nao, self.nao)
+
+    def load_integrals(self, V_stored):
+        V = _c128(V_stored)
+        _check(lib().ggpu_gw_load_integrals(self._h, _dptr(V)))
+
+    def synth_integrals(self, seed, sigma=0.1):
+        _check(lib().ggpu_gw_synth_integrals(self._h, int(seed), float(sigma)))
+
+    def solve(self, G, Sigma=None):
+        """One GW iteration; returns (and fills, if given) Sigma(tau)."""
+        G = _c128(G, self.shape)
+        out = np.empty(self.shape, dtype=np.complex128) if Sigma is None else Sigma
+        if out.dtype != np.complex128 or not out.flags.c_contiguous or out.shape != self.shape:
+            raise ValueError("Sigma must be a C-contiguous complex128 array of shape %s" % (self.shape,))
+        _check(lib().ggpu_gw_iterate(self._h, _dptr(G), _dptr(out)))
+        return out
+
+    def solve_device(self, G_ptr, Sigma_ptr):
+        """Iteration on device-resident G / Sigma (raw device pointers)."""
+        _check(lib().ggpu_gw_iterate_device(self._h, C.c_void_p(G_ptr), C.c_void_p(Sigma_ptr)))
+
+    def polarization(self, G, iq, which):
+        G = _c128(G, self.shape)
+        n = {0: self.nts, 1: self.nw_b, 2: self.nts}[which]
+        out = np.empty((n, self.NQ, self.NQ), dtype=np.complex128)
+        _check(lib().ggpu_gw_polarization(self._h, _dptr(G), int(iq), int(which), _dptr(out)))
+        return out
+
+    def timings(self):
+        t = np.zeros(6)
+        _check(lib().ggpu_gw_last_timings(self._h, _dptr(t)))
+        return dict(zip(["total", "stage_G", "p0", "ir_dyson", "sigma", "stage_sigma"], t))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ggpu_gw_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()

@@ -1,0 +1,171 @@
+// extern "C" entry points: every call catches the library's exceptions and
+// turns them into a status code plus a thread-local message.
+#include "../../include/green_gpu_amd.h"
+#include "engine.hpp"
+
+#include <new>
+#include <string>
+
+struct ggpu_hf_plan { ggpu::HfPlan* impl; };
+struct ggpu_gw_plan { ggpu::GwPlan* impl; };
+
+namespace {
+thread_local std::string g_last_error;
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    g_last_error.clear();
+    return GGPU_OK;
+  } catch (const ggpu::Error& e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return GGPU_ERR_MEMORY;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return GGPU_ERR_ARGUMENT;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+const char* ggpu_last_error(void) { return g_last_error.c_str(); }
+
+int ggpu_device_count(int* count) {
+  return guarded([&] {
+    ggpu::require(count != nullptr, "NULL count");
+    int n = 0;
+    GGPU_HIP(hipGetDeviceCount(&n));
+    *count = n;
+  });
+}
+
+int ggpu_device_memory(size_t* free_bytes, size_t* total_bytes) {
+  return guarded([&] {
+    ggpu::require(free_bytes && total_bytes, "NULL pointer");
+    GGPU_HIP(hipMemGetInfo(free_bytes, total_bytes));
+  });
+}
+
+int ggpu_hf_plan_create(ggpu_hf_plan** plan, int nao, int ns, int NQ, const ggpu_symmetry* sym) {
+  return guarded([&] {
+    ggpu::require(plan && sym, "NULL plan or symmetry");
+    *plan = nullptr;
+    auto* p = new ggpu_hf_plan{nullptr};
+    try {
+      p->impl = new ggpu::HfPlan(nao, ns, NQ, *sym);
+    } catch (...) {
+      delete p;
+      throw;
+    }
+    *plan = p;
+  });
+}
+
+int ggpu_hf_load_integrals(ggpu_hf_plan* plan, const double* V_stored) {
+  return guarded([&] {
+    ggpu::require(plan && plan->impl, "NULL plan");
+    auto* p = plan->impl;
+    p->store.load_host(V_stored, p->sym.num_stored * (int64_t)p->NQ * p->nao * p->nao, p->stream);
+  });
+}
+
+int ggpu_hf_synth_integrals(ggpu_hf_plan* plan, uint64_t seed, double sigma) {
+  return guarded([&] {
+    ggpu::require(plan && plan->impl, "NULL plan");
+    auto* p = plan->impl;
+    p->store.synth(p->sym, p->NQ, p->nao, seed, sigma, p->stream);
+  });
+}
+
+int ggpu_hf_exchange(ggpu_hf_plan* plan, const double* dm, double* K_out) {
+  return guarded([&] {
+    ggpu::require(plan && plan->impl, "NULL plan");
+    plan->impl->exchange(dm, K_out);
+  });
+}
+
+int ggpu_hf_solve(ggpu_hf_plan* plan, const double* dm, const double* S_k, double madelung, double* F_out) {
+  return guarded([&] {
+    ggpu::require(plan && plan->impl, "NULL plan");
+    plan->impl->solve(dm, S_k, madelung, F_out);
+  });
+}
+
+void ggpu_hf_plan_destroy(ggpu_hf_plan* plan) {
+  if (!plan) return;
+  delete plan->impl;
+  delete plan;
+}
+
+int ggpu_gw_plan_create(ggpu_gw_plan** plan, int nao, int ns, int NQ, int nts, int nw_b, const ggpu_symmetry* sym,
+                        const double* T_wt, const double* T_tw, int precision, int solver, size_t work_bytes) {
+  return guarded([&] {
+    ggpu::require(plan && sym, "NULL plan or symmetry");
+    *plan = nullptr;
+    auto* p = new ggpu_gw_plan{nullptr};
+    try {
+      p->impl = new ggpu::GwPlan(nao, ns, NQ, nts, nw_b, *sym, T_wt, T_tw, precision, solver, work_bytes);
+    } catch (...) {
+      delete p;
+      throw;
+    }
+    *plan = p;
+  });
+}
+
+int ggpu_gw_load_integrals(ggpu_gw_plan* plan, const double* V_stored) {
+  return guarded([&] {
+    ggpu::require(plan && plan->impl, "NULL plan");
+    auto* p = plan->impl;
+    p->store.load_host(V_stored, p->sym.num_stored * (int64_t)p->NQ * p->nao * p->nao, p->stream);
+  });
+}
+
+int ggpu_gw_synth_integrals(ggpu_gw_plan* plan, uint64_t seed, double sigma) {
+  return guarded([&] {
+    ggpu::require(plan && plan->impl, "NULL plan");
+    auto* p = plan->impl;
+    p->store.synth(p->sym, p->NQ, p->nao, seed, sigma, p->stream);
+  });
+}
+
+int ggpu_gw_iterate(ggpu_gw_plan* plan, const double* G, double* Sigma_out) {
+  return guarded([&] {
+    ggpu::require(plan && plan->impl, "NULL plan");
+    plan->impl->iterate(G, Sigma_out);
+  });
+}
+
+int ggpu_gw_iterate_device(ggpu_gw_plan* plan, const void* G_device, void* Sigma_device) {
+  return guarded([&] {
+    ggpu::require(plan && plan->impl && G_device && Sigma_device, "NULL argument");
+    plan->impl->iterate_device(static_cast<const ggpu::z64*>(G_device), static_cast<ggpu::z64*>(Sigma_device));
+  });
+}
+
+int ggpu_gw_polarization(ggpu_gw_plan* plan, const double* G, int iq, int which, double* out) {
+  return guarded([&] {
+    ggpu::require(plan && plan->impl, "NULL plan");
+    plan->impl->polarization(G, iq, which, out);
+  });
+}
+
+int ggpu_gw_last_timings(ggpu_gw_plan* plan, double* ms6) {
+  return guarded([&] {
+    ggpu::require(plan && plan->impl && ms6, "NULL argument");
+    for (int i = 0; i < 6; ++i) ms6[i] = plan->impl->last_ms[i];
+  });
+}
+
+void ggpu_gw_plan_destroy(ggpu_gw_plan* plan) {
+  if (!plan) return;
+  delete plan->impl;
+  delete plan;
+}
+
+}  // extern "C"

@@ -1,0 +1,848 @@
+// Host orchestration of the HF exchange and GW iteration on one gfx950 GPU.
+//
+// Device data layout (HBM-resident for the whole plan / iteration):
+//   V store   (num_stored, NQ, nao, nao)   stored k-pairs, as the reference's integral reader holds them
+//   Gf        (nk, ns, nts, nao, nao)      G on the full BZ (rotated once per iteration)
+//   Sig_acc   (ink, ns, nts, nao, nao)     Sigma accumulator for the IBZ k
+//   P0        (nts/2, NQ, NQ)              bare bubble of the current q, t < nts/2
+//   P0w/Amat  (nw_b, NQ, NQ)               P0(Omega) -> right-hand side / (1 - P0) factor
+//   Pt        (nts, NQ, NQ)                dressed polarization of the current q
+//   work      pair-batch intermediates (X1, X2 for P0; Y1, Y2 for Sigma)
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace ggpu {
+
+// Launch a GEMM with every operand tied to its allocation, so that the
+// host-side bounds check covers the whole launch.
+template <class BA, class BB, class BC>
+static void gemm_on(GemmDesc g, const BA& a, const BB& b, const BC& c, hipStream_t s) {
+  g.A.buf = a.get(); g.A.buf_elems = (int64_t)a.size();
+  g.B.buf = b.get(); g.B.buf_elems = (int64_t)b.size();
+  g.C.buf = c.get(); g.C.buf_elems = (int64_t)c.size();
+  gemm(g, s);
+}
+
+// View of caller memory with a known element count (device pointers handed in).
+struct Span {
+  const void* p;
+  size_t n;
+  const void* get() const { return p; }
+  size_t size() const { return n; }
+};
+
+// ---------------------------------------------------------------------------
+// Symmetry tables
+// ---------------------------------------------------------------------------
+static bool is_identity(const double* U, int64_t count, int n) {
+  for (int64_t m = 0; m < count; ++m)
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) {
+        const double* z = U + 2 * ((m * n + i) * (int64_t)n + j);
+        if (z[0] != (i == j ? 1.0 : 0.0) || z[1] != 0.0) return false;
+      }
+  return true;
+}
+
+static std::vector<int64_t> copy_idx(const int64_t* p, int64_t n, const char* name) {
+  require(p != nullptr, std::string("symmetry: ") + name + " is NULL");
+  return std::vector<int64_t>(p, p + n);
+}
+
+static void check_range(const std::vector<int64_t>& v, int64_t lo, int64_t hi, const char* name) {
+  for (auto x : v) require(x >= lo && x < hi, std::string("symmetry: ") + name + " entry out of range");
+}
+
+void SymTables::load(const ggpu_symmetry& s, int nao, int NQ) {
+  require(s.nk > 0 && s.ink > 0 && s.nq > 0 && s.inq > 0 && s.ink <= s.nk && s.inq <= s.nq,
+          "symmetry: bad k/q counts");
+  require(s.nq == s.nk, "symmetry: the q mesh must equal the k mesh");
+  nk = s.nk; ink = s.ink; nq = s.nq; inq = s.inq;
+  k_f2r = copy_idx(s.k_full_to_reduced, nk, "k_full_to_reduced");
+  k_r2f = copy_idx(s.k_reduced_to_full, ink, "k_reduced_to_full");
+  k_tr = copy_idx(s.k_tr_conj, nk, "k_tr_conj");
+  q_f2r = copy_idx(s.q_full_to_reduced, nq, "q_full_to_reduced");
+  q_r2f = copy_idx(s.q_reduced_to_full, inq, "q_reduced_to_full");
+  q_tr = copy_idx(s.q_tr_conj, nq, "q_tr_conj");
+  k1_from_k2q = copy_idx(s.k1_from_k2q, (int64_t)nk * nq, "k1_from_k2q");
+  k2_from_k1q = copy_idx(s.k2_from_k1q, (int64_t)nk * nq, "k2_from_k1q");
+  check_range(k_f2r, 0, ink, "k_full_to_reduced");
+  check_range(k_r2f, 0, nk, "k_reduced_to_full");
+  check_range(q_f2r, 0, inq, "q_full_to_reduced");
+  check_range(q_r2f, 0, nq, "q_reduced_to_full");
+  check_range(k1_from_k2q, 0, nk, "k1_from_k2q");
+  check_range(k2_from_k1q, 0, nk, "k2_from_k1q");
+  auto stars = [&](const int64_t* off, const int64_t* mem, int n, int64_t full, const char* nm) {
+    require(off && mem, std::string("symmetry: ") + nm + " star arrays are NULL");
+    std::vector<std::vector<int64_t>> out(n);
+    require(off[0] == 0 && off[n] == full, std::string("symmetry: ") + nm + " stars must cover the mesh");
+    for (int i = 0; i < n; ++i) {
+      require(off[i + 1] > off[i], std::string("symmetry: empty ") + nm + " star");
+      out[i].assign(mem + off[i], mem + off[i + 1]);
+      check_range(out[i], 0, full, nm);
+    }
+    return out;
+  };
+  k_stars = stars(s.k_star_offsets, s.k_star_members, ink, nk, "k");
+  q_stars = stars(s.q_star_offsets, s.q_star_members, inq, nq, "q");
+  U_k.clear();
+  U_q.clear();
+  if (s.U_k && !is_identity(s.U_k, nk, nao)) {
+    U_k.resize((size_t)nk * nao * nao);
+    std::memcpy(U_k.data(), s.U_k, U_k.size() * sizeof(z64));
+  }
+  if (s.U_q && !is_identity(s.U_q, nq, NQ)) {
+    U_q.resize((size_t)nq * NQ * NQ);
+    std::memcpy(U_q.data(), s.U_q, U_q.size() * sizeof(z64));
+  }
+  const int64_t npair = (int64_t)nk * (nk + 1) / 2;
+  if (s.conj_kpair || s.trans_kpair || s.irre_pos) {
+    conj_kpair = copy_idx(s.conj_kpair, npair, "conj_kpair");
+    trans_kpair = copy_idx(s.trans_kpair, npair, "trans_kpair");
+    irre_pos = copy_idx(s.irre_pos, npair, "irre_pos");
+    num_stored = s.num_kpair_stored;
+    check_range(conj_kpair, 0, npair, "conj_kpair");
+    check_range(trans_kpair, 0, npair, "trans_kpair");
+    check_range(irre_pos, -1, num_stored, "irre_pos");
+  } else {
+    conj_kpair.resize(npair);
+    trans_kpair.resize(npair);
+    irre_pos.resize(npair);
+    for (int64_t i = 0; i < npair; ++i) conj_kpair[i] = trans_kpair[i] = irre_pos[i] = i;
+    num_stored = npair;
+    require(s.num_kpair_stored == 0 || s.num_kpair_stored == npair,
+            "symmetry: num_kpair_stored must be nk(nk+1)/2 without pair tables");
+  }
+  require(num_stored > 0, "symmetry: no stored pairs");
+}
+
+PairView SymTables::view(int64_t k1, int64_t k2) const {
+  int64_t idx = (k1 >= k2) ? k1 * (k1 + 1) / 2 + k2 : k2 * (k2 + 1) / 2 + k1;
+  const bool neg = k1 < k2;
+  int kind = 0;  // 0 direct, 1 conjugated, 2 transposed
+  if (conj_kpair[idx] != idx) { kind = 1; idx = conj_kpair[idx]; }
+  else if (trans_kpair[idx] != idx) { kind = 2; idx = trans_kpair[idx]; }
+  const int64_t pos = irre_pos[idx];
+  require(pos >= 0 && pos < num_stored, "integral pair is not stored");
+  PairView v;
+  v.pos = pos;
+  v.trans = (neg ? 1 : 0) ^ (kind == 2 ? 1 : 0);
+  v.conj = (neg ? 1 : 0) ^ (kind == 1 ? 1 : 0);
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// Integral store and pair unfolding
+// ---------------------------------------------------------------------------
+void IntegralStore::load_host(const double* V_stored, int64_t n_elems, hipStream_t s) {
+  require(V_stored != nullptr, "integrals: NULL pointer");
+  V.resize(n_elems);
+  GGPU_HIP(hipMemcpyAsync(V.get(), V_stored, n_elems * sizeof(z64), hipMemcpyHostToDevice, s));
+  GGPU_HIP(hipStreamSynchronize(s));
+  loaded = true;
+}
+
+void IntegralStore::synth(const SymTables& t, int NQ, int nao, uint64_t seed, double sigma, hipStream_t s) {
+  // diagonal flag per stored position: find the pair index that maps to it
+  std::vector<int> diag(t.num_stored, 0);
+  for (int64_t k1 = 0; k1 < t.nk; ++k1) {
+    const int64_t idx = k1 * (k1 + 1) / 2 + k1;
+    if (t.conj_kpair[idx] == idx && t.trans_kpair[idx] == idx && t.irre_pos[idx] >= 0) diag[t.irre_pos[idx]] = 1;
+  }
+  DeviceBuffer<int> d(diag.size());
+  GGPU_HIP(hipMemcpy(d.get(), diag.data(), diag.size() * sizeof(int), hipMemcpyHostToDevice));
+  V.resize((size_t)t.num_stored * NQ * nao * nao);
+  synth_integrals(V.get(), 0, t.num_stored, d.get(), NQ, nao, seed, sigma, s);
+  GGPU_HIP(hipStreamSynchronize(s));
+  loaded = true;
+}
+
+void PairTable::build(const std::vector<PairView>& views, int64_t stride, hipStream_t s) {
+  (void)s;
+  host = views;
+  std::vector<int64_t> off(views.size());
+  std::vector<int> tr(views.size()), cj(views.size());
+  for (size_t i = 0; i < views.size(); ++i) {
+    off[i] = views[i].pos * stride;
+    tr[i] = views[i].trans;
+    cj[i] = views[i].conj;
+  }
+  in_off.resize(off.size());
+  trans.resize(tr.size());
+  conj.resize(cj.size());
+  GGPU_HIP(hipMemcpy(in_off.get(), off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+  GGPU_HIP(hipMemcpy(trans.get(), tr.data(), tr.size() * sizeof(int), hipMemcpyHostToDevice));
+  GGPU_HIP(hipMemcpy(conj.get(), cj.data(), cj.size() * sizeof(int), hipMemcpyHostToDevice));
+}
+
+void unfold_pairs(const IntegralStore& st, const PairTable& tab, int i0, int n, const int64_t* out_off_dev,
+                  z64* dst, int NQ, int nao, hipStream_t s) {
+  require(i0 >= 0 && i0 + n <= (int)tab.host.size(), "unfold_pairs: range");
+  for (int i = i0; i < i0 + n; ++i)
+    require((tab.host[i].pos + 1) * (int64_t)NQ * nao * nao <= (int64_t)st.V.size(), "unfold_pairs: pair outside store");
+  BlockCopy c;
+  c.in = st.V.get();
+  c.out = dst;
+  c.rows = nao;
+  c.cols = nao;
+  c.in_ld = nao;
+  c.out_ld = nao;
+  c.n_inner = NQ;
+  c.in_bs = (int64_t)nao * nao;
+  c.out_bs = (int64_t)nao * nao;
+  c.n_outer = n;
+  c.in_off = tab.in_off.get() + i0;
+  c.out_off = out_off_dev;
+  c.trans = tab.trans.get() + i0;
+  c.conj = tab.conj.get() + i0;
+  block_copy(c, s);
+}
+
+// ---------------------------------------------------------------------------
+// IBZ -> full BZ rotation
+// ---------------------------------------------------------------------------
+void FullBZRotator::init(const SymTables& t, int n, int ns, int nt, hipStream_t s) {
+  (void)s;
+  t_ = &t;
+  n_ = n; ns_ = ns; nt_ = nt;
+  bool any_tr = false;
+  for (auto x : t.k_tr) any_tr |= (x != 0);
+  const int64_t n2 = (int64_t)n * n;
+  if (t.k_identity() && !any_tr) {
+    std::vector<int64_t> src((size_t)t.nk * ns * nt), dst(src.size());
+    size_t o = 0;
+    for (int k = 0; k < t.nk; ++k)
+      for (int sp = 0; sp < ns; ++sp)
+        for (int tt = 0; tt < nt; ++tt, ++o) {
+          src[o] = (((int64_t)tt * ns + sp) * t.ink + t.k_f2r[k]) * n2;
+          dst[o] = (((int64_t)k * ns + sp) * nt + tt) * n2;
+        }
+    src_.resize(src.size());
+    dst_.resize(dst.size());
+    GGPU_HIP(hipMemcpy(src_.get(), src.data(), src.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    GGPU_HIP(hipMemcpy(dst_.get(), dst.data(), dst.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+  } else {
+    std::vector<z64> U((size_t)t.nk * n2);
+    if (t.k_identity()) {
+      for (int k = 0; k < t.nk; ++k)
+        for (int i = 0; i < n; ++i)
+          for (int j = 0; j < n; ++j) U[k * n2 + i * n + j] = z64{i == j ? 1.0 : 0.0, 0.0};
+    } else {
+      U = t.U_k;
+    }
+    Uk_.resize(U.size());
+    GGPU_HIP(hipMemcpy(Uk_.get(), U.data(), U.size() * sizeof(z64), hipMemcpyHostToDevice));
+    tmp_.resize((size_t)ns * nt * n2);
+  }
+}
+
+void FullBZRotator::run(const z64* X, z64* out, hipStream_t s) {
+  const SymTables& t = *t_;
+  const int64_t n2 = (int64_t)n_ * n_;
+  const size_t in_elems = (size_t)nt_ * ns_ * t.ink * n2, out_elems = (size_t)t.nk * ns_ * nt_ * n2;
+  if (src_.size()) {
+    slab_gather(X, src_.get(), out, dst_.get(), (int)src_.size(), n2, 1.0, s);
+    return;
+  }
+  for (int k = 0; k < t.nk; ++k) {
+    const int tr = t.k_tr[k] != 0;
+    GemmDesc g;
+    g.M = n_; g.N = n_; g.K = n_; g.nb0 = nt_; g.nb1 = ns_;
+    g.A.ptr = Uk_.get() + k * n2; g.A.rs = n_; g.A.cs = 1; g.A.conj = tr;
+    g.B.ptr = X + t.k_f2r[k] * n2; g.B.rs = n_; g.B.cs = 1;
+    g.B.bs0 = (int64_t)ns_ * t.ink * n2; g.B.bs1 = (int64_t)t.ink * n2; g.B.conj = tr;
+    g.C.ptr = tmp_.get(); g.C.rs = n_; g.C.cs = 1; g.C.bs0 = n2; g.C.bs1 = (int64_t)nt_ * n2;
+    gemm_on(g, Uk_, Span{X, in_elems}, tmp_, s);
+    GemmDesc h;
+    h.M = n_; h.N = n_; h.K = n_; h.nb0 = nt_; h.nb1 = ns_;
+    h.A.ptr = tmp_.get(); h.A.rs = n_; h.A.cs = 1; h.A.bs0 = n2; h.A.bs1 = (int64_t)nt_ * n2;
+    h.B.ptr = Uk_.get() + k * n2; h.B.rs = 1; h.B.cs = n_; h.B.conj = !tr;
+    h.C.ptr = out + (int64_t)k * ns_ * nt_ * n2; h.C.rs = n_; h.C.cs = 1; h.C.bs0 = n2; h.C.bs1 = (int64_t)nt_ * n2;
+    gemm_on(h, tmp_, Uk_, Span{out, out_elems}, s);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// GW plan
+// ---------------------------------------------------------------------------
+GwPlan::GwPlan(int nao_, int ns_, int NQ_, int nts_, int nw_b_, const ggpu_symmetry& s, const double* T_wt,
+               const double* T_tw_h, int precision, int solver_, size_t work_bytes)
+    : nao(nao_), ns(ns_), NQ(NQ_), nts(nts_), nw_b(nw_b_), half(nts_ / 2), solver(solver_) {
+  require(nao > 0 && NQ > 0, "gw: nao and NQ must be positive");
+  require(ns == 1 || ns == 2, "gw: ns must be 1 or 2");
+  require(nts % 2 == 0, "gw: nts must be even");
+  require(nts >= 4, "gw: nts must be at least 4");
+  require(nts >= nw_b, "gw: Nt is not greater than or equal to Nw_b");
+  require(nw_b > 0, "gw: nw_b must be positive");
+  require(solver == GGPU_SOLVER_CHOLESKY || solver == GGPU_SOLVER_LU, "gw: unknown linear solver");
+  require(precision == GGPU_PRECISION_FP64, "gw: only the FP64 precision is available in this build",
+          kErrArgument);
+  require(T_wt && T_tw_h, "gw: NULL transform matrices");
+  compute = Compute::F64;
+  sym.load(s, nao, NQ);
+  GGPU_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  for (auto& e : ev) GGPU_HIP(hipEventCreate(&e));
+
+  // tau -> Omega with the mirror P0(beta - tau) = P0(tau) folded into the
+  // matrix (the reference copies the mirrored slices, cugw_qpt.cu:184-191)
+  const int ni = nts - 2;
+  std::vector<z64> tf((size_t)nw_b * (half - 1));
+  const z64* Tw = reinterpret_cast<const z64*>(T_wt);
+  for (int w = 0; w < nw_b; ++w)
+    for (int t = 1; t < half; ++t) {
+      const z64 a = Tw[(int64_t)w * ni + (t - 1)], b = Tw[(int64_t)w * ni + (nts - 2 - t)];
+      tf[(size_t)w * (half - 1) + (t - 1)] = z64{a.x + b.x, a.y + b.y};
+    }
+  Tf_wt.resize(tf.size());
+  GGPU_HIP(hipMemcpy(Tf_wt.get(), tf.data(), tf.size() * sizeof(z64), hipMemcpyHostToDevice));
+  T_tw.resize((size_t)nts * nw_b);
+  GGPU_HIP(hipMemcpy(T_tw.get(), T_tw_h, T_tw.bytes(), hipMemcpyHostToDevice));
+
+  const int64_t n2 = (int64_t)nao * nao, Q2 = (int64_t)NQ * NQ;
+  G_host_dev.resize((size_t)nts * ns * sym.ink * n2);
+  Gf.resize((size_t)sym.nk * ns * nts * n2);
+  Sig_acc.resize((size_t)sym.ink * ns * nts * n2);
+  Sig_host_dev.resize(Sig_acc.size());
+  P0.resize((size_t)half * Q2);
+  P0w.resize((size_t)nw_b * Q2);
+  Amat.resize((size_t)nw_b * Q2);
+  Pt.resize((size_t)nts * Q2);
+  if (!sym.q_identity()) {
+    Prot.resize((size_t)nts * Q2);
+    Ptmp.resize((size_t)nts * Q2);
+    Uq_dev.resize(sym.U_q.size());
+    GGPU_HIP(hipMemcpy(Uq_dev.get(), sym.U_q.data(), Uq_dev.bytes(), hipMemcpyHostToDevice));
+  }
+  info.resize(nw_b);
+
+  // pair tables
+  const int64_t stride = (int64_t)NQ * n2;
+  p0_tables.resize(sym.inq);
+  sig_tables.resize(sym.inq);
+  for (int iq = 0; iq < sym.inq; ++iq) {
+    const int64_t q = sym.q_r2f[iq];
+    std::vector<PairView> v;
+    for (int ik = 0; ik < sym.ink; ++ik)
+      for (auto k : sym.k_stars[ik]) v.push_back(sym.view(k, sym.k1_from_k2q[k * sym.nq + q]));
+    p0_tables[iq].build(v, stride, stream);
+    sig_tables[iq].resize(sym.q_stars[iq].size());
+    for (size_t j = 0; j < sym.q_stars[iq].size(); ++j) {
+      const int64_t qd = sym.q_stars[iq][j];
+      std::vector<PairView> u;
+      for (int ik = 0; ik < sym.ink; ++ik) {
+        const int64_t k = sym.k_r2f[ik];
+        u.push_back(sym.view(k, sym.k2_from_k1q[k * sym.nq + qd]));
+      }
+      sig_tables[iq][j].build(u, stride, stream);
+    }
+  }
+
+  // workspace for pair batches
+  const int64_t per_pair_p0 = (2 + 2LL * ns * half) * stride;
+  const int64_t per_pair_sig = (1 + 2LL * ns * nts) * stride;
+  const int64_t per_pair = std::max(per_pair_p0, per_pair_sig) * (int64_t)sizeof(z64);
+  size_t budget = work_bytes;
+  if (budget == 0) {
+    size_t fr = 0, tot = 0;
+    GGPU_HIP(hipMemGetInfo(&fr, &tot));
+    budget = std::min<size_t>((size_t)(0.6 * fr), (size_t)48 << 30);
+  }
+  kb = (int)std::max<int64_t>(1, std::min<int64_t>(budget / per_pair, std::max(sym.nk, sym.ink)));
+  kb = std::min(kb, 4096);
+  work.resize((size_t)kb * (per_pair / sizeof(z64)));
+  std::vector<int64_t> bo(kb);
+  for (int b = 0; b < kb; ++b) bo[b] = b * stride;
+  bidx_off.resize(kb);
+  GGPU_HIP(hipMemcpy(bidx_off.get(), bo.data(), bo.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+
+  // Sigma (ik, s, t) -> host layout (t, s, ik)
+  std::vector<int64_t> src((size_t)sym.ink * ns * nts), dst(src.size());
+  size_t o = 0;
+  for (int ik = 0; ik < sym.ink; ++ik)
+    for (int sp = 0; sp < ns; ++sp)
+      for (int t = 0; t < nts; ++t, ++o) {
+        src[o] = (((int64_t)ik * ns + sp) * nts + t) * n2;
+        dst[o] = (((int64_t)t * ns + sp) * sym.ink + ik) * n2;
+      }
+  sig_src.resize(src.size());
+  sig_dst.resize(dst.size());
+  GGPU_HIP(hipMemcpy(sig_src.get(), src.data(), src.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+  GGPU_HIP(hipMemcpy(sig_dst.get(), dst.data(), dst.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+
+  rot.init(sym, nao, ns, nts, stream);
+}
+
+GwPlan::~GwPlan() {
+  if (stream) (void)hipStreamSynchronize(stream);
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+void GwPlan::stage_G(const z64* G_dev) { rot.run(G_dev, Gf.get(), stream); }
+
+// P0(q, tau_t)[Q'][Q] += pref * Tr[G(k, beta - tau_t) V^{Q'} G(k+q, tau_t) V^{Q dagger}]
+// for t < nts/2 (cugw_qkpt.cu:202-246), as three GEMMs per pair batch:
+//   X1[t][n][m][Q'] = sum_p G(k, nts-1-t)[n][p] V^{Q'}[p][m]
+//   X2[t][Q][p][i]  = sum_m G(k+q, t)[i][m] conj(V^Q[p][m])
+//   P0[t][Q'][Q]   += pref * sum_{b, s, (n, m)} X1[t][(n,m)][Q'] X2[t][Q][(n,m)]
+void GwPlan::p0_for_q(int iq) {
+  const int64_t n2 = (int64_t)nao * nao, stride = (int64_t)NQ * n2;
+  const PairTable& tab = p0_tables[iq];
+  const int64_t q = sym.q_r2f[iq];
+  const int npairs = (int)tab.host.size();
+  const double pref = (ns == 1) ? -2.0 : -1.0;
+  zero(P0.get(), (int64_t)half * NQ * NQ, stream);
+  // full-BZ k of every pair, in table order
+  std::vector<int64_t> ks;
+  for (int ik = 0; ik < sym.ink; ++ik)
+    for (auto k : sym.k_stars[ik]) ks.push_back(k);
+  z64* Vdir = work.get();
+  z64* VpmQ = Vdir + kb * stride;
+  z64* X1 = VpmQ + kb * stride;
+  z64* X2 = X1 + (int64_t)kb * ns * half * stride;
+  const int64_t xs = (int64_t)half * stride;  // one (pair, spin) slab of X1 / X2
+  for (int i0 = 0; i0 < npairs; i0 += kb) {
+    const int n = std::min(kb, npairs - i0);
+    unfold_pairs(store, tab, i0, n, bidx_off.get(), Vdir, NQ, nao, stream);
+    BlockCopy tr;
+    tr.in = Vdir; tr.out = VpmQ; tr.rows = (int)n2; tr.cols = NQ; tr.in_ld = n2; tr.out_ld = NQ;
+    tr.n_inner = n; tr.in_bs = stride; tr.out_bs = stride; tr.trans_all = 1;
+    block_copy(tr, stream);
+    for (int b = 0; b < n; ++b) {
+      const int64_t k = ks[i0 + b], k1 = sym.k1_from_k2q[k * sym.nq + q];
+      for (int s = 0; s < ns; ++s) {
+        GemmDesc g;
+        g.M = nao; g.N = nao * NQ; g.K = nao; g.nb0 = half;
+        g.A.ptr = Gf.get() + ((k * ns + s) * nts + (nts - 1)) * n2; g.A.rs = nao; g.A.cs = 1; g.A.bs0 = -n2;
+        g.B.ptr = VpmQ + b * stride; g.B.rs = (int64_t)nao * NQ; g.B.cs = 1;
+        g.C.ptr = X1 + (b * ns + s) * xs; g.C.rs = (int64_t)nao * NQ; g.C.cs = 1; g.C.bs0 = stride;
+        gemm_on(g, Gf, work, work, stream);
+        GemmDesc h;
+        h.M = NQ * nao; h.N = nao; h.K = nao; h.nb0 = half;
+        h.A.ptr = Vdir + b * stride; h.A.rs = nao; h.A.cs = 1; h.A.conj = 1;
+        h.B.ptr = Gf.get() + (k1 * ns + s) * nts * n2; h.B.rs = 1; h.B.cs = nao; h.B.bs0 = n2;
+        h.C.ptr = X2 + (b * ns + s) * xs; h.C.rs = nao; h.C.cs = 1; h.C.bs0 = stride;
+        gemm_on(h, work, Gf, work, stream);
+      }
+    }
+    GemmDesc p;
+    p.M = NQ; p.N = NQ; p.K = (int)n2; p.KB = n * ns; p.nb0 = half;
+    p.A.ptr = X1; p.A.rs = 1; p.A.cs = NQ; p.A.kbs = xs; p.A.bs0 = stride;
+    p.B.ptr = X2; p.B.rs = 1; p.B.cs = n2; p.B.kbs = xs; p.B.bs0 = stride;
+    p.C.ptr = P0.get(); p.C.rs = NQ; p.C.cs = 1; p.C.bs0 = (int64_t)NQ * NQ;
+    p.alpha_re = pref; p.beta_re = 1.0;
+    gemm_on(p, work, work, P0, stream);
+  }
+}
+
+// tau -> Omega, Dyson solve, Omega -> tau for the current q (cugw_qpt.cu:174-358)
+void GwPlan::dyson_for_q() {
+  const int64_t Q2 = (int64_t)NQ * NQ;
+  const int n = NQ;
+  {
+    GemmDesc g;
+    g.M = nw_b; g.N = (int)Q2; g.K = half - 1;
+    g.A.ptr = Tf_wt.get(); g.A.rs = half - 1; g.A.cs = 1;
+    g.B.ptr = P0.get() + Q2; g.B.rs = Q2; g.B.cs = 1;
+    g.C.ptr = P0w.get(); g.C.rs = Q2; g.C.cs = 1;
+    g.alpha_re = 1.0 / sym.nk;
+    gemm_on(g, Tf_wt, P0, P0w, stream);
+  }
+  herm_one_minus(P0w.get(), Amat.get(), n, nw_b, stream);
+  // blocked Cholesky of (1 - P0), right-looking
+  const int nb = 32;
+  for (int j0 = 0; j0 < n; j0 += nb) {
+    const int jb = std::min(nb, n - j0);
+    chol_diag_block(Amat.get(), n, nw_b, j0, jb, info.get(), stream);
+    chol_panel(Amat.get(), n, nw_b, j0, jb, stream);
+    const int rem = n - j0 - jb;
+    if (rem > 0) {
+      GemmDesc h;
+      h.M = rem; h.N = rem; h.K = jb; h.nb0 = nw_b;
+      h.A.ptr = Amat.get() + (int64_t)(j0 + jb) * n + j0; h.A.rs = n; h.A.cs = 1; h.A.bs0 = Q2;
+      h.B.ptr = Amat.get() + (int64_t)(j0 + jb) * n + j0; h.B.rs = 1; h.B.cs = n; h.B.bs0 = Q2; h.B.conj = 1;
+      h.C.ptr = Amat.get() + (int64_t)(j0 + jb) * n + (j0 + jb); h.C.rs = n; h.C.cs = 1; h.C.bs0 = Q2;
+      h.alpha_re = -1.0; h.beta_re = 1.0;
+      gemm_on(h, Amat, Amat, Amat, stream);
+    }
+  }
+  // solve L Y = P0 (forward), L^dagger X = Y (backward); X overwrites P0w
+  for (int i0 = 0; i0 < n; i0 += nb) {
+    const int ib = std::min(nb, n - i0);
+    if (i0 > 0) {
+      GemmDesc g;
+      g.M = ib; g.N = n; g.K = i0; g.nb0 = nw_b;
+      g.A.ptr = Amat.get() + (int64_t)i0 * n; g.A.rs = n; g.A.cs = 1; g.A.bs0 = Q2;
+      g.B.ptr = P0w.get(); g.B.rs = n; g.B.cs = 1; g.B.bs0 = Q2;
+      g.C.ptr = P0w.get() + (int64_t)i0 * n; g.C.rs = n; g.C.cs = 1; g.C.bs0 = Q2;
+      g.alpha_re = -1.0; g.beta_re = 1.0;
+      gemm_on(g, Amat, P0w, P0w, stream);
+    }
+    trsm_lower_block(Amat.get(), P0w.get(), n, n, nw_b, i0, ib, stream);
+  }
+  const int nblk = (n + nb - 1) / nb;
+  for (int bi = nblk - 1; bi >= 0; --bi) {
+    const int i0 = bi * nb, ib = std::min(nb, n - i0);
+    const int rest = n - i0 - ib;
+    if (rest > 0) {
+      GemmDesc g;
+      g.M = ib; g.N = n; g.K = rest; g.nb0 = nw_b;
+      g.A.ptr = Amat.get() + (int64_t)(i0 + ib) * n + i0; g.A.rs = 1; g.A.cs = n; g.A.bs0 = Q2; g.A.conj = 1;
+      g.B.ptr = P0w.get() + (int64_t)(i0 + ib) * n; g.B.rs = n; g.B.cs = 1; g.B.bs0 = Q2;
+      g.C.ptr = P0w.get() + (int64_t)i0 * n; g.C.rs = n; g.C.cs = 1; g.C.bs0 = Q2;
+      g.alpha_re = -1.0; g.beta_re = 1.0;
+      gemm_on(g, Amat, P0w, P0w, stream);
+    }
+    trsm_upper_h_block(Amat.get(), P0w.get(), n, n, nw_b, i0, ib, stream);
+  }
+  if (solver == GGPU_SOLVER_LU) herm_one_minus(P0w.get(), nullptr, n, nw_b, stream);
+  {
+    GemmDesc g;
+    g.M = nts; g.N = (int)Q2; g.K = nw_b;
+    g.A.ptr = T_tw.get(); g.A.rs = nw_b; g.A.cs = 1;
+    g.B.ptr = P0w.get(); g.B.rs = Q2; g.B.cs = 1;
+    g.C.ptr = Pt.get(); g.C.rs = Q2; g.C.cs = 1;
+    gemm_on(g, T_tw, P0w, Pt, stream);
+  }
+}
+
+// Sigma(k, tau) -= sum_{Q,Q'} P~(q, tau)[Q'][Q] V^{Q'}(k,k-q) G(k-q, tau) V^{Q dagger}(k,k-q)
+// for every q in the star of the IBZ q and every IBZ k (cugw_qkpt.cu:248-338):
+//   Y1[t][Q'][i][n] = sum_m V^{Q'}[i][m] G(k-q, t)[m][n]
+//   Y2[Q][t][i][n]  = sum_Q' P~[t][Q'][Q] Y1[t][Q'][i][n]
+//   Sigma[t][i][j] -= sum_{Q,m} Y2[Q][t][i][m] conj(V^Q[j][m])
+void GwPlan::sigma_for_q(int iq) {
+  const int64_t n2 = (int64_t)nao * nao, stride = (int64_t)NQ * n2, Q2 = (int64_t)NQ * NQ;
+  z64* Vdir = work.get();
+  z64* Y1 = Vdir + kb * stride;
+  z64* Y2 = Y1 + (int64_t)kb * ns * nts * stride;
+  const int64_t ys = (int64_t)nts * stride;   // one (pair, spin) slab of Y1 / Y2
+  for (size_t j = 0; j < sym.q_stars[iq].size(); ++j) {
+    const int64_t qd = sym.q_stars[iq][j];
+    const int tr = sym.q_tr[qd] != 0;
+    const z64* Pq = Pt.get();
+    if (!sym.q_identity()) {
+      // P~ = U_q P U_q^dagger (conjugated at use for a time-reversal image)
+      GemmDesc a;
+      a.M = NQ; a.N = NQ; a.K = NQ; a.nb0 = nts;
+      a.A.ptr = Uq_dev.get() + qd * Q2; a.A.rs = NQ; a.A.cs = 1;
+      a.B.ptr = Pt.get(); a.B.rs = NQ; a.B.cs = 1; a.B.bs0 = Q2;
+      a.C.ptr = Ptmp.get(); a.C.rs = NQ; a.C.cs = 1; a.C.bs0 = Q2;
+      gemm_on(a, Uq_dev, Pt, Ptmp, stream);
+      GemmDesc c;
+      c.M = NQ; c.N = NQ; c.K = NQ; c.nb0 = nts;
+      c.A.ptr = Ptmp.get(); c.A.rs = NQ; c.A.cs = 1; c.A.bs0 = Q2;
+      c.B.ptr = Uq_dev.get() + qd * Q2; c.B.rs = 1; c.B.cs = NQ; c.B.conj = 1;
+      c.C.ptr = Prot.get(); c.C.rs = NQ; c.C.cs = 1; c.C.bs0 = Q2;
+      gemm_on(c, Ptmp, Uq_dev, Prot, stream);
+      Pq = Prot.get();
+    }
+    const DeviceBuffer<z64>& Pbuf = sym.q_identity() ? Pt : Prot;
+    const PairTable& tab = sig_tables[iq][j];
+    for (int i0 = 0; i0 < sym.ink; i0 += kb) {
+      const int n = std::min(kb, sym.ink - i0);
+      unfold_pairs(store, tab, i0, n, bidx_off.get(), Vdir, NQ, nao, stream);
+      for (int b = 0; b < n; ++b) {
+        const int64_t k = sym.k_r2f[i0 + b];
+        const int64_t k1 = sym.k2_from_k1q[k * sym.nq + qd];
+        for (int s = 0; s < ns; ++s) {
+          GemmDesc g;
+          g.M = NQ * nao; g.N = nao; g.K = nao; g.nb0 = nts;
+          g.A.ptr = Vdir + b * stride; g.A.rs = nao; g.A.cs = 1;
+          g.B.ptr = Gf.get() + (k1 * ns + s) * nts * n2; g.B.rs = nao; g.B.cs = 1; g.B.bs0 = n2;
+          g.C.ptr = Y1 + (b * ns + s) * ys; g.C.rs = nao; g.C.cs = 1; g.C.bs0 = stride;
+          gemm_on(g, work, Gf, work, stream);
+          GemmDesc h;
+          h.M = NQ; h.N = (int)n2; h.K = NQ; h.nb0 = nts;
+          h.A.ptr = Pq; h.A.rs = 1; h.A.cs = NQ; h.A.bs0 = Q2; h.A.conj = tr;
+          h.B.ptr = Y1 + (b * ns + s) * ys; h.B.rs = n2; h.B.cs = 1; h.B.bs0 = stride;
+          h.C.ptr = Y2 + (b * ns + s) * ys; h.C.rs = (int64_t)nts * n2; h.C.cs = 1; h.C.bs0 = n2;
+          gemm_on(h, Pbuf, work, work, stream);
+        }
+      }
+      GemmDesc f;
+      f.M = nts * nao; f.N = nao; f.K = nao; f.KB = NQ; f.nb0 = n; f.nb1 = ns;
+      f.A.ptr = Y2; f.A.rs = nao; f.A.cs = 1; f.A.kbs = (int64_t)nts * n2; f.A.bs0 = ns * ys; f.A.bs1 = ys;
+      f.B.ptr = Vdir; f.B.rs = 1; f.B.cs = nao; f.B.kbs = n2; f.B.bs0 = stride; f.B.bs1 = 0; f.B.conj = 1;
+      f.C.ptr = Sig_acc.get() + (int64_t)i0 * ns * nts * n2; f.C.rs = nao; f.C.cs = 1;
+      f.C.bs0 = (int64_t)ns * nts * n2; f.C.bs1 = (int64_t)nts * n2;
+      f.alpha_re = -1.0; f.beta_re = 1.0;
+      gemm_on(f, work, work, Sig_acc, stream);
+    }
+  }
+}
+
+void GwPlan::check_info() {
+  std::vector<int> h(nw_b);
+  GGPU_HIP(hipMemcpy(h.data(), info.get(), nw_b * sizeof(int), hipMemcpyDeviceToHost));
+  for (int w = 0; w < nw_b; ++w)
+    if (h[w] != 0)
+      throw Error(kErrNotPositiveDefinite, "gw: 1 - P0(Omega) is not positive definite (Omega index " +
+                                               std::to_string(w) + ", column " + std::to_string(h[w]) + ")");
+}
+
+void GwPlan::iterate_device(const z64* G_dev, z64* S_dev) {
+  require(store.loaded, "gw: integrals not loaded", kErrState);
+  const int64_t n2 = (int64_t)nao * nao;
+  std::vector<hipEvent_t> evs;
+  GGPU_HIP(hipEventRecord(ev[0], stream));
+  stage_G(G_dev);
+  GGPU_HIP(hipEventRecord(ev[1], stream));
+  zero(Sig_acc.get(), Sig_acc.size(), stream);
+  zero_int(info.get(), nw_b, stream);
+  double t_p0 = 0, t_dy = 0, t_sig = 0;
+  for (int iq = 0; iq < sym.inq; ++iq) {
+    GGPU_HIP(hipEventRecord(ev[2], stream));
+    p0_for_q(iq);
+    GGPU_HIP(hipEventRecord(ev[3], stream));
+    dyson_for_q();
+    GGPU_HIP(hipEventRecord(ev[4], stream));
+    sigma_for_q(iq);
+    GGPU_HIP(hipEventRecord(ev[5], stream));
+    GGPU_HIP(hipEventSynchronize(ev[5]));
+    float a = 0, b = 0, c = 0;
+    GGPU_HIP(hipEventElapsedTime(&a, ev[2], ev[3]));
+    GGPU_HIP(hipEventElapsedTime(&b, ev[3], ev[4]));
+    GGPU_HIP(hipEventElapsedTime(&c, ev[4], ev[5]));
+    t_p0 += a; t_dy += b; t_sig += c;
+  }
+  GGPU_HIP(hipEventRecord(ev[5], stream));
+  slab_gather(Sig_acc.get(), sig_src.get(), S_dev, sig_dst.get(), (int)sig_src.size(), n2, 1.0 / sym.nk, stream);
+  GGPU_HIP(hipEventRecord(ev[6], stream));
+  GGPU_HIP(hipStreamSynchronize(stream));
+  float tot = 0, g = 0, sg = 0;
+  GGPU_HIP(hipEventElapsedTime(&tot, ev[0], ev[6]));
+  GGPU_HIP(hipEventElapsedTime(&g, ev[0], ev[1]));
+  GGPU_HIP(hipEventElapsedTime(&sg, ev[5], ev[6]));
+  last_ms[0] = tot; last_ms[1] = g; last_ms[2] = t_p0; last_ms[3] = t_dy; last_ms[4] = t_sig; last_ms[5] = sg;
+  check_info();
+}
+
+void GwPlan::iterate(const double* G, double* Sigma) {
+  require(G && Sigma, "gw: NULL G or Sigma");
+  GGPU_HIP(hipMemcpyAsync(G_host_dev.get(), G, G_host_dev.bytes(), hipMemcpyHostToDevice, stream));
+  iterate_device(G_host_dev.get(), Sig_host_dev.get());
+  GGPU_HIP(hipMemcpyAsync(Sigma, Sig_host_dev.get(), Sig_host_dev.bytes(), hipMemcpyDeviceToHost, stream));
+  GGPU_HIP(hipStreamSynchronize(stream));
+}
+
+void GwPlan::polarization(const double* G, int iq, int which, double* out) {
+  require(store.loaded, "gw: integrals not loaded", kErrState);
+  require(iq >= 0 && iq < sym.inq, "gw: iq out of range");
+  require(which >= 0 && which <= 2, "gw: which must be 0, 1 or 2");
+  require(G && out, "gw: NULL pointer");
+  const int64_t Q2 = (int64_t)NQ * NQ;
+  GGPU_HIP(hipMemcpyAsync(G_host_dev.get(), G, G_host_dev.bytes(), hipMemcpyHostToDevice, stream));
+  stage_G(G_host_dev.get());
+  zero_int(info.get(), nw_b, stream);
+  p0_for_q(iq);
+  if (which == 0) {
+    std::vector<z64> h((size_t)half * Q2);
+    GGPU_HIP(hipMemcpyAsync(h.data(), P0.get(), h.size() * sizeof(z64), hipMemcpyDeviceToHost, stream));
+    GGPU_HIP(hipStreamSynchronize(stream));
+    z64* o = reinterpret_cast<z64*>(out);
+    const double sc = 1.0 / sym.nk;
+    for (int t = 0; t < half; ++t)
+      for (int64_t e = 0; e < Q2; ++e) {
+        const z64 v = h[(size_t)t * Q2 + e];
+        o[(size_t)t * Q2 + e] = z64{v.x * sc, v.y * sc};
+        o[(size_t)(nts - 1 - t) * Q2 + e] = z64{v.x * sc, v.y * sc};
+      }
+    return;
+  }
+  dyson_for_q();
+  const z64* src = (which == 1) ? P0w.get() : Pt.get();
+  const size_t cnt = (which == 1 ? (size_t)nw_b : (size_t)nts) * Q2;
+  GGPU_HIP(hipMemcpyAsync(out, src, cnt * sizeof(z64), hipMemcpyDeviceToHost, stream));
+  GGPU_HIP(hipStreamSynchronize(stream));
+  check_info();
+}
+
+// ---------------------------------------------------------------------------
+// HF plan
+// ---------------------------------------------------------------------------
+HfPlan::HfPlan(int nao_, int ns_, int NQ_, const ggpu_symmetry& s) : nao(nao_), ns(ns_), NQ(NQ_) {
+  require(nao > 0 && NQ > 0, "hf: nao and NQ must be positive");
+  require(ns == 1 || ns == 2, "hf: Invalid value of \"ns\"");
+  sym.load(s, nao, NQ);
+  GGPU_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  const int64_t n2 = (int64_t)nao * nao, stride = (int64_t)NQ * n2;
+  tables.resize(sym.ink);
+  for (int ik = 0; ik < sym.ink; ++ik) {
+    const int64_t k = sym.k_r2f[ik];
+    std::vector<PairView> v;
+    for (int k2 = 0; k2 < sym.nk; ++k2) v.push_back(sym.view(k, k2));
+    tables[ik].build(v, stride, stream);
+  }
+  dm_host_dev.resize((size_t)ns * sym.ink * n2);
+  dm_fbz.resize((size_t)sym.nk * ns * n2);
+  const int64_t per_pair = (1 + ns) * stride * (int64_t)sizeof(z64);
+  size_t fr = 0, tot = 0;
+  GGPU_HIP(hipMemGetInfo(&fr, &tot));
+  const size_t budget = std::min<size_t>((size_t)(0.4 * fr), (size_t)16 << 30);
+  kb = (int)std::max<int64_t>(1, std::min<int64_t>(budget / per_pair, sym.nk));
+  work.resize((size_t)kb * (per_pair / sizeof(z64)));
+  nsplit = 64;
+  Kpart.resize((size_t)nsplit * ns * n2);
+  Kout.resize((size_t)sym.ink * ns * n2);
+  jvec.resize(NQ);
+  S_dev.resize((size_t)ns * sym.ink * n2);
+  Ftmp.resize((size_t)ns * sym.ink * n2);
+  diag_tables.resize(2);
+  {
+    std::vector<PairView> all, ibz;
+    for (int k = 0; k < sym.nk; ++k) all.push_back(sym.view(k, k));
+    for (int ik = 0; ik < sym.ink; ++ik) ibz.push_back(sym.view(sym.k_r2f[ik], sym.k_r2f[ik]));
+    diag_tables[0].build(all, stride, stream);
+    diag_tables[1].build(ibz, stride, stream);
+  }
+  std::vector<int64_t> bo(kb);
+  for (int b = 0; b < kb; ++b) bo[b] = b * stride;
+  bidx_off.resize(kb);
+  GGPU_HIP(hipMemcpy(bidx_off.get(), bo.data(), bo.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+  rot.init(sym, nao, ns, 1, stream);
+}
+
+HfPlan::~HfPlan() {
+  if (stream) {
+    (void)hipStreamSynchronize(stream);
+    (void)hipStreamDestroy(stream);
+  }
+}
+
+// K_s(k) = (px/nk) sum_{k2} sum_Q V^Q(k,k2) D_s(k2) V^Q(k,k2)^dagger  (cuhf_utils.cu:108-159):
+//   X[s][b][Q][i][c] = sum_a V^Q(k,k2_b)[i][a] D_s(k2_b)[a][c]
+//   K[s][i][j]     += (px/nk) sum_{b,Q,c} X[s][b][Q][i][c] conj(V^Q(k,k2_b)[j][c])
+// The (b, Q) reduction is split into `nsplit` slices written to partial
+// buffers, so each launch has nsplit * ns workgroups instead of ns.
+void HfPlan::exchange_device() {
+  const int64_t n2 = (int64_t)nao * nao, stride = (int64_t)NQ * n2;
+  const double px = (ns == 1) ? -0.5 : -1.0;
+  z64* Vdir = work.get();
+  z64* X = Vdir + kb * stride;
+  const int64_t xs = (int64_t)kb * stride;          // one spin slab of X
+  for (int ik = 0; ik < sym.ink; ++ik) {
+    zero(Kpart.get(), Kpart.size(), stream);
+    for (int i0 = 0; i0 < sym.nk; i0 += kb) {
+      const int n = std::min(kb, sym.nk - i0);
+      unfold_pairs(store, tables[ik], i0, n, bidx_off.get(), Vdir, NQ, nao, stream);
+      for (int b = 0; b < n; ++b) {
+        const int64_t k2 = i0 + b;
+        GemmDesc g;
+        g.M = NQ * nao; g.N = nao; g.K = nao; g.nb0 = ns;
+        g.A.ptr = Vdir + b * stride; g.A.rs = nao; g.A.cs = 1;
+        g.B.ptr = dm_fbz.get() + k2 * ns * n2; g.B.rs = nao; g.B.cs = 1; g.B.bs0 = n2;
+        g.C.ptr = X + b * stride; g.C.rs = nao; g.C.cs = 1; g.C.bs0 = xs;
+        gemm_on(g, work, dm_fbz, work, stream);
+      }
+      const int total = n * NQ;
+      const int ch = (total + nsplit - 1) / nsplit;
+      const int full = total / ch, rem = total - full * ch;
+      for (int part = 0; part < 2; ++part) {
+        const int slices = part == 0 ? full : (rem > 0 ? 1 : 0);
+        if (slices == 0) continue;
+        const int kbat = part == 0 ? ch : rem;
+        const int64_t first = part == 0 ? 0 : (int64_t)full * ch;
+        GemmDesc f;
+        f.M = nao; f.N = nao; f.K = nao; f.KB = kbat; f.nb0 = ns; f.nb1 = slices;
+        f.A.ptr = X + first * n2; f.A.rs = nao; f.A.cs = 1; f.A.kbs = n2; f.A.bs0 = xs; f.A.bs1 = (int64_t)ch * n2;
+        f.B.ptr = Vdir + first * n2; f.B.rs = 1; f.B.cs = nao; f.B.kbs = n2; f.B.bs0 = 0; f.B.bs1 = (int64_t)ch * n2;
+        f.B.conj = 1;
+        f.C.ptr = Kpart.get() + (part == 0 ? 0 : (int64_t)full * ns * n2); f.C.rs = nao; f.C.cs = 1;
+        f.C.bs0 = n2; f.C.bs1 = (int64_t)ns * n2;
+        f.alpha_re = px / sym.nk; f.beta_re = 1.0;
+        gemm_on(f, work, work, Kpart, stream);
+      }
+    }
+    sum_partials(Kpart.get(), (int64_t)ns * n2, nsplit, Kout.get() + (int64_t)ik * ns * n2, (int64_t)ns * n2, stream);
+  }
+}
+
+static void ibz_to_host_layout(const std::vector<z64>& h, double* out, int ink, int ns, int64_t n2) {
+  z64* o = reinterpret_cast<z64*>(out);
+  for (int ik = 0; ik < ink; ++ik)
+    for (int s = 0; s < ns; ++s)
+      std::memcpy(o + ((int64_t)s * ink + ik) * n2, h.data() + ((int64_t)ik * ns + s) * n2, n2 * sizeof(z64));
+}
+
+void HfPlan::exchange(const double* dm, double* K) {
+  require(store.loaded, "hf: integrals not loaded", kErrState);
+  require(dm && K, "hf: NULL pointer");
+  GGPU_HIP(hipMemcpyAsync(dm_host_dev.get(), dm, dm_host_dev.bytes(), hipMemcpyHostToDevice, stream));
+  rot.run(dm_host_dev.get(), dm_fbz.get(), stream);   // (nk, ns, 1, nao, nao)
+  exchange_device();
+  std::vector<z64> h(Kout.size());
+  GGPU_HIP(hipMemcpyAsync(h.data(), Kout.get(), Kout.bytes(), hipMemcpyDeviceToHost, stream));
+  GGPU_HIP(hipStreamSynchronize(stream));
+  ibz_to_host_layout(h, K, sym.ink, ns, (int64_t)nao * nao);
+}
+
+// F = J + K - c madelung S D S  (hf_gpu_kernel.cpp:66-104, :141-201)
+void HfPlan::solve(const double* dm, const double* S_k, double madelung, double* F) {
+  require(store.loaded, "hf: integrals not loaded", kErrState);
+  require(dm && S_k && F, "hf: NULL pointer");
+  const int64_t n2 = (int64_t)nao * nao, stride = (int64_t)NQ * n2;
+  GGPU_HIP(hipMemcpyAsync(dm_host_dev.get(), dm, dm_host_dev.bytes(), hipMemcpyHostToDevice, stream));
+  GGPU_HIP(hipMemcpyAsync(S_dev.get(), S_k, S_dev.bytes(), hipMemcpyHostToDevice, stream));
+  rot.run(dm_host_dev.get(), dm_fbz.get(), stream);
+  exchange_device();                                   // Kout[ik][s]
+  // Hartree: jvec[Q] = (1/nk) sum_{k', s} sum_{a,b} V^Q(k',k')[a][b] D_s(k')[b][a]
+  z64* Vdir = work.get();
+  zero(jvec.get(), jvec.size(), stream);
+  for (int i0 = 0; i0 < sym.nk; i0 += kb) {
+    const int n = std::min(kb, sym.nk - i0);
+    unfold_pairs(store, diag_tables[0], i0, n, bidx_off.get(), Vdir, NQ, nao, stream);
+    for (int b = 0; b < n; ++b)
+      for (int s = 0; s < ns; ++s) {
+        GemmDesc g;   // (NQ x 1) = sum_a sum_b V[Q][a][b] D[b][a]: K index b, K-batch a
+        g.M = NQ; g.N = 1; g.K = nao; g.KB = nao;
+        g.A.ptr = Vdir + b * stride; g.A.rs = n2; g.A.cs = 1; g.A.kbs = nao;
+        g.B.ptr = dm_fbz.get() + ((int64_t)(i0 + b) * ns + s) * n2; g.B.rs = nao; g.B.cs = 0; g.B.kbs = 1;
+        g.C.ptr = jvec.get(); g.C.rs = 1; g.C.cs = 1;
+        g.alpha_re = 1.0 / sym.nk; g.beta_re = 1.0;
+        gemm_on(g, work, dm_fbz, jvec, stream);
+      }
+  }
+  // F[ik][s] = K + sum_Q jvec[Q] V^Q(k,k) - c madelung S D S
+  const double c = (ns == 2) ? 1.0 : 0.5;
+  for (int i0 = 0; i0 < sym.ink; i0 += kb) {
+    const int n = std::min(kb, sym.ink - i0);
+    unfold_pairs(store, diag_tables[1], i0, n, bidx_off.get(), Vdir, NQ, nao, stream);
+    for (int b = 0; b < n; ++b) {
+      GemmDesc g;   // (1 x nao^2) += jvec^T (1 x NQ) . V (NQ x nao^2), for both spins
+      g.M = 1; g.N = (int)n2; g.K = NQ; g.nb0 = ns;
+      g.A.ptr = jvec.get(); g.A.rs = 0; g.A.cs = 1;
+      g.B.ptr = Vdir + b * stride; g.B.rs = n2; g.B.cs = 1;
+      g.C.ptr = Kout.get() + (int64_t)(i0 + b) * ns * n2; g.C.rs = 0; g.C.cs = 1; g.C.bs0 = n2;
+      g.beta_re = 1.0;
+      gemm_on(g, jvec, work, Kout, stream);
+    }
+  }
+  // Ewald: Ftmp[s][ik] = S D, then Kout[ik][s] -= c madelung (S D) S
+  {
+    GemmDesc g;
+    g.M = nao; g.N = nao; g.K = nao; g.nb0 = ns * sym.ink;
+    g.A.ptr = S_dev.get(); g.A.rs = nao; g.A.cs = 1; g.A.bs0 = n2;
+    g.B.ptr = dm_host_dev.get(); g.B.rs = nao; g.B.cs = 1; g.B.bs0 = n2;
+    g.C.ptr = Ftmp.get(); g.C.rs = nao; g.C.cs = 1; g.C.bs0 = n2;
+    gemm_on(g, S_dev, dm_host_dev, Ftmp, stream);
+    for (int s = 0; s < ns; ++s) {
+      GemmDesc h;
+      h.M = nao; h.N = nao; h.K = nao; h.nb0 = sym.ink;
+      h.A.ptr = Ftmp.get() + (int64_t)s * sym.ink * n2; h.A.rs = nao; h.A.cs = 1; h.A.bs0 = n2;
+      h.B.ptr = S_dev.get() + (int64_t)s * sym.ink * n2; h.B.rs = nao; h.B.cs = 1; h.B.bs0 = n2;
+      h.C.ptr = Kout.get() + (int64_t)s * n2; h.C.rs = nao; h.C.cs = 1; h.C.bs0 = (int64_t)ns * n2;
+      h.alpha_re = -c * madelung; h.beta_re = 1.0;
+      gemm_on(h, Ftmp, S_dev, Kout, stream);
+    }
+  }
+  std::vector<z64> h(Kout.size());
+  GGPU_HIP(hipMemcpyAsync(h.data(), Kout.get(), Kout.bytes(), hipMemcpyDeviceToHost, stream));
+  GGPU_HIP(hipStreamSynchronize(stream));
+  ibz_to_host_layout(h, F, sym.ink, ns, n2);
+}
+
+}  // namespace ggpu

@@ -1,0 +1,134 @@
+// Host-side engines of the HF / GW path: symmetry tables, the device
+// integral store, and the plans behind the C ABI.
+#pragma once
+
+#include "../../include/green_gpu_amd.h"
+#include "common.hpp"
+#include "kernels.hpp"
+#include "zgemm.hpp"
+
+#include <array>
+
+namespace ggpu {
+
+// How stored pair data is turned into V^Q(k1,k2): per-Q transpose and/or
+// complex conjugation of the stored block at position `pos`
+// (df_integral_t.h:146-235: conj-transpose if k1 < k2, then conj for a
+// conjugated pair, transpose for a transposed pair).
+struct PairView {
+  int64_t pos;
+  int trans;
+  int conj;
+};
+
+struct SymTables {
+  int nk = 0, ink = 0, nq = 0, inq = 0;
+  std::vector<int64_t> k_f2r, k_r2f, k_tr, q_f2r, q_r2f, q_tr, k1_from_k2q, k2_from_k1q;
+  std::vector<std::vector<int64_t>> k_stars, q_stars;
+  std::vector<z64> U_k, U_q;  // empty = identity
+  std::vector<int64_t> conj_kpair, trans_kpair, irre_pos;
+  int64_t num_stored = 0;
+
+  void load(const ggpu_symmetry& s, int nao, int NQ);
+  PairView view(int64_t k1, int64_t k2) const;
+  bool k_identity() const { return U_k.empty(); }
+  bool q_identity() const { return U_q.empty(); }
+};
+
+// Device copy of the stored k-pair integrals, (num_stored, NQ, nao, nao).
+struct IntegralStore {
+  DeviceBuffer<z64> V;
+  bool loaded = false;
+  void load_host(const double* V_stored, int64_t n_elems, hipStream_t s);
+  void synth(const SymTables& t, int NQ, int nao, uint64_t seed, double sigma, hipStream_t s);
+};
+
+// Per-outer-batch tables for BlockCopy (device arrays uploaded once).
+struct PairTable {
+  DeviceBuffer<int64_t> in_off;   // pos * NQ * nao^2
+  DeviceBuffer<int> trans, conj;
+  std::vector<PairView> host;
+  void build(const std::vector<PairView>& views, int64_t stride, hipStream_t s);
+};
+
+// Unfold pairs [i0, i0+n) of a PairTable into dst[b] (NQ x nao x nao each).
+void unfold_pairs(const IntegralStore& st, const PairTable& tab, int i0, int n, const int64_t* out_off_dev,
+                  z64* dst, int NQ, int nao, hipStream_t s);
+
+// Rotate IBZ matrices into the full BZ: out[k][s][t] = U_k X[t][s][ibz(k)] U_k^dagger
+// (conjugated for time-reversal images).  X has host layout (nt, ns, ink, n, n),
+// out has (nk, ns, nt, n, n).
+class FullBZRotator {
+ public:
+  void init(const SymTables& t, int n, int ns, int nt, hipStream_t s);
+  void run(const z64* X, z64* out, hipStream_t s);
+
+ private:
+  const SymTables* t_ = nullptr;
+  int n_ = 0, ns_ = 0, nt_ = 0;
+  DeviceBuffer<int64_t> src_, dst_;
+  DeviceBuffer<z64> Uk_, tmp_;
+};
+
+struct GwPlan {
+  int nao, ns, NQ, nts, nw_b, half;
+  Compute compute;
+  int solver;
+  SymTables sym;
+  IntegralStore store;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[7];
+
+  DeviceBuffer<z64> Tf_wt;   // folded tau -> Omega transform, (nw_b, half-1)
+  DeviceBuffer<z64> T_tw;    // (nts, nw_b)
+  DeviceBuffer<z64> G_host_dev, Gf, Sig_acc, Sig_host_dev;
+  DeviceBuffer<z64> P0, P0w, Amat, Pt, Prot, Ptmp, Uq_dev;
+  DeviceBuffer<int> info;
+  DeviceBuffer<z64> work;
+  DeviceBuffer<int64_t> bidx_off;       // b * NQ * nao^2
+  DeviceBuffer<int64_t> sig_src, sig_dst;
+  std::vector<PairTable> p0_tables;     // per IBZ q
+  std::vector<std::vector<PairTable>> sig_tables;  // per IBZ q, per star member
+  FullBZRotator rot;
+  int kb = 1;                           // pairs per batch
+  double last_ms[6] = {0, 0, 0, 0, 0, 0};
+
+  GwPlan(int nao, int ns, int NQ, int nts, int nw_b, const ggpu_symmetry& s, const double* T_wt,
+         const double* T_tw, int precision, int solver, size_t work_bytes);
+  ~GwPlan();
+
+  void iterate_device(const z64* G_dev_hostlayout, z64* Sigma_dev_hostlayout);
+  void iterate(const double* G, double* Sigma);
+  void polarization(const double* G, int iq, int which, double* out);
+
+ private:
+  void stage_G(const z64* G_dev_hostlayout);
+  void p0_for_q(int iq);
+  void dyson_for_q();
+  void sigma_for_q(int iq);
+  void check_info();
+};
+
+struct HfPlan {
+  int nao, ns, NQ;
+  SymTables sym;
+  IntegralStore store;
+  hipStream_t stream = nullptr;
+  FullBZRotator rot;
+  DeviceBuffer<z64> dm_host_dev, dm_fbz, Kpart, Kout, work;
+  DeviceBuffer<int64_t> bidx_off;
+  std::vector<PairTable> tables;   // per IBZ k: pairs (k, k2) for all k2
+  int kb = 1, nsplit = 1;
+
+  HfPlan(int nao, int ns, int NQ, const ggpu_symmetry& s);
+  ~HfPlan();
+  void exchange(const double* dm, double* K);
+  void solve(const double* dm, const double* S_k, double madelung, double* F);
+
+ private:
+  void exchange_device();   // Kout[ik][s] from dm_fbz
+  std::vector<PairTable> diag_tables;  // (k, k) pairs: [0] all full k, [1] IBZ representatives
+  DeviceBuffer<z64> jvec, S_dev, Ftmp;
+};
+
+}  // namespace ggpu

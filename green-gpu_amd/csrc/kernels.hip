@@ -1,0 +1,367 @@
+// Bandwidth-bound and small-matrix kernels of the HF/GW path (gfx950).
+#include "kernels.hpp"
+
+#include <cmath>
+
+namespace ggpu {
+namespace {
+
+constexpr int TILE = 32;
+
+__device__ inline z64 cj(z64 a) { return z64{a.x, -a.y}; }
+__device__ inline z64 cmul(z64 a, z64 b) { return z64{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__device__ inline z64 csub(z64 a, z64 b) { return z64{a.x - b.x, a.y - b.y}; }
+
+// 32 x 32 tiles through LDS; block = 32 x 8 threads.
+__global__ __launch_bounds__(256) void block_copy_kernel(const BlockCopy a) {
+  __shared__ z64 tile[TILE][TILE + 1];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int c0 = blockIdx.x * TILE, r0 = blockIdx.y * TILE;
+  const int outer = blockIdx.z / a.n_inner, inner = blockIdx.z - outer * a.n_inner;
+  const z64* in = a.in + (a.in_off ? a.in_off[outer] : 0) + inner * a.in_bs;
+  z64* out = a.out + (a.out_off ? a.out_off[outer] : 0) + inner * a.out_bs;
+  const bool tr = a.trans ? a.trans[outer] != 0 : a.trans_all != 0;
+  const bool cg = a.conj ? a.conj[outer] != 0 : a.conj_all != 0;
+  if (tr) {
+    // in is cols x rows: element (r, c) of the output is in[c][r].
+#pragma unroll
+    for (int j = 0; j < TILE; j += 8) {
+      const int c = c0 + ty + j, r = r0 + tx;
+      if (c < a.cols && r < a.rows) tile[ty + j][tx] = in[c * a.in_ld + r];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < TILE; j += 8) {
+      const int r = r0 + ty + j, c = c0 + tx;
+      if (r < a.rows && c < a.cols) {
+        z64 v = tile[tx][ty + j];
+        out[r * a.out_ld + c] = cg ? cj(v) : v;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < TILE; j += 8) {
+      const int r = r0 + ty + j, c = c0 + tx;
+      if (r < a.rows && c < a.cols) {
+        z64 v = in[r * a.in_ld + c];
+        out[r * a.out_ld + c] = cg ? cj(v) : v;
+      }
+    }
+  }
+}
+
+__global__ void slab_gather_kernel(const z64* in, const int64_t* src_off, z64* out,
+                                   const int64_t* dst_off, int n_slabs, int64_t len, double scale) {
+  const int64_t total = (int64_t)n_slabs * len;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t o = e / len, i = e - o * len;
+    const z64 v = in[src_off[o] + i];
+    out[dst_off[o] + i] = z64{scale * v.x, scale * v.y};
+  }
+}
+
+__global__ __launch_bounds__(256) void herm_one_minus_kernel(z64* M, z64* A, int n) {
+  __shared__ z64 t1[TILE][TILE + 1];
+  __shared__ z64 t2[TILE][TILE + 1];
+  // lower-triangular tile pair index -> (bi, bj), bi >= bj
+  const int x = blockIdx.x;
+  int bi = (int)((sqrt(8.0 * x + 1.0) - 1.0) * 0.5);
+  while ((bi + 1) * (bi + 2) / 2 <= x) ++bi;
+  while (bi * (bi + 1) / 2 > x) --bi;
+  const int bj = x - bi * (bi + 1) / 2;
+  const int64_t w = blockIdx.y;
+  z64* Mw = M + w * (int64_t)n * n;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int ri = bi * TILE, rj = bj * TILE;
+#pragma unroll
+  for (int j = 0; j < TILE; j += 8) {
+    const int r = ty + j;
+    if (ri + r < n && rj + tx < n) t1[r][tx] = Mw[(int64_t)(ri + r) * n + rj + tx];   // tile (bi, bj)
+    if (rj + r < n && ri + tx < n) t2[r][tx] = Mw[(int64_t)(rj + r) * n + ri + tx];   // tile (bj, bi)
+  }
+  __syncthreads();
+  z64* Aw = A ? A + w * (int64_t)n * n : nullptr;
+#pragma unroll
+  for (int j = 0; j < TILE; j += 8) {
+    const int r = ty + j;
+    // element (ri + r, rj + tx) of H = (M + M^dagger)/2
+    if (ri + r < n && rj + tx < n) {
+      const z64 a = t1[r][tx], b = t2[tx][r];
+      z64 h = z64{0.5 * (a.x + b.x), 0.5 * (a.y - b.y)};
+      const int64_t idx = (int64_t)(ri + r) * n + rj + tx;
+      Mw[idx] = h;
+      if (Aw) Aw[idx] = z64{(ri + r == rj + tx ? 1.0 : 0.0) - h.x, -h.y};
+    }
+    // element (rj + r, ri + tx)
+    if (bi != bj && rj + r < n && ri + tx < n) {
+      const z64 a = t2[r][tx], b = t1[tx][r];
+      z64 h = z64{0.5 * (a.x + b.x), 0.5 * (a.y - b.y)};
+      const int64_t idx = (int64_t)(rj + r) * n + ri + tx;
+      Mw[idx] = h;
+      if (Aw) Aw[idx] = z64{-h.x, -h.y};
+    }
+  }
+}
+
+__global__ void zero_kernel(z64* p, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = z64{0.0, 0.0};
+}
+__global__ void zero_int_kernel(int* p, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+__global__ void sum_partials_kernel(const z64* in, int64_t stride, int np, z64* out, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double re = 0.0, im = 0.0;
+    for (int p = 0; p < np; ++p) { const z64 v = in[p * stride + i]; re += v.x; im += v.y; }
+    out[i] = z64{re, im};
+  }
+}
+
+constexpr int NBMAX = 32;
+
+// One workgroup per matrix; the nb x nb diagonal block lives in LDS.
+__global__ __launch_bounds__(256) void chol_diag_kernel(z64* A, int n, int j0, int nb, int* info) {
+  __shared__ z64 L[NBMAX][NBMAX + 1];
+  const int64_t w = blockIdx.x;
+  z64* Aw = A + w * (int64_t)n * n;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < nb * nb; e += blockDim.x) {
+    const int r = e / nb, c = e - r * nb;
+    L[r][c] = (c <= r) ? Aw[(int64_t)(j0 + r) * n + j0 + c] : z64{0.0, 0.0};
+  }
+  __syncthreads();
+  for (int c = 0; c < nb; ++c) {
+    if (tid == 0) {
+      const double d = L[c][c].x;
+      if (!(d > 0.0)) {                    // not positive definite (or NaN)
+        if (info[w] == 0) info[w] = j0 + c + 1;
+        L[c][c] = z64{1.0, 0.0};
+      } else {
+        L[c][c] = z64{sqrt(d), 0.0};
+      }
+    }
+    __syncthreads();
+    const double inv = 1.0 / L[c][c].x;
+    for (int r = c + 1 + tid; r < nb; r += blockDim.x) L[r][c] = z64{L[r][c].x * inv, L[r][c].y * inv};
+    __syncthreads();
+    const int m = nb - c - 1;
+    for (int e = tid; e < m * m; e += blockDim.x) {
+      const int r = c + 1 + e / m, s = c + 1 + e % m;
+      if (s <= r) L[r][s] = csub(L[r][s], cmul(L[r][c], cj(L[s][c])));
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < nb * nb; e += blockDim.x) {
+    const int r = e / nb, c = e - r * nb;
+    if (c <= r) Aw[(int64_t)(j0 + r) * n + j0 + c] = L[r][c];
+  }
+}
+
+// Rows below the diagonal block: x L11^dagger = a.  64 rows per workgroup.
+__global__ __launch_bounds__(64) void chol_panel_kernel(z64* A, int n, int j0, int nb) {
+  __shared__ z64 L[NBMAX][NBMAX + 1];
+  __shared__ z64 X[64][NBMAX + 1];
+  const int64_t w = blockIdx.y;
+  z64* Aw = A + w * (int64_t)n * n;
+  const int tid = threadIdx.x;
+  const int r0 = j0 + nb + blockIdx.x * 64;
+  for (int e = tid; e < nb * nb; e += 64) {
+    const int r = e / nb, c = e - r * nb;
+    L[r][c] = Aw[(int64_t)(j0 + r) * n + j0 + c];
+  }
+  for (int e = tid; e < 64 * nb; e += 64) {
+    const int r = e / nb, c = e - r * nb;
+    if (r0 + r < n) X[r][c] = Aw[(int64_t)(r0 + r) * n + j0 + c];
+  }
+  __syncthreads();
+  if (r0 + tid < n) {
+    for (int c = 0; c < nb; ++c) {
+      z64 acc = X[tid][c];
+      for (int s = 0; s < c; ++s) acc = csub(acc, cmul(X[tid][s], cj(L[c][s])));
+      const double inv = 1.0 / L[c][c].x;
+      X[tid][c] = z64{acc.x * inv, acc.y * inv};
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < 64 * nb; e += 64) {
+    const int r = e / nb, c = e - r * nb;
+    if (r0 + r < n) Aw[(int64_t)(r0 + r) * n + j0 + c] = X[r][c];
+  }
+}
+
+// Forward substitution with the nb x nb lower block, 64 right-hand-side
+// columns per workgroup (one column per thread).
+template <bool UPPER_H>
+__global__ __launch_bounds__(64) void trsm_block_kernel(const z64* L, z64* B, int n, int nrhs, int i0, int nb) {
+  __shared__ z64 T[NBMAX][NBMAX + 1];
+  __shared__ z64 Y[NBMAX][64 + 1];
+  const int64_t w = blockIdx.y;
+  const z64* Lw = L + w * (int64_t)n * n;
+  z64* Bw = B + w * (int64_t)n * nrhs;
+  const int tid = threadIdx.x;
+  const int c0 = blockIdx.x * 64;
+  for (int e = tid; e < nb * nb; e += 64) {
+    const int r = e / nb, c = e - r * nb;
+    T[r][c] = Lw[(int64_t)(i0 + r) * n + i0 + c];
+  }
+  for (int e = tid; e < nb * 64; e += 64) {
+    const int r = e / 64, c = e - r * 64;
+    if (c0 + c < nrhs) Y[r][c] = Bw[(int64_t)(i0 + r) * nrhs + c0 + c];
+  }
+  __syncthreads();
+  if (c0 + tid < nrhs) {
+    if (!UPPER_H) {
+      for (int r = 0; r < nb; ++r) {
+        z64 acc = Y[r][tid];
+        for (int s = 0; s < r; ++s) acc = csub(acc, cmul(T[r][s], Y[s][tid]));
+        const double inv = 1.0 / T[r][r].x;
+        Y[r][tid] = z64{acc.x * inv, acc.y * inv};
+      }
+    } else {
+      for (int r = nb - 1; r >= 0; --r) {
+        z64 acc = Y[r][tid];
+        for (int s = r + 1; s < nb; ++s) acc = csub(acc, cmul(cj(T[s][r]), Y[s][tid]));
+        const double inv = 1.0 / T[r][r].x;
+        Y[r][tid] = z64{acc.x * inv, acc.y * inv};
+      }
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < nb * 64; e += 64) {
+    const int r = e / 64, c = e - r * 64;
+    if (c0 + c < nrhs) Bw[(int64_t)(i0 + r) * nrhs + c0 + c] = Y[r][c];
+  }
+}
+
+__device__ inline uint64_t splitmix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// Complex Gaussian sample for counter e (two uniforms from one hash chain,
+// Box-Muller).  The host generator of the test suite uses the same formulas.
+__device__ inline z64 gauss_c(uint64_t seed, uint64_t e, double sigma) {
+  const uint64_t h1 = splitmix64(seed * 0xD1B54A32D192ED03ull + e);
+  const uint64_t h2 = splitmix64(h1);
+  const double u1 = ((double)(h1 >> 11) + 0.5) * 0x1.0p-53;
+  const double u2 = (double)(h2 >> 11) * 0x1.0p-53;
+  const double rad = sigma * sqrt(-2.0 * log(u1));
+  const double ang = 6.283185307179586 * u2;
+  return z64{rad * cos(ang), rad * sin(ang)};
+}
+
+__global__ void synth_integrals_kernel(z64* V, int64_t p0, int64_t np, const int* diag, int NQ, int nao,
+                                       uint64_t seed, double sigma) {
+  const int64_t per_pair = (int64_t)NQ * nao * nao;
+  const int64_t total = np * per_pair;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t lp = e / per_pair;
+    const int64_t rem = e - lp * per_pair;
+    const int64_t Q = rem / ((int64_t)nao * nao);
+    const int ij = (int)(rem - Q * nao * nao);
+    const int i = ij / nao, j = ij - i * nao;
+    const int64_t p = p0 + lp;
+    const int64_t base = (p * NQ + Q) * nao * nao;
+    z64 v = gauss_c(seed, (uint64_t)(base + ij), sigma);
+    if (diag[lp]) {
+      const z64 t = gauss_c(seed, (uint64_t)(base + j * nao + i), sigma);
+      v = z64{0.5 * (v.x + t.x), 0.5 * (v.y - t.y)};
+    }
+    V[e] = v;
+  }
+}
+
+inline int grid_for(int64_t n, int block) {
+  int64_t g = (n + block - 1) / block;
+  if (g > 65536) g = 65536;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+}  // namespace
+
+void block_copy(const BlockCopy& a, hipStream_t s) {
+  if (a.rows <= 0 || a.cols <= 0) return;
+  require(a.n_outer * (int64_t)a.n_inner <= 2147483647, "block_copy: batch too large");
+  dim3 grid((a.cols + TILE - 1) / TILE, (a.rows + TILE - 1) / TILE, a.n_outer * a.n_inner);
+  require(grid.z <= 65535u, "block_copy: batch above 65535");
+  hipLaunchKernelGGL(block_copy_kernel, grid, dim3(256), 0, s, a);
+  GGPU_HIP(hipGetLastError());
+}
+
+void slab_gather(const z64* in, const int64_t* src_off, z64* out, const int64_t* dst_off, int n_slabs,
+                 int64_t len, double scale, hipStream_t s) {
+  if (n_slabs <= 0 || len <= 0) return;
+  hipLaunchKernelGGL(slab_gather_kernel, dim3(grid_for(n_slabs * len, 256)), dim3(256), 0, s, in, src_off,
+                     out, dst_off, n_slabs, len, scale);
+  GGPU_HIP(hipGetLastError());
+}
+
+void herm_one_minus(z64* M, z64* A, int n, int nw, hipStream_t s) {
+  if (n <= 0 || nw <= 0) return;
+  const int T = (n + TILE - 1) / TILE;
+  require(nw <= 65535, "herm_one_minus: too many matrices");
+  hipLaunchKernelGGL(herm_one_minus_kernel, dim3(T * (T + 1) / 2, nw), dim3(256), 0, s, M, A, n);
+  GGPU_HIP(hipGetLastError());
+}
+
+void zero(z64* p, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(zero_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n);
+  GGPU_HIP(hipGetLastError());
+}
+
+void zero_int(int* p, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(zero_int_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n);
+  GGPU_HIP(hipGetLastError());
+}
+
+void sum_partials(const z64* in, int64_t stride, int np, z64* out, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(sum_partials_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, in, stride, np, out, n);
+  GGPU_HIP(hipGetLastError());
+}
+
+void chol_diag_block(z64* A, int n, int nw, int j0, int nb, int* info, hipStream_t s) {
+  require(nb >= 1 && nb <= NBMAX && j0 >= 0 && j0 + nb <= n, "chol_diag_block: bad block");
+  require(nw <= 2147483647, "chol_diag_block: nw");
+  hipLaunchKernelGGL(chol_diag_kernel, dim3(nw), dim3(256), 0, s, A, n, j0, nb, info);
+  GGPU_HIP(hipGetLastError());
+}
+
+void chol_panel(z64* A, int n, int nw, int j0, int nb, hipStream_t s) {
+  require(nb >= 1 && nb <= NBMAX && j0 >= 0 && j0 + nb <= n, "chol_panel: bad block");
+  const int rows = n - j0 - nb;
+  if (rows <= 0) return;
+  require(nw <= 65535, "chol_panel: nw");
+  hipLaunchKernelGGL(chol_panel_kernel, dim3((rows + 63) / 64, nw), dim3(64), 0, s, A, n, j0, nb);
+  GGPU_HIP(hipGetLastError());
+}
+
+void trsm_lower_block(const z64* L, z64* B, int n, int nrhs, int nw, int i0, int nb, hipStream_t s) {
+  require(nb >= 1 && nb <= NBMAX && i0 >= 0 && i0 + nb <= n && nw <= 65535, "trsm_lower_block: bad block");
+  hipLaunchKernelGGL((trsm_block_kernel<false>), dim3((nrhs + 63) / 64, nw), dim3(64), 0, s, L, B, n, nrhs, i0, nb);
+  GGPU_HIP(hipGetLastError());
+}
+
+void trsm_upper_h_block(const z64* L, z64* B, int n, int nrhs, int nw, int i0, int nb, hipStream_t s) {
+  require(nb >= 1 && nb <= NBMAX && i0 >= 0 && i0 + nb <= n && nw <= 65535, "trsm_upper_h_block: bad block");
+  hipLaunchKernelGGL((trsm_block_kernel<true>), dim3((nrhs + 63) / 64, nw), dim3(64), 0, s, L, B, n, nrhs, i0, nb);
+  GGPU_HIP(hipGetLastError());
+}
+
+void synth_integrals(z64* V, int64_t p0, int64_t np, const int* pair_is_diag, int NQ, int nao, uint64_t seed,
+                     double sigma, hipStream_t s) {
+  if (np <= 0) return;
+  hipLaunchKernelGGL(synth_integrals_kernel, dim3(grid_for(np * NQ * (int64_t)nao * nao, 256)), dim3(256), 0, s,
+                     V, p0, np, pair_is_diag, NQ, nao, seed, sigma);
+  GGPU_HIP(hipGetLastError());
+}
+
+}  // namespace ggpu

@@ -1,0 +1,62 @@
+// Bandwidth-bound and small-matrix kernels of the HF/GW path (gfx950).
+#pragma once
+
+#include "common.hpp"
+
+namespace ggpu {
+
+// out[b][r][c] = op(in[b]) where op is a copy (trans = 0) or a transpose
+// (trans = 1, reads in[b][c][r]) of an rows x cols block, optionally
+// complex-conjugated.  Batch b = (outer, inner): element offsets are
+// in_off[outer] + inner * in_bs (same for out); flags are per outer batch.
+struct BlockCopy {
+  const z64* in = nullptr;
+  z64* out = nullptr;
+  int rows = 0, cols = 0;          // shape of the OUTPUT block
+  int64_t in_ld = 0, out_ld = 0;   // leading dimensions (row strides)
+  int n_inner = 1;
+  int64_t in_bs = 0, out_bs = 0;
+  int n_outer = 1;
+  const int64_t* in_off = nullptr;   // device, n_outer entries
+  const int64_t* out_off = nullptr;  // device, n_outer entries
+  const int* trans = nullptr;        // device, n_outer entries (nullptr: trans_all)
+  const int* conj = nullptr;         // device, n_outer entries (nullptr: conj_all)
+  int trans_all = 0, conj_all = 0;
+};
+void block_copy(const BlockCopy& a, hipStream_t s);
+
+// Contiguous-slab gather: out[o][i] = scale * in[src[o]][i] for i < len
+// (complex elements; src offsets in elements).  Used for the host <-> device
+// layout changes of G and Sigma.
+void slab_gather(const z64* in, const int64_t* src_off, z64* out, const int64_t* dst_off,
+                 int n_slabs, int64_t len, double scale, hipStream_t s);
+
+// For every w: H = (M + M^dagger)/2 (real diagonal), written back to M, and
+// A = I - H written to A (if A != nullptr).  M, A: nw x n x n.
+void herm_one_minus(z64* M, z64* A, int n, int nw, hipStream_t s);
+
+// Fill n complex elements with zero.
+void zero(z64* p, int64_t n, hipStream_t s);
+void zero_int(int* p, int64_t n, hipStream_t s);
+
+// out[i] = sum_p in[p * stride + i], i < n.
+void sum_partials(const z64* in, int64_t stride, int np, z64* out, int64_t n, hipStream_t s);
+
+// ---- batched blocked Cholesky (lower, A = L L^dagger), nw matrices n x n ----
+// Factor the nb x nb diagonal block at (j0, j0) in place; info[w] = column+1
+// of the first non-positive pivot (0 = fine).
+void chol_diag_block(z64* A, int n, int nw, int j0, int nb, int* info, hipStream_t s);
+// A[j0+nb:n, j0:j0+nb] <- A[...] * L11^-dagger
+void chol_panel(z64* A, int n, int nw, int j0, int nb, hipStream_t s);
+// B[i0:i0+nb, :] <- L[i0:i0+nb, i0:i0+nb]^-1 B[i0:i0+nb, :]    (lower, forward)
+void trsm_lower_block(const z64* L, z64* B, int n, int nrhs, int nw, int i0, int nb, hipStream_t s);
+// B[i0:i0+nb, :] <- (L[i0:i0+nb, i0:i0+nb]^dagger)^-1 B[i0:i0+nb, :] (backward)
+void trsm_upper_h_block(const z64* L, z64* B, int n, int nrhs, int nw, int i0, int nb, hipStream_t s);
+
+// ---- synthetic density-fitted integrals (counter-based, reproducible) ----
+// V[p][Q][i][j] for stored pairs p in [p0, p0+np) of the pair list k1 >= k2,
+// Re/Im ~ N(0, sigma^2); diagonal pairs (k1 == k2) are Hermitised per Q.
+void synth_integrals(z64* V, int64_t p0, int64_t np, const int* pair_is_diag, int NQ, int nao,
+                     uint64_t seed, double sigma, hipStream_t s);
+
+}  // namespace ggpu

@@ -1,0 +1,262 @@
+// Batched complex GEMM on the gfx950 matrix cores.
+//
+// Workgroup = 4 wave64s computing a 64 x 64 complex tile of C; each wave owns
+// a 32 x 32 quadrant = 2 x 2 blocks of the 16 x 16 MFMA output, kept as
+// separate real / imaginary accumulators (4 real MFMAs per complex block and
+// k-step: re += ar*br - ai*bi, im += ar*bi + ai*br).
+//
+// Tiles of A (64 x 16) and B (16 x 64) are staged through LDS as split
+// real / imaginary planes, k-major, so the MFMA operand reads (lane -> row
+// lane%16, k lane/16) are unit-stride across 16 lanes.  The next tile is
+// prefetched from global memory into registers while the current one is
+// multiplied.  Global loads follow whichever operand dimension is contiguous
+// (template flags), so every operand layout of the HF/GW chains streams with
+// full 64-byte segments.
+#include "zgemm.hpp"
+
+#include <algorithm>
+#include <cstdio>
+
+namespace ggpu {
+namespace {
+
+constexpr int BM = 64;
+constexpr int BN = 64;
+constexpr int BKT = 16;
+constexpr int NTHREADS = 256;
+constexpr int PAD = 2;
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <class T> struct Cplx;
+template <> struct Cplx<double> { using type = double2; };
+template <> struct Cplx<float> { using type = float2; };
+
+template <class T, class V>
+__device__ inline V mfma(T a, T b, V c);
+template <>
+__device__ inline v4d mfma<double, v4d>(double a, double b, v4d c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+template <>
+__device__ inline v4f mfma<float, v4f>(float a, float b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+template <class T> struct Acc;
+template <> struct Acc<double> { using type = v4d; };
+template <> struct Acc<float> { using type = v4f; };
+
+// Row (within the 16 x 16 block) of accumulator register r of lane group
+// lk = lane / 16.  Measured on gfx950: v_mfma_f64_16x16x4_f64 holds rows
+// lk + 4 r, v_mfma_f32_16x16x4_f32 holds rows 4 lk + r; the column is lane % 16
+// for both.
+template <class T> __device__ inline int acc_row(int lk, int r);
+template <> __device__ inline int acc_row<double>(int lk, int r) { return lk + 4 * r; }
+template <> __device__ inline int acc_row<float>(int lk, int r) { return 4 * lk + r; }
+
+// T: arithmetic type of operands and MFMA; TO: complex element type of C.
+// AKC: A's k index is the contiguous one (else its m index).
+// BKC: B's k index is the contiguous one (else its n index).
+template <class T, class TO, bool AKC, bool BKC>
+__global__ __launch_bounds__(NTHREADS) void gemm_mfma_kernel(const GemmDesc d) {
+  using CT = typename Cplx<T>::type;
+  using AV = typename Acc<T>::type;
+  __shared__ T sA[2][BKT][BM + PAD];   // [re/im][k][m]
+  __shared__ T sB[2][BKT][BN + PAD];   // [re/im][k][n]
+
+  const int tid = threadIdx.x;
+  const int tiles_n = (d.N + BN - 1) / BN;
+  const int m0 = (blockIdx.x / tiles_n) * BM;
+  const int n0 = (blockIdx.x % tiles_n) * BN;
+  const int64_t b1 = blockIdx.y, b0 = blockIdx.z;
+
+  const CT* A = static_cast<const CT*>(d.A.ptr) + b0 * d.A.bs0 + (d.A.off1 ? d.A.off1[b1] : b1 * d.A.bs1);
+  const CT* B = static_cast<const CT*>(d.B.ptr) + b0 * d.B.bs0 + (d.B.off1 ? d.B.off1[b1] : b1 * d.B.bs1);
+  TO* C = static_cast<TO*>(d.C.ptr) + b0 * d.C.bs0 + (d.C.off1 ? d.C.off1[b1] : b1 * d.C.bs1);
+
+  // Loader coordinates inside the tile: 4 complex elements per thread.
+  int a_m[4], a_k[4], b_k[4], b_n[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    if (AKC) { a_m[j] = tid >> 2; a_k[j] = ((tid & 3) << 2) + j; }
+    else     { a_k[j] = tid >> 4; a_m[j] = ((tid & 15) << 2) + j; }
+    if (BKC) { b_n[j] = tid >> 2; b_k[j] = ((tid & 3) << 2) + j; }
+    else     { b_k[j] = tid >> 4; b_n[j] = ((tid & 15) << 2) + j; }
+  }
+
+  const int ktiles = (d.K + BKT - 1) / BKT;
+  const int niter = d.KB * ktiles;
+  CT ra[4], rb[4];
+
+  auto gload = [&](int it) {
+    const int kb = it / ktiles;
+    const int k0 = (it - kb * ktiles) * BKT;
+    const CT* Ab = A + kb * d.A.kbs;
+    const CT* Bb = B + kb * d.B.kbs;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int gm = m0 + a_m[j], gk = k0 + a_k[j];
+      ra[j] = (gm < d.M && gk < d.K) ? Ab[gm * d.A.rs + gk * d.A.cs] : CT{0, 0};
+      const int gk2 = k0 + b_k[j], gn = n0 + b_n[j];
+      rb[j] = (gk2 < d.K && gn < d.N) ? Bb[gk2 * d.B.rs + gn * d.B.cs] : CT{0, 0};
+    }
+  };
+  auto sstore = [&]() {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      sA[0][a_k[j]][a_m[j]] = ra[j].x;
+      sA[1][a_k[j]][a_m[j]] = d.A.conj ? -ra[j].y : ra[j].y;
+      sB[0][b_k[j]][b_n[j]] = rb[j].x;
+      sB[1][b_k[j]][b_n[j]] = d.B.conj ? -rb[j].y : rb[j].y;
+    }
+  };
+
+  const int lane = tid & 63, w = tid >> 6;
+  const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+  const int lr = lane & 15, lk = lane >> 4;
+  AV cr[2][2], ci[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) { cr[i][j] = AV{0, 0, 0, 0}; ci[i][j] = AV{0, 0, 0, 0}; }
+
+  if (niter > 0) gload(0);
+  for (int it = 0; it < niter; ++it) {
+    sstore();
+    __syncthreads();
+    if (it + 1 < niter) gload(it + 1);
+#pragma unroll
+    for (int kk = 0; kk < BKT; kk += 4) {
+      T ar[2], ai[2], nai[2], br[2], bi[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        ar[i] = sA[0][kk + lk][wm + 16 * i + lr];
+        ai[i] = sA[1][kk + lk][wm + 16 * i + lr];
+        nai[i] = -ai[i];
+        br[i] = sB[0][kk + lk][wn + 16 * i + lr];
+        bi[i] = sB[1][kk + lk][wn + 16 * i + lr];
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          cr[i][j] = mfma<T, AV>(ar[i], br[j], cr[i][j]);
+          ci[i][j] = mfma<T, AV>(ar[i], bi[j], ci[i][j]);
+          cr[i][j] = mfma<T, AV>(nai[i], bi[j], cr[i][j]);
+          ci[i][j] = mfma<T, AV>(ai[i], br[j], ci[i][j]);
+        }
+    }
+    __syncthreads();
+  }
+
+  const bool use_beta = (d.beta_re != 0.0 || d.beta_im != 0.0);
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm + 16 * i + acc_row<T>(lk, r);
+        const int col = n0 + wn + 16 * j + lr;
+        if (row < d.M && col < d.N) {
+          const double xr = cr[i][j][r], xi = ci[i][j][r];
+          double vr = d.alpha_re * xr - d.alpha_im * xi;
+          double vi = d.alpha_re * xi + d.alpha_im * xr;
+          TO* p = C + row * d.C.rs + col * d.C.cs;
+          if (use_beta) {
+            const TO o = *p;
+            vr += d.beta_re * o.x - d.beta_im * o.y;
+            vi += d.beta_re * o.y + d.beta_im * o.x;
+          }
+          using TS = decltype(TO{}.x);
+          *p = TO{static_cast<TS>(vr), static_cast<TS>(vi)};
+        }
+      }
+}
+
+template <class T, class TO>
+void launch_typed(const GemmDesc& d, hipStream_t stream) {
+  const int tiles = ((d.M + BM - 1) / BM) * ((d.N + BN - 1) / BN);
+  dim3 grid(tiles, d.nb1, d.nb0), block(NTHREADS);
+  const bool akc = (d.A.cs == 1) || (d.A.rs != 1);
+  const bool bkc = (d.B.rs == 1) || (d.B.cs != 1);
+  if (akc && bkc) hipLaunchKernelGGL((gemm_mfma_kernel<T, TO, true, true>), grid, block, 0, stream, d);
+  else if (akc) hipLaunchKernelGGL((gemm_mfma_kernel<T, TO, true, false>), grid, block, 0, stream, d);
+  else if (bkc) hipLaunchKernelGGL((gemm_mfma_kernel<T, TO, false, true>), grid, block, 0, stream, d);
+  else hipLaunchKernelGGL((gemm_mfma_kernel<T, TO, false, false>), grid, block, 0, stream, d);
+  GGPU_HIP(hipGetLastError());
+}
+
+// Range of element offsets (relative to ptr) touched by a 2-D view with the
+// given extents and strides, over all batch indices.
+struct Range { int64_t lo = 0, hi = 0; };
+inline void extend(Range& r, int64_t n, int64_t stride) {
+  if (n <= 1 || stride == 0) return;
+  const int64_t span = (n - 1) * stride;
+  if (span > 0) r.hi += span; else r.lo += span;
+}
+
+void check_operand(const char* name, const void* ptr, const void* buf, int64_t buf_elems, size_t esize,
+                   const int64_t* off_host, int nb1, Range r) {
+  if (!buf) return;
+  const int64_t base = (int64_t)(((const char*)ptr - (const char*)buf) / (int64_t)esize);
+  int64_t lo = base + r.lo, hi = base + r.hi;
+  if (off_host) {
+    int64_t omin = off_host[0], omax = off_host[0];
+    for (int i = 1; i < nb1; ++i) { omin = std::min(omin, off_host[i]); omax = std::max(omax, off_host[i]); }
+    lo += omin; hi += omax;
+  }
+  if (lo < 0 || hi >= buf_elems) {
+    char msg[256];
+    snprintf(msg, sizeof(msg), "gemm operand %s addresses [%lld, %lld] outside its allocation of %lld elements",
+             name, (long long)lo, (long long)hi, (long long)buf_elems);
+    throw Error(kErrArgument, msg);
+  }
+}
+
+}  // namespace
+
+void gemm_check_bounds(const GemmDesc& d) {
+  require(d.M >= 0 && d.N >= 0 && d.K >= 0 && d.KB >= 1 && d.nb0 >= 1 && d.nb1 >= 1,
+          "gemm: bad dimensions");
+  require(d.nb0 <= 65535 && d.nb1 <= 65535, "gemm: batch count above 65535");
+  if (d.M == 0 || d.N == 0) return;
+  Range ra, rb, rc;
+  extend(ra, d.M, d.A.rs); extend(ra, d.K, d.A.cs); extend(ra, d.KB, d.A.kbs);
+  extend(ra, d.nb0, d.A.bs0); if (!d.A.off1) extend(ra, d.nb1, d.A.bs1);
+  extend(rb, d.K, d.B.rs); extend(rb, d.N, d.B.cs); extend(rb, d.KB, d.B.kbs);
+  extend(rb, d.nb0, d.B.bs0); if (!d.B.off1) extend(rb, d.nb1, d.B.bs1);
+  extend(rc, d.M, d.C.rs); extend(rc, d.N, d.C.cs);
+  extend(rc, d.nb0, d.C.bs0); if (!d.C.off1) extend(rc, d.nb1, d.C.bs1);
+  const size_t ein = (d.compute == Compute::F64) ? sizeof(double2) : sizeof(float2);
+  const size_t eout = d.out_f32 ? sizeof(float2) : sizeof(double2);
+  require(!d.A.off1 || d.A.off1_host || !d.A.buf, "gemm: A offset table without host copy");
+  require(!d.B.off1 || d.B.off1_host || !d.B.buf, "gemm: B offset table without host copy");
+  require(!d.C.off1 || d.C.off1_host || !d.C.buf, "gemm: C offset table without host copy");
+  if (d.K > 0) {
+    check_operand("A", d.A.ptr, d.A.buf, d.A.buf_elems, ein, d.A.off1_host, d.nb1, ra);
+    check_operand("B", d.B.ptr, d.B.buf, d.B.buf_elems, ein, d.B.off1_host, d.nb1, rb);
+  }
+  check_operand("C", d.C.ptr, d.C.buf, d.C.buf_elems, eout, d.C.off1_host, d.nb1, rc);
+}
+
+void gemm(const GemmDesc& d, hipStream_t stream) {
+  gemm_check_bounds(d);
+  if (d.M == 0 || d.N == 0) return;
+  switch (d.compute) {
+    case Compute::F64:
+      require(!d.out_f32, "gemm: FP64 compute writes complex128");
+      launch_typed<double, double2>(d, stream);
+      break;
+    case Compute::F32:
+      if (d.out_f32) launch_typed<float, float2>(d, stream);
+      else launch_typed<float, double2>(d, stream);
+      break;
+    default:
+      throw Error(kErrArgument, "gemm: compute mode not available");
+  }
+}
+
+}  // namespace ggpu

@@ -1,0 +1,67 @@
+// Batched complex GEMM engine for gfx950 (MFMA).
+//
+//   C[b0,b1] (M x N) = alpha * sum_{kb < KB} sum_{k < K} op(A)[m][k] op(B)[k][n]  +  beta * C
+//
+// Every contraction of the HF/GW path is expressed through this one
+// descriptor: each operand is a 2-D strided view (row stride, column stride)
+// with an extra "K-batch" stride that lets one launch reduce over an outer
+// index (spins, k-pairs, auxiliary index Q), and two batch indices with their
+// own strides (or a per-b1 offset table for gathers over k-pairs).  Operand
+// conjugation is applied while staging tiles; there is no transposed or
+// conjugated copy of any operand.
+#pragma once
+
+#include "common.hpp"
+
+namespace ggpu {
+
+enum class Compute : int {
+  F64 = 0,     // complex128 operands, v_mfma_f64_16x16x4_f64, FP64 accumulation
+  F32 = 1,     // complex64 operands, v_mfma_f32_16x16x4_f32, FP32 accumulation
+  BF16X3 = 2,  // complex64 operands split hi+lo into bf16, 3 products on
+               // v_mfma_f32_16x16x32_bf16, FP32 accumulation
+};
+
+struct Operand {
+  const void* ptr = nullptr;   // element 0 of batch (0,0)
+  int64_t rs = 0, cs = 0;      // stride of the first / second logical index (elements)
+  int64_t kbs = 0;             // stride of the K-batch index
+  int64_t bs0 = 0, bs1 = 0;    // batch strides
+  const int64_t* off1 = nullptr;  // optional device table: offset of batch b1 (replaces bs1)
+  int conj = 0;
+  // Host-side bounds check: the allocation [buf, buf + buf_elems) that every
+  // touched element must lie in (buf == nullptr: unchecked).
+  const void* buf = nullptr;
+  int64_t buf_elems = 0;
+  const int64_t* off1_host = nullptr;  // host copy of off1 for the bounds check
+};
+
+struct OutOperand {
+  void* ptr = nullptr;
+  int64_t rs = 0, cs = 0;
+  int64_t bs0 = 0, bs1 = 0;
+  const int64_t* off1 = nullptr;
+  const void* buf = nullptr;
+  int64_t buf_elems = 0;
+  const int64_t* off1_host = nullptr;
+};
+
+struct GemmDesc {
+  int M = 0, N = 0, K = 0, KB = 1;
+  int nb0 = 1, nb1 = 1;
+  Operand A, B;
+  OutOperand C;
+  double alpha_re = 1.0, alpha_im = 0.0;
+  double beta_re = 0.0, beta_im = 0.0;
+  Compute compute = Compute::F64;
+  int out_f32 = 0;             // C holds complex64 (only with F32 / BF16X3 compute)
+};
+
+// Throws Error(kErrArgument) if any address the launch would touch falls
+// outside an operand's declared allocation.
+void gemm_check_bounds(const GemmDesc& d);
+
+// Enqueue the GEMM on `stream`.
+void gemm(const GemmDesc& d, hipStream_t stream);
+
+}  // namespace ggpu

@@ -1,0 +1,165 @@
+/*
+ * C ABI of the gfx950 HF-exchange / GW self-energy library.
+ *
+ * This is the boundary green-mbpt's custom-kernel plugin API sits on.  The
+ * reference exposes the same path as C++ objects behind three factory
+ * functions (src/green/gpu/gpu_factory.h:47-62 custom_hf_kernel, :78-89
+ * custom_gw_kernel, :95-102 custom_kernel_parameters) whose kernels own all
+ * device memory for the duration of one solve (hf_gpu_kernel.cpp:66-104,
+ * gw_gpu_kernel.cpp:163-205).  Here the same work is split into plans that
+ * own their device memory across calls; include/green/gpu/gpu_factory.h
+ * rebuilds the reference's C++ signatures on top of these functions.
+ *
+ * Conventions
+ *   - complex numbers are interleaved (re, im) doubles, i.e. std::complex<double>;
+ *   - every tensor is row-major with the reference's host layout;
+ *   - functions return 0 on success or a ggpu_status code; the message of the
+ *     last failure on the calling thread is returned by ggpu_last_error();
+ *   - plans are not thread-safe: one thread per plan.
+ */
+#ifndef GREEN_GPU_AMD_H
+#define GREEN_GPU_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  GGPU_OK = 0,
+  GGPU_ERR_ARGUMENT = 1,             /* reference: std::runtime_error / std::logic_error on bad input */
+  GGPU_ERR_DEVICE = 2,               /* HIP runtime failure                                            */
+  GGPU_ERR_MEMORY = 3,               /* device or pinned allocation failed (cugw_qkpt.cu:39-75)       */
+  GGPU_ERR_NOT_POSITIVE_DEFINITE = 4,/* 1 - P0(Omega) not HPD; reference aborts (cu_compute_Pq.cu:24-43) */
+  GGPU_ERR_STATE = 5                 /* call out of order (e.g. iterate before integrals are loaded)  */
+} ggpu_status;
+
+/* Linear solver for the Dyson equation, same order as LinearSolverType
+ * (common_defs.h:137-139). */
+typedef enum { GGPU_SOLVER_CHOLESKY = 0, GGPU_SOLVER_LU = 1 } ggpu_linear_solver;
+
+/* Arithmetic of the G.V, P0 and Sigma contractions and the IR transforms.
+ * The Dyson factorisation and the Hermitisation of P0 always run in FP64. */
+typedef enum {
+  GGPU_PRECISION_FP64 = 0,    /* reference "double" path (gw_gpu_kernel.cpp:278-284, _sp = false) */
+  GGPU_PRECISION_FP32 = 1,    /* complex64 contractions (reference _sp = P_sp && Sigma_sp)        */
+  GGPU_PRECISION_BF16X3 = 2   /* complex64 data, split-bf16 x3 products, FP32 accumulation        */
+} ggpu_precision;
+
+/*
+ * Symmetry and integral-storage tables: the plain data green-symmetry
+ * provides (cu_symmetry.h:41-66, built at gw_gpu_kernel.cpp:34-91) plus the
+ * k-pair tables the integral reader uses (df_integral_t.h:68-86).
+ * Integer arrays are int64.  Optional members may be NULL:
+ *   U_k == NULL      -> identity AO representation for every k
+ *   U_q == NULL      -> identity auxiliary representation for every q
+ *   conj_kpair/trans_kpair/irre_pos == NULL -> every pair k1 >= k2 stored at
+ *                       its pair index k1(k1+1)/2 + k2
+ */
+typedef struct {
+  int32_t nk, ink, nq, inq;
+  const int64_t* k_full_to_reduced;  /* [nk]                                   */
+  const int64_t* k_reduced_to_full;  /* [ink]                                  */
+  const int64_t* k_tr_conj;          /* [nk]  non-zero: time-reversal image    */
+  const int64_t* k_star_offsets;     /* [ink+1]                                */
+  const int64_t* k_star_members;     /* [k_star_offsets[ink]] full-BZ k        */
+  const int64_t* q_full_to_reduced;  /* [nq]                                   */
+  const int64_t* q_reduced_to_full;  /* [inq]                                  */
+  const int64_t* q_tr_conj;          /* [nq]                                   */
+  const int64_t* q_star_offsets;     /* [inq+1]                                */
+  const int64_t* q_star_members;     /* [q_star_offsets[inq]]                  */
+  const int64_t* k1_from_k2q;        /* [nk*nq]  k1 = k2 + q, index k2*nq + q  */
+  const int64_t* k2_from_k1q;        /* [nk*nq]  k2 = k1 - q, index k1*nq + q  */
+  const double* U_k;                 /* [nk][nao][nao] complex, or NULL        */
+  const double* U_q;                 /* [nq][NQ][NQ] complex, or NULL          */
+  const int64_t* conj_kpair;         /* [nk(nk+1)/2] or NULL                   */
+  const int64_t* trans_kpair;        /* [nk(nk+1)/2] or NULL                   */
+  const int64_t* irre_pos;           /* [nk(nk+1)/2] stored position, -1 = not stored, or NULL */
+  int64_t num_kpair_stored;          /* number of stored pairs                 */
+} ggpu_symmetry;
+
+typedef struct ggpu_hf_plan ggpu_hf_plan;
+typedef struct ggpu_gw_plan ggpu_gw_plan;
+
+/* Message of the last failure on this thread ("" if none). */
+const char* ggpu_last_error(void);
+
+/* Number of visible GPUs (reference: check_for_cuda, cuda_check.cpp:29-47). */
+int ggpu_device_count(int* count);
+
+/* Bytes of device memory currently free / total on the current device. */
+int ggpu_device_memory(size_t* free_bytes, size_t* total_bytes);
+
+/* ------------------------------------------------------------------------
+ * HF exchange  (replaces scalar_hf_gpu_kernel::compute_exchange_selfenergy,
+ * hf_gpu_kernel.cpp:106-139, and cuhf_utils, cuhf_utils.cu:32-159)
+ * ------------------------------------------------------------------------ */
+int ggpu_hf_plan_create(ggpu_hf_plan** plan, int nao, int ns, int NQ, const ggpu_symmetry* sym);
+
+/* Copy the stored integral pairs (num_kpair_stored x NQ x nao x nao complex,
+ * the buffer df_integral_t::read_entire fills, df_integral_t.h:96-129) to the
+ * device.  The host array is not referenced after the call. */
+int ggpu_hf_load_integrals(ggpu_hf_plan* plan, const double* V_stored);
+
+/* Fill the device integral store with the synthetic counter-based generator
+ * (Re, Im ~ N(0, sigma^2); diagonal pairs Hermitised).  Benchmarks only. */
+int ggpu_hf_synth_integrals(ggpu_hf_plan* plan, uint64_t seed, double sigma);
+
+/* K_s(k) for the IBZ k:  K = (px/nk) sum_{k2,Q} V^Q(k,k2) D_s(k2) V^Q(k,k2)^dagger,
+ * px = -0.5 (ns = 1) or -1 (ns = 2).  dm: (ns, ink, nao, nao) on the IBZ (the
+ * library unfolds it to the full BZ, hf_gpu_kernel.cpp:203-209).
+ * K_out: (ns, ink, nao, nao), overwritten. */
+int ggpu_hf_exchange(ggpu_hf_plan* plan, const double* dm, double* K_out);
+
+/* Full HF self-energy, as scalar_hf_gpu_kernel::solve returns it
+ * (hf_gpu_kernel.cpp:66-104, without the MPI reduction):
+ *   F = J + K - c * madelung * S D S,   c = 1 (ns = 2) or 0.5 (ns = 1),
+ *   J(k) = sum_Q V^Q(k,k) (1/nk) sum_{s,k'} Tr[V^Q(k',k') D_s(k')]  (hf_gpu_kernel.cpp:141-186).
+ * dm, S_k, F_out: (ns, ink, nao, nao) on the IBZ. */
+int ggpu_hf_solve(ggpu_hf_plan* plan, const double* dm, const double* S_k, double madelung, double* F_out);
+
+void ggpu_hf_plan_destroy(ggpu_hf_plan* plan);
+
+/* ------------------------------------------------------------------------
+ * GW self-energy  (replaces scalar_gw_gpu_kernel::compute_gw_selfenergy,
+ * gw_gpu_kernel.cpp:294-360, and cugw_utils::accumulate_gw_selfenergy_on_device,
+ * cugw_utils.cu:103-202)
+ * ------------------------------------------------------------------------ */
+/* T_wt: (nw_b, nts-2) complex  (transformer_t::Tnt_BF, cugw_qpt.cu:329-339)
+ * T_tw: (nts, nw_b)   complex  (transformer_t::Ttn_FB, cugw_qpt.cu:341-358)
+ * nts must be even and nts >= nw_b (gw_gpu_kernel.h:67, cugw_qpt.cu:79).
+ * work_bytes: device workspace budget for pair batches (0 = automatic). */
+int ggpu_gw_plan_create(ggpu_gw_plan** plan, int nao, int ns, int NQ, int nts, int nw_b,
+                        const ggpu_symmetry* sym, const double* T_wt, const double* T_tw,
+                        int precision, int solver, size_t work_bytes);
+
+int ggpu_gw_load_integrals(ggpu_gw_plan* plan, const double* V_stored);
+int ggpu_gw_synth_integrals(ggpu_gw_plan* plan, uint64_t seed, double sigma);
+
+/* One GW iteration: Sigma(tau) from G(tau).
+ * G, Sigma_out: (nts, ns, ink, nao, nao) complex, Sigma_out overwritten
+ * (already divided by nk, as gw_gpu_kernel.cpp:188 leaves it). */
+int ggpu_gw_iterate(ggpu_gw_plan* plan, const double* G, double* Sigma_out);
+
+/* Same, with G and Sigma already in device memory (HBM-resident inputs). */
+int ggpu_gw_iterate_device(ggpu_gw_plan* plan, const void* G_device, void* Sigma_device);
+
+/* Intermediates of one IBZ q for error reports and tests:
+ * which = 0: P0(tau)   (nts, NQ, NQ)   bare bubble, scaled by 1/nk and mirrored
+ * which = 1: P(Omega)  (nw_b, NQ, NQ)  after the Dyson solve
+ * which = 2: P(tau)    (nts, NQ, NQ)   dressed polarization */
+int ggpu_gw_polarization(ggpu_gw_plan* plan, const double* G, int iq, int which, double* out);
+
+/* Wall time (ms, device events) of the phases of the last iterate:
+ * [0] total, [1] G staging, [2] P0, [3] IR + Dyson, [4] Sigma, [5] Sigma staging. */
+int ggpu_gw_last_timings(ggpu_gw_plan* plan, double* ms6);
+
+void ggpu_gw_plan_destroy(ggpu_gw_plan* plan);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GREEN_GPU_AMD_H */
