@@ -1,0 +1,190 @@
+#!/usr/bin/env python3
+"""GW-iteration benchmark on one MI355X.
+
+A step is one full GW iteration (P0 bubble for every q and k, tau -> Omega,
+Dyson solve, Omega -> tau, Sigma contraction, Sigma staging) on synthetic
+inputs already resident in HBM.  Default workload: nk = 4x4x4, nao = 52,
+NQ = 300, ns = 2, beta = 100, sparse-IR Lambda = 1e4 (the largest of the
+reference benchmark shapes whose integrals fit one GPU).
+
+Prints one JSON line (rank 0).  `value` is the algorithmic TFLOP/s of the
+iteration (complex multiply-add = 8 FLOP, minimal contraction set).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "GW iteration wall-time (s) + TFLOP/s, mixed vs FP64 on MI355X at Σ err≤1e-6"
+PEAK = {"f64": 78.6, "f32": 157.3, "bf16x3": 2500.0}      # TFLOP/s, MI355X dense matrix
+HBM_PEAK = 8000.0                                           # GB/s
+
+WORKLOADS = {
+    # the reference's small CPU-runnable system
+    "gw_nk8_nao26_nq100": dict(mesh=(2, 2, 2), nao=26, NQ=100, ns=1, beta=50.0, lam=1e3),
+    # the plain-FP64 / mixed-precision single-GPU system
+    "gw_nk64_nao52_nq300": dict(mesh=(4, 4, 4), nao=52, NQ=300, ns=2, beta=100.0, lam=1e4),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def gw_flops(nk, nao, NQ, ns, nts, nw_b):
+    c = 16.0 * NQ * nao ** 3 + 8.0 * NQ * NQ * nao * nao
+    p0 = nk * nk * ns * (nts // 2) * c
+    sig = nk * nk * ns * nts * c
+    ir = nk * 8.0 * NQ * NQ * (nw_b * (nts - 2) + nts * nw_b)
+    dyson = nk * nw_b * (4.0 / 3.0 + 8.0) * NQ ** 3
+    return {"p0": p0, "sigma": sig, "ir": ir, "dyson": dyson, "total": p0 + sig + ir + dyson}
+
+
+def cpu_baseline(w, G, V_pairs, T_wt, T_tw, sym, npairs=2):
+    """Oracle (numpy/OpenBLAS) on a bounded sample of the same iteration:
+    q = 0, P0 and Sigma contributions of `npairs` k-points, one Dyson solve.
+    Returns the sustained FLOP rate; the iteration time is extrapolated."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import gw_oracle as O
+    try:
+        from threadpoolctl import threadpool_info
+        cores = max([i.get("num_threads", 1) for i in threadpool_info() if i.get("user_api") == "blas"] or [1])
+    except Exception:
+        cores = int(os.environ.get("OMP_NUM_THREADS", "1"))
+    nao, NQ, ns = w["nao"], w["NQ"], w["ns"]
+    nts = T_tw.shape[0]
+    half = nts // 2
+    q = 0
+    flops = 0.0
+    t0 = time.perf_counter()
+    P0 = np.zeros((nts, NQ, NQ), dtype=np.complex128)
+    for k in range(npairs):
+        k1 = int(sym.k1_from_k2q[k, q])
+        V = V_pairs[(k, k1)]
+        for s in range(ns):
+            Gm = G[nts - 1 - np.arange(half), s, k]
+            G1 = G[:half, s, k1]
+            P0[:half] += O.p0_pair(Gm, G1, V, -1.0 if ns == 2 else -2.0)
+            flops += half * (16.0 * NQ * nao ** 3 + 8.0 * NQ * NQ * nao * nao)
+    P0[nts - 1 - np.arange(half)] = P0[:half]
+    P0w = O.tau_to_omega(P0, T_wt)
+    Pw = O.gw_dyson(P0w, "cholesky")
+    Pt = O.omega_to_tau(Pw, T_tw)
+    nw_b = T_wt.shape[0]
+    flops += 8.0 * NQ * NQ * (nw_b * (nts - 2) + nts * nw_b) + nw_b * (4.0 / 3.0 + 8.0) * NQ ** 3
+    for k in range(npairs):
+        k1 = int(sym.k2_from_k1q[k, q])
+        V = V_pairs[(k, k1)]
+        for s in range(ns):
+            O.sigma_pair(Pt, G[:, s, k1], V)
+            flops += nts * (16.0 * NQ * nao ** 3 + 8.0 * NQ * NQ * nao * nao)
+    dt = time.perf_counter() - t0
+    return flops / dt, dt, cores, flops
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="gw_nk64_nao52_nq300", choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16x3"])
+    ap.add_argument("--solver", default="cholesky", choices=["cholesky", "lu"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
+    ap.add_argument("--cpu-pairs", type=int, default=2)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
+                          "n_gpus": a.gpus}))
+        sys.exit(2)
+
+    import torch
+    import __graft_entry__ as ge
+    pkg = ge._load_package()
+    import green_gpu_amd.synth as synth
+
+    w = WORKLOADS[a.workload]
+    sym = pkg.mesh_symmetry_none(w["mesh"])
+    nk, nao, NQ, ns = sym.nk, w["nao"], w["NQ"], w["ns"]
+    tau, T_wt, T_tw = synth.load_ir(w["beta"], w["lam"])
+    nts, nw_b = tau.size, T_wt.shape[0]
+    log(f"workload {a.workload}: nk={nk} nao={nao} NQ={NQ} ns={ns} nts={nts} nw_b={nw_b}")
+    t = time.perf_counter()
+    G, _ = synth.make_green(nk, nao, ns, w["beta"], tau, 1000)
+    log(f"G generated in {time.perf_counter() - t:.1f}s")
+
+    kern = pkg.GwKernel(nao, ns, NQ, T_wt, T_tw, sym, precision=a.precision, solver=a.solver)
+    seed, sigma = 1017, 0.1
+    kern.synth_integrals(seed, sigma)
+    dev = torch.device("cuda:0")
+    Gd = torch.from_numpy(G.view(np.float64)).to(dev)
+    Sd = torch.empty_like(Gd)
+    torch.cuda.synchronize()
+    log("inputs resident in HBM")
+
+    for i in range(a.warmup):
+        t = time.perf_counter()
+        kern.solve_device(Gd.data_ptr(), Sd.data_ptr())
+        torch.cuda.synchronize()
+        log(f"warmup step {i}: {time.perf_counter() - t:.2f}s")
+    kern.set_profiling(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        kern.solve_device(Gd.data_ptr(), Sd.data_ptr())
+        log(f"step {i} done")
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / a.steps
+    kern.set_profiling(False)
+    stats = kern.site_stats()
+    phases = kern.timings()
+
+    fl = gw_flops(nk, nao, NQ, ns, nts, nw_b)
+    tflops = fl["total"] / dt / 1e12
+    dtype = {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16x3"}[a.precision]
+    dom = max(stats.items(), key=lambda kv: kv[1][0])
+    dom_name, (dom_ms, dom_fl, dom_n) = dom
+    achieved = dom_fl / (dom_ms * 1e-3) / 1e12
+    roofline = {"bound": "mfma", "kernel": f"gemm site {dom_name}", "achieved": round(achieved, 3),
+                "peak": PEAK[dtype], "unit": "TFLOP/s", "frac": round(achieved / PEAK[dtype], 4),
+                "traffic": None, "launches": dom_n, "avg_launch_ms": round(dom_ms / dom_n, 4)}
+    out = {
+        "metric": METRIC, "value": round(tflops, 3), "unit": "TFLOP/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(dt * 1e3, 1),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+        "data": "synthetic (seeded random H(k) -> G(tau); Gaussian V, sigma 0.1)",
+        "config": {"workload": a.workload, "nk": nk, "nao": nao, "NQ": NQ, "ns": ns, "nts": nts,
+                   "nw_b": nw_b, "beta": w["beta"], "ir_lambda": w["lam"], "precision": a.precision,
+                   "solver": a.solver},
+        "gw_iteration_s": round(dt, 3),
+        "gw_flop_per_iteration": fl["total"],
+        "phases_ms": {k: round(v, 1) for k, v in phases.items()},
+        "sites": {k: {"ms": round(v[0], 1), "tflops": round(v[1] / (v[0] * 1e-3) / 1e12, 2), "launches": v[2]}
+                  for k, v in stats.items()},
+        "roofline": roofline,
+    }
+    if not a.no_cpu:
+        pairs = {}
+        for k in range(a.cpu_pairs):
+            for k1 in (int(sym.k1_from_k2q[k, 0]), int(sym.k2_from_k1q[k, 0])):
+                lo, hi = min(k, k1), max(k, k1)
+                p = hi * (hi + 1) // 2 + lo
+                S = synth.make_integrals(nk, NQ, nao, seed, sigma, pairs=[p])[0]
+                pairs[(k, k1)] = S if k >= k1 else np.conj(np.swapaxes(S, -1, -2))
+        rate, cdt, cores, cfl = cpu_baseline(w, G, pairs, T_wt, T_tw, sym, a.cpu_pairs)
+        out["cpu_baseline"] = {
+            "value": round(rate / 1e12, 5), "unit": "TFLOP/s", "cores": cores, "kind": "port",
+            "sample": f"oracle (numpy/OpenBLAS, FP64) on q=0: P0 and Sigma of {a.cpu_pairs} k-points + one "
+                      f"Dyson solve ({cfl:.3g} FLOP in {cdt:.1f}s); extrapolated iteration "
+                      f"{fl['total'] / rate:.0f}s"}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

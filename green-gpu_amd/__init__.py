@@ -60,6 +60,8 @@ def lib():
         L.ggpu_gw_iterate_device.argtypes = [vp, vp, vp]
         L.ggpu_gw_polarization.argtypes = [vp, pd, i32, i32, pd]
         L.ggpu_gw_last_timings.argtypes = [vp, pd]
+        L.ggpu_gw_set_profiling.argtypes = [vp, i32]
+        L.ggpu_gw_site_stats.argtypes = [vp, i32, pd, pd, C.POINTER(C.c_int64)]
         L.ggpu_gw_plan_destroy.argtypes = [vp]
         L.ggpu_gw_plan_destroy.restype = None
         _lib = L
@@ -71,7 +73,11 @@ EXPORTED = ["ggpu_last_error", "ggpu_device_count", "ggpu_device_memory",
             "ggpu_hf_exchange", "ggpu_hf_solve", "ggpu_hf_plan_destroy",
             "ggpu_gw_plan_create", "ggpu_gw_load_integrals", "ggpu_gw_synth_integrals",
             "ggpu_gw_iterate", "ggpu_gw_iterate_device", "ggpu_gw_polarization",
-            "ggpu_gw_last_timings", "ggpu_gw_plan_destroy"]
+            "ggpu_gw_last_timings", "ggpu_gw_set_profiling", "ggpu_gw_site_stats",
+            "ggpu_gw_plan_destroy"]
+
+SITES = ["other", "p0_gv", "p0_gvh", "p0_contract", "sig_vg", "sig_py", "sig_contract",
+         "ir", "dyson", "hf", "rotate"]
 
 
 def _check(rc):
@@ -312,6 +318,19 @@ class GwKernel:
         t = np.zeros(6)
         _check(lib().ggpu_gw_last_timings(self._h, _dptr(t)))
         return dict(zip(["total", "stage_G", "p0", "ir_dyson", "sigma", "stage_sigma"], t))
+
+    def set_profiling(self, on=True):
+        _check(lib().ggpu_gw_set_profiling(self._h, 1 if on else 0))
+
+    def site_stats(self):
+        """{site: (ms, algorithmic FLOPs, launches)} of the last profiled iteration."""
+        out = {}
+        for i, name in enumerate(SITES):
+            ms, fl, n = C.c_double(), C.c_double(), C.c_int64()
+            _check(lib().ggpu_gw_site_stats(self._h, i, C.byref(ms), C.byref(fl), C.byref(n)))
+            if n.value:
+                out[name] = (ms.value, fl.value, n.value)
+        return out
 
     def close(self):
         if getattr(self, "_h", None):

@@ -162,6 +162,23 @@ int ggpu_gw_last_timings(ggpu_gw_plan* plan, double* ms6) {
   });
 }
 
+int ggpu_gw_set_profiling(ggpu_gw_plan* plan, int enable) {
+  return guarded([&] {
+    ggpu::require(plan && plan->impl, "NULL plan");
+    plan->impl->profiling = enable != 0;
+  });
+}
+
+int ggpu_gw_site_stats(ggpu_gw_plan* plan, int site, double* ms, double* flops, int64_t* launches) {
+  return guarded([&] {
+    ggpu::require(plan && plan->impl && ms && flops && launches, "NULL argument");
+    ggpu::require(site >= 0 && site < ggpu::kNumSites, "site out of range");
+    *ms = plan->impl->prof.ms[site];
+    *flops = plan->impl->prof.flops[site];
+    *launches = plan->impl->prof.launches[site];
+  });
+}
+
 void ggpu_gw_plan_destroy(ggpu_gw_plan* plan) {
   if (!plan) return;
   delete plan->impl;

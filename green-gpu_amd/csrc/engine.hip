@@ -19,7 +19,8 @@ namespace ggpu {
 // Launch a GEMM with every operand tied to its allocation, so that the
 // host-side bounds check covers the whole launch.
 template <class BA, class BB, class BC>
-static void gemm_on(GemmDesc g, const BA& a, const BB& b, const BC& c, hipStream_t s) {
+static void gemm_on(GemmDesc g, const BA& a, const BB& b, const BC& c, hipStream_t s, int site = -1) {
+  if (site >= 0) g.site = site;
   g.A.buf = a.get(); g.A.buf_elems = (int64_t)a.size();
   g.B.buf = b.get(); g.B.buf_elems = (int64_t)b.size();
   g.C.buf = c.get(); g.C.buf_elems = (int64_t)c.size();
@@ -255,14 +256,54 @@ void FullBZRotator::run(const z64* X, z64* out, hipStream_t s) {
     g.B.ptr = X + t.k_f2r[k] * n2; g.B.rs = n_; g.B.cs = 1;
     g.B.bs0 = (int64_t)ns_ * t.ink * n2; g.B.bs1 = (int64_t)t.ink * n2; g.B.conj = tr;
     g.C.ptr = tmp_.get(); g.C.rs = n_; g.C.cs = 1; g.C.bs0 = n2; g.C.bs1 = (int64_t)nt_ * n2;
-    gemm_on(g, Uk_, Span{X, in_elems}, tmp_, s);
+    gemm_on(g, Uk_, Span{X, in_elems}, tmp_, s, kSiteRotate);
     GemmDesc h;
     h.M = n_; h.N = n_; h.K = n_; h.nb0 = nt_; h.nb1 = ns_;
     h.A.ptr = tmp_.get(); h.A.rs = n_; h.A.cs = 1; h.A.bs0 = n2; h.A.bs1 = (int64_t)nt_ * n2;
     h.B.ptr = Uk_.get() + k * n2; h.B.rs = 1; h.B.cs = n_; h.B.conj = !tr;
     h.C.ptr = out + (int64_t)k * ns_ * nt_ * n2; h.C.rs = n_; h.C.cs = 1; h.C.bs0 = n2; h.C.bs1 = (int64_t)nt_ * n2;
-    gemm_on(h, tmp_, Uk_, Span{out, out_elems}, s);
+    gemm_on(h, tmp_, Uk_, Span{out, out_elems}, s, kSiteRotate);
   }
+}
+
+// ---------------------------------------------------------------------------
+// Per-site GEMM timing
+// ---------------------------------------------------------------------------
+SiteProfiler::~SiteProfiler() {
+  for (auto e : pool_) (void)hipEventDestroy(e);
+}
+
+void SiteProfiler::before(const GemmDesc& d, hipStream_t s) {
+  while (pool_.size() < used_ + 2) {
+    hipEvent_t e;
+    GGPU_HIP(hipEventCreate(&e));
+    pool_.push_back(e);
+  }
+  GGPU_HIP(hipEventRecord(pool_[used_], s));
+  recs_.push_back(Rec{d.site, gemm_flops(d), used_});
+  used_ += 2;
+}
+
+void SiteProfiler::after(const GemmDesc&, hipStream_t s) {
+  GGPU_HIP(hipEventRecord(pool_[recs_.back().ev + 1], s));
+}
+
+void SiteProfiler::reset() {
+  used_ = 0;
+  recs_.clear();
+  for (int i = 0; i < kNumSites; ++i) { ms[i] = 0; flops[i] = 0; launches[i] = 0; }
+}
+
+void SiteProfiler::collect() {
+  for (const Rec& r : recs_) {
+    float t = 0;
+    GGPU_HIP(hipEventElapsedTime(&t, pool_[r.ev], pool_[r.ev + 1]));
+    ms[r.site] += t;
+    flops[r.site] += r.flops;
+    launches[r.site] += 1;
+  }
+  recs_.clear();
+  used_ = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -378,6 +419,7 @@ GwPlan::GwPlan(int nao_, int ns_, int NQ_, int nts_, int nw_b_, const ggpu_symme
 GwPlan::~GwPlan() {
   if (stream) (void)hipStreamSynchronize(stream);
   for (auto& e : ev) (void)hipEventDestroy(e);
+  for (auto& e : qev) (void)hipEventDestroy(e);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -419,12 +461,14 @@ void GwPlan::p0_for_q(int iq) {
         g.A.ptr = Gf.get() + ((k * ns + s) * nts + (nts - 1)) * n2; g.A.rs = nao; g.A.cs = 1; g.A.bs0 = -n2;
         g.B.ptr = VpmQ + b * stride; g.B.rs = (int64_t)nao * NQ; g.B.cs = 1;
         g.C.ptr = X1 + (b * ns + s) * xs; g.C.rs = (int64_t)nao * NQ; g.C.cs = 1; g.C.bs0 = stride;
+        g.site = kSiteP0X1;
         gemm_on(g, Gf, work, work, stream);
         GemmDesc h;
         h.M = NQ * nao; h.N = nao; h.K = nao; h.nb0 = half;
         h.A.ptr = Vdir + b * stride; h.A.rs = nao; h.A.cs = 1; h.A.conj = 1;
         h.B.ptr = Gf.get() + (k1 * ns + s) * nts * n2; h.B.rs = 1; h.B.cs = nao; h.B.bs0 = n2;
         h.C.ptr = X2 + (b * ns + s) * xs; h.C.rs = nao; h.C.cs = 1; h.C.bs0 = stride;
+        h.site = kSiteP0X2;
         gemm_on(h, work, Gf, work, stream);
       }
     }
@@ -434,6 +478,7 @@ void GwPlan::p0_for_q(int iq) {
     p.B.ptr = X2; p.B.rs = 1; p.B.cs = n2; p.B.kbs = xs; p.B.bs0 = stride;
     p.C.ptr = P0.get(); p.C.rs = NQ; p.C.cs = 1; p.C.bs0 = (int64_t)NQ * NQ;
     p.alpha_re = pref; p.beta_re = 1.0;
+    p.site = kSiteP0;
     gemm_on(p, work, work, P0, stream);
   }
 }
@@ -449,6 +494,7 @@ void GwPlan::dyson_for_q() {
     g.B.ptr = P0.get() + Q2; g.B.rs = Q2; g.B.cs = 1;
     g.C.ptr = P0w.get(); g.C.rs = Q2; g.C.cs = 1;
     g.alpha_re = 1.0 / sym.nk;
+    g.site = kSiteIR;
     gemm_on(g, Tf_wt, P0, P0w, stream);
   }
   herm_one_minus(P0w.get(), Amat.get(), n, nw_b, stream);
@@ -466,6 +512,7 @@ void GwPlan::dyson_for_q() {
       h.B.ptr = Amat.get() + (int64_t)(j0 + jb) * n + j0; h.B.rs = 1; h.B.cs = n; h.B.bs0 = Q2; h.B.conj = 1;
       h.C.ptr = Amat.get() + (int64_t)(j0 + jb) * n + (j0 + jb); h.C.rs = n; h.C.cs = 1; h.C.bs0 = Q2;
       h.alpha_re = -1.0; h.beta_re = 1.0;
+      h.site = kSiteDyson;
       gemm_on(h, Amat, Amat, Amat, stream);
     }
   }
@@ -479,6 +526,7 @@ void GwPlan::dyson_for_q() {
       g.B.ptr = P0w.get(); g.B.rs = n; g.B.cs = 1; g.B.bs0 = Q2;
       g.C.ptr = P0w.get() + (int64_t)i0 * n; g.C.rs = n; g.C.cs = 1; g.C.bs0 = Q2;
       g.alpha_re = -1.0; g.beta_re = 1.0;
+      g.site = kSiteDyson;
       gemm_on(g, Amat, P0w, P0w, stream);
     }
     trsm_lower_block(Amat.get(), P0w.get(), n, n, nw_b, i0, ib, stream);
@@ -494,6 +542,7 @@ void GwPlan::dyson_for_q() {
       g.B.ptr = P0w.get() + (int64_t)(i0 + ib) * n; g.B.rs = n; g.B.cs = 1; g.B.bs0 = Q2;
       g.C.ptr = P0w.get() + (int64_t)i0 * n; g.C.rs = n; g.C.cs = 1; g.C.bs0 = Q2;
       g.alpha_re = -1.0; g.beta_re = 1.0;
+      g.site = kSiteDyson;
       gemm_on(g, Amat, P0w, P0w, stream);
     }
     trsm_upper_h_block(Amat.get(), P0w.get(), n, n, nw_b, i0, ib, stream);
@@ -505,6 +554,7 @@ void GwPlan::dyson_for_q() {
     g.A.ptr = T_tw.get(); g.A.rs = nw_b; g.A.cs = 1;
     g.B.ptr = P0w.get(); g.B.rs = Q2; g.B.cs = 1;
     g.C.ptr = Pt.get(); g.C.rs = Q2; g.C.cs = 1;
+    g.site = kSiteIR;
     gemm_on(g, T_tw, P0w, Pt, stream);
   }
 }
@@ -531,12 +581,14 @@ void GwPlan::sigma_for_q(int iq) {
       a.A.ptr = Uq_dev.get() + qd * Q2; a.A.rs = NQ; a.A.cs = 1;
       a.B.ptr = Pt.get(); a.B.rs = NQ; a.B.cs = 1; a.B.bs0 = Q2;
       a.C.ptr = Ptmp.get(); a.C.rs = NQ; a.C.cs = 1; a.C.bs0 = Q2;
+      a.site = kSiteRotate;
       gemm_on(a, Uq_dev, Pt, Ptmp, stream);
       GemmDesc c;
       c.M = NQ; c.N = NQ; c.K = NQ; c.nb0 = nts;
       c.A.ptr = Ptmp.get(); c.A.rs = NQ; c.A.cs = 1; c.A.bs0 = Q2;
       c.B.ptr = Uq_dev.get() + qd * Q2; c.B.rs = 1; c.B.cs = NQ; c.B.conj = 1;
       c.C.ptr = Prot.get(); c.C.rs = NQ; c.C.cs = 1; c.C.bs0 = Q2;
+      c.site = kSiteRotate;
       gemm_on(c, Ptmp, Uq_dev, Prot, stream);
       Pq = Prot.get();
     }
@@ -554,12 +606,14 @@ void GwPlan::sigma_for_q(int iq) {
           g.A.ptr = Vdir + b * stride; g.A.rs = nao; g.A.cs = 1;
           g.B.ptr = Gf.get() + (k1 * ns + s) * nts * n2; g.B.rs = nao; g.B.cs = 1; g.B.bs0 = n2;
           g.C.ptr = Y1 + (b * ns + s) * ys; g.C.rs = nao; g.C.cs = 1; g.C.bs0 = stride;
+          g.site = kSiteSigY1;
           gemm_on(g, work, Gf, work, stream);
           GemmDesc h;
           h.M = NQ; h.N = (int)n2; h.K = NQ; h.nb0 = nts;
           h.A.ptr = Pq; h.A.rs = 1; h.A.cs = NQ; h.A.bs0 = Q2; h.A.conj = tr;
           h.B.ptr = Y1 + (b * ns + s) * ys; h.B.rs = n2; h.B.cs = 1; h.B.bs0 = stride;
           h.C.ptr = Y2 + (b * ns + s) * ys; h.C.rs = (int64_t)nts * n2; h.C.cs = 1; h.C.bs0 = n2;
+          h.site = kSiteSigY2;
           gemm_on(h, Pbuf, work, work, stream);
         }
       }
@@ -570,6 +624,7 @@ void GwPlan::sigma_for_q(int iq) {
       f.C.ptr = Sig_acc.get() + (int64_t)i0 * ns * nts * n2; f.C.rs = nao; f.C.cs = 1;
       f.C.bs0 = (int64_t)ns * nts * n2; f.C.bs1 = (int64_t)nts * n2;
       f.alpha_re = -1.0; f.beta_re = 1.0;
+      f.site = kSiteSigma;
       gemm_on(f, work, work, Sig_acc, stream);
     }
   }
@@ -587,37 +642,49 @@ void GwPlan::check_info() {
 void GwPlan::iterate_device(const z64* G_dev, z64* S_dev) {
   require(store.loaded, "gw: integrals not loaded", kErrState);
   const int64_t n2 = (int64_t)nao * nao;
-  std::vector<hipEvent_t> evs;
+  while (qev.size() < (size_t)4 * sym.inq) {
+    hipEvent_t e;
+    GGPU_HIP(hipEventCreate(&e));
+    qev.push_back(e);
+  }
+  struct Guard {
+    bool on;
+    explicit Guard(GemmProfiler* p) : on(p != nullptr) { set_gemm_profiler(p); }
+    ~Guard() { set_gemm_profiler(nullptr); }
+  } guard(profiling ? &prof : nullptr);
+  if (profiling) prof.reset();
   GGPU_HIP(hipEventRecord(ev[0], stream));
   stage_G(G_dev);
   GGPU_HIP(hipEventRecord(ev[1], stream));
   zero(Sig_acc.get(), Sig_acc.size(), stream);
   zero_int(info.get(), nw_b, stream);
-  double t_p0 = 0, t_dy = 0, t_sig = 0;
   for (int iq = 0; iq < sym.inq; ++iq) {
-    GGPU_HIP(hipEventRecord(ev[2], stream));
+    GGPU_HIP(hipEventRecord(qev[4 * iq + 0], stream));
     p0_for_q(iq);
-    GGPU_HIP(hipEventRecord(ev[3], stream));
+    GGPU_HIP(hipEventRecord(qev[4 * iq + 1], stream));
     dyson_for_q();
-    GGPU_HIP(hipEventRecord(ev[4], stream));
+    GGPU_HIP(hipEventRecord(qev[4 * iq + 2], stream));
     sigma_for_q(iq);
-    GGPU_HIP(hipEventRecord(ev[5], stream));
-    GGPU_HIP(hipEventSynchronize(ev[5]));
-    float a = 0, b = 0, c = 0;
-    GGPU_HIP(hipEventElapsedTime(&a, ev[2], ev[3]));
-    GGPU_HIP(hipEventElapsedTime(&b, ev[3], ev[4]));
-    GGPU_HIP(hipEventElapsedTime(&c, ev[4], ev[5]));
-    t_p0 += a; t_dy += b; t_sig += c;
+    GGPU_HIP(hipEventRecord(qev[4 * iq + 3], stream));
   }
   GGPU_HIP(hipEventRecord(ev[5], stream));
   slab_gather(Sig_acc.get(), sig_src.get(), S_dev, sig_dst.get(), (int)sig_src.size(), n2, 1.0 / sym.nk, stream);
   GGPU_HIP(hipEventRecord(ev[6], stream));
   GGPU_HIP(hipStreamSynchronize(stream));
+  double t_p0 = 0, t_dy = 0, t_sig = 0;
+  for (int iq = 0; iq < sym.inq; ++iq) {
+    float a = 0, b = 0, c = 0;
+    GGPU_HIP(hipEventElapsedTime(&a, qev[4 * iq + 0], qev[4 * iq + 1]));
+    GGPU_HIP(hipEventElapsedTime(&b, qev[4 * iq + 1], qev[4 * iq + 2]));
+    GGPU_HIP(hipEventElapsedTime(&c, qev[4 * iq + 2], qev[4 * iq + 3]));
+    t_p0 += a; t_dy += b; t_sig += c;
+  }
   float tot = 0, g = 0, sg = 0;
   GGPU_HIP(hipEventElapsedTime(&tot, ev[0], ev[6]));
   GGPU_HIP(hipEventElapsedTime(&g, ev[0], ev[1]));
   GGPU_HIP(hipEventElapsedTime(&sg, ev[5], ev[6]));
   last_ms[0] = tot; last_ms[1] = g; last_ms[2] = t_p0; last_ms[3] = t_dy; last_ms[4] = t_sig; last_ms[5] = sg;
+  if (profiling) prof.collect();
   check_info();
 }
 
@@ -736,7 +803,7 @@ void HfPlan::exchange_device() {
         g.A.ptr = Vdir + b * stride; g.A.rs = nao; g.A.cs = 1;
         g.B.ptr = dm_fbz.get() + k2 * ns * n2; g.B.rs = nao; g.B.cs = 1; g.B.bs0 = n2;
         g.C.ptr = X + b * stride; g.C.rs = nao; g.C.cs = 1; g.C.bs0 = xs;
-        gemm_on(g, work, dm_fbz, work, stream);
+        gemm_on(g, work, dm_fbz, work, stream, kSiteHF);
       }
       const int total = n * NQ;
       const int ch = (total + nsplit - 1) / nsplit;
@@ -754,7 +821,7 @@ void HfPlan::exchange_device() {
         f.C.ptr = Kpart.get() + (part == 0 ? 0 : (int64_t)full * ns * n2); f.C.rs = nao; f.C.cs = 1;
         f.C.bs0 = n2; f.C.bs1 = (int64_t)ns * n2;
         f.alpha_re = px / sym.nk; f.beta_re = 1.0;
-        gemm_on(f, work, work, Kpart, stream);
+        gemm_on(f, work, work, Kpart, stream, kSiteHF);
       }
     }
     sum_partials(Kpart.get(), (int64_t)ns * n2, nsplit, Kout.get() + (int64_t)ik * ns * n2, (int64_t)ns * n2, stream);
@@ -803,7 +870,7 @@ void HfPlan::solve(const double* dm, const double* S_k, double madelung, double*
         g.B.ptr = dm_fbz.get() + ((int64_t)(i0 + b) * ns + s) * n2; g.B.rs = nao; g.B.cs = 0; g.B.kbs = 1;
         g.C.ptr = jvec.get(); g.C.rs = 1; g.C.cs = 1;
         g.alpha_re = 1.0 / sym.nk; g.beta_re = 1.0;
-        gemm_on(g, work, dm_fbz, jvec, stream);
+        gemm_on(g, work, dm_fbz, jvec, stream, kSiteHF);
       }
   }
   // F[ik][s] = K + sum_Q jvec[Q] V^Q(k,k) - c madelung S D S
@@ -818,7 +885,7 @@ void HfPlan::solve(const double* dm, const double* S_k, double madelung, double*
       g.B.ptr = Vdir + b * stride; g.B.rs = n2; g.B.cs = 1;
       g.C.ptr = Kout.get() + (int64_t)(i0 + b) * ns * n2; g.C.rs = 0; g.C.cs = 1; g.C.bs0 = n2;
       g.beta_re = 1.0;
-      gemm_on(g, jvec, work, Kout, stream);
+      gemm_on(g, jvec, work, Kout, stream, kSiteHF);
     }
   }
   // Ewald: Ftmp[s][ik] = S D, then Kout[ik][s] -= c madelung (S D) S
@@ -828,7 +895,7 @@ void HfPlan::solve(const double* dm, const double* S_k, double madelung, double*
     g.A.ptr = S_dev.get(); g.A.rs = nao; g.A.cs = 1; g.A.bs0 = n2;
     g.B.ptr = dm_host_dev.get(); g.B.rs = nao; g.B.cs = 1; g.B.bs0 = n2;
     g.C.ptr = Ftmp.get(); g.C.rs = nao; g.C.cs = 1; g.C.bs0 = n2;
-    gemm_on(g, S_dev, dm_host_dev, Ftmp, stream);
+    gemm_on(g, S_dev, dm_host_dev, Ftmp, stream, kSiteHF);
     for (int s = 0; s < ns; ++s) {
       GemmDesc h;
       h.M = nao; h.N = nao; h.K = nao; h.nb0 = sym.ink;
@@ -836,7 +903,7 @@ void HfPlan::solve(const double* dm, const double* S_k, double madelung, double*
       h.B.ptr = S_dev.get() + (int64_t)s * sym.ink * n2; h.B.rs = nao; h.B.cs = 1; h.B.bs0 = n2;
       h.C.ptr = Kout.get() + (int64_t)s * n2; h.C.rs = nao; h.C.cs = 1; h.C.bs0 = (int64_t)ns * n2;
       h.alpha_re = -c * madelung; h.beta_re = 1.0;
-      gemm_on(h, Ftmp, S_dev, Kout, stream);
+      gemm_on(h, Ftmp, S_dev, Kout, stream, kSiteHF);
     }
   }
   std::vector<z64> h(Kout.size());

@@ -70,6 +70,25 @@ class FullBZRotator {
   DeviceBuffer<z64> Uk_, tmp_;
 };
 
+// Times every GEMM launch with a pair of events on its stream and sums
+// time, algorithmic FLOPs and launch counts per call site.
+class SiteProfiler : public GemmProfiler {
+ public:
+  ~SiteProfiler() override;
+  void before(const GemmDesc& d, hipStream_t s) override;
+  void after(const GemmDesc& d, hipStream_t s) override;
+  void reset();
+  void collect();   // after the stream has been synchronised
+  double ms[kNumSites] = {}, flops[kNumSites] = {};
+  int64_t launches[kNumSites] = {};
+
+ private:
+  struct Rec { int site; double flops; size_t ev; };
+  std::vector<hipEvent_t> pool_;
+  std::vector<Rec> recs_;
+  size_t used_ = 0;
+};
+
 struct GwPlan {
   int nao, ns, NQ, nts, nw_b, half;
   Compute compute;
@@ -92,6 +111,9 @@ struct GwPlan {
   FullBZRotator rot;
   int kb = 1;                           // pairs per batch
   double last_ms[6] = {0, 0, 0, 0, 0, 0};
+  bool profiling = false;
+  SiteProfiler prof;
+  std::vector<hipEvent_t> qev;          // per-q phase events
 
   GwPlan(int nao, int ns, int NQ, int nts, int nw_b, const ggpu_symmetry& s, const double* T_wt,
          const double* T_tw, int precision, int solver, size_t work_bytes);

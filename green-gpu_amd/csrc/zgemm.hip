@@ -59,7 +59,7 @@ template <> __device__ inline int acc_row<float>(int lk, int r) { return 4 * lk 
 // T: arithmetic type of operands and MFMA; TO: complex element type of C.
 // AKC: A's k index is the contiguous one (else its m index).
 // BKC: B's k index is the contiguous one (else its n index).
-template <class T, class TO, bool AKC, bool BKC>
+template <class T, class TO, bool AKC, bool BKC, int SITE>
 __global__ __launch_bounds__(NTHREADS) void gemm_mfma_kernel(const GemmDesc d) {
   using CT = typename Cplx<T>::type;
   using AV = typename Acc<T>::type;
@@ -176,18 +176,33 @@ __global__ __launch_bounds__(NTHREADS) void gemm_mfma_kernel(const GemmDesc d) {
       }
 }
 
-template <class T, class TO>
-void launch_typed(const GemmDesc& d, hipStream_t stream) {
+template <class T, class TO, int SITE>
+void launch_site(const GemmDesc& d, hipStream_t stream) {
   const int tiles = ((d.M + BM - 1) / BM) * ((d.N + BN - 1) / BN);
   dim3 grid(tiles, d.nb1, d.nb0), block(NTHREADS);
   const bool akc = (d.A.cs == 1) || (d.A.rs != 1);
   const bool bkc = (d.B.rs == 1) || (d.B.cs != 1);
-  if (akc && bkc) hipLaunchKernelGGL((gemm_mfma_kernel<T, TO, true, true>), grid, block, 0, stream, d);
-  else if (akc) hipLaunchKernelGGL((gemm_mfma_kernel<T, TO, true, false>), grid, block, 0, stream, d);
-  else if (bkc) hipLaunchKernelGGL((gemm_mfma_kernel<T, TO, false, true>), grid, block, 0, stream, d);
-  else hipLaunchKernelGGL((gemm_mfma_kernel<T, TO, false, false>), grid, block, 0, stream, d);
+  if (akc && bkc) hipLaunchKernelGGL((gemm_mfma_kernel<T, TO, true, true, SITE>), grid, block, 0, stream, d);
+  else if (akc) hipLaunchKernelGGL((gemm_mfma_kernel<T, TO, true, false, SITE>), grid, block, 0, stream, d);
+  else if (bkc) hipLaunchKernelGGL((gemm_mfma_kernel<T, TO, false, true, SITE>), grid, block, 0, stream, d);
+  else hipLaunchKernelGGL((gemm_mfma_kernel<T, TO, false, false, SITE>), grid, block, 0, stream, d);
   GGPU_HIP(hipGetLastError());
 }
+
+template <class T, class TO>
+void launch_typed(const GemmDesc& d, hipStream_t stream) {
+  switch (d.site) {
+    case kSiteP0X1: launch_site<T, TO, kSiteP0X1>(d, stream); break;
+    case kSiteP0X2: launch_site<T, TO, kSiteP0X2>(d, stream); break;
+    case kSiteP0: launch_site<T, TO, kSiteP0>(d, stream); break;
+    case kSiteSigY1: launch_site<T, TO, kSiteSigY1>(d, stream); break;
+    case kSiteSigY2: launch_site<T, TO, kSiteSigY2>(d, stream); break;
+    case kSiteSigma: launch_site<T, TO, kSiteSigma>(d, stream); break;
+    default: launch_site<T, TO, kSiteOther>(d, stream); break;
+  }
+}
+
+thread_local GemmProfiler* g_profiler = nullptr;
 
 // Range of element offsets (relative to ptr) touched by a 2-D view with the
 // given extents and strides, over all batch indices.
@@ -242,9 +257,12 @@ void gemm_check_bounds(const GemmDesc& d) {
   check_operand("C", d.C.ptr, d.C.buf, d.C.buf_elems, eout, d.C.off1_host, d.nb1, rc);
 }
 
+void set_gemm_profiler(GemmProfiler* p) { g_profiler = p; }
+
 void gemm(const GemmDesc& d, hipStream_t stream) {
   gemm_check_bounds(d);
   if (d.M == 0 || d.N == 0) return;
+  if (g_profiler) g_profiler->before(d, stream);
   switch (d.compute) {
     case Compute::F64:
       require(!d.out_f32, "gemm: FP64 compute writes complex128");
@@ -257,6 +275,7 @@ void gemm(const GemmDesc& d, hipStream_t stream) {
     default:
       throw Error(kErrArgument, "gemm: compute mode not available");
   }
+  if (g_profiler) g_profiler->after(d, stream);
 }
 
 }  // namespace ggpu

@@ -46,7 +46,35 @@ struct OutOperand {
   const int64_t* off1_host = nullptr;
 };
 
+// Call sites of the HF/GW path.  Each heavy site launches its own kernel
+// symbol (so profilers attribute time per site) and is timed separately when
+// a GemmProfiler is installed.
+enum Site : int {
+  kSiteOther = 0,
+  kSiteP0X1 = 1,     // X1 = G(k, beta - tau) V          (P0, first product)
+  kSiteP0X2 = 2,     // X2 = G(k+q, tau) V^dagger        (P0, second product)
+  kSiteP0 = 3,       // P0 += X1 . X2                    (P0 contraction)
+  kSiteSigY1 = 4,    // Y1 = V G(k-q, tau)               (Sigma, first product)
+  kSiteSigY2 = 5,    // Y2 = P~(tau) . Y1                (Sigma, screened-interaction product)
+  kSiteSigma = 6,    // Sigma -= Y2 . V^dagger           (Sigma contraction)
+  kSiteIR = 7,       // tau <-> Omega transforms
+  kSiteDyson = 8,    // Cholesky updates and triangular-solve updates
+  kSiteHF = 9,       // HF exchange / Hartree / Ewald products
+  kSiteRotate = 10,  // k- and q-symmetry rotations
+  kNumSites = 11,
+};
+
+struct GemmDesc;
+// Hook called around every launch (installed per thread).
+struct GemmProfiler {
+  virtual ~GemmProfiler() = default;
+  virtual void before(const GemmDesc& d, hipStream_t s) = 0;
+  virtual void after(const GemmDesc& d, hipStream_t s) = 0;
+};
+void set_gemm_profiler(GemmProfiler* p);
+
 struct GemmDesc {
+  int site = kSiteOther;
   int M = 0, N = 0, K = 0, KB = 1;
   int nb0 = 1, nb1 = 1;
   Operand A, B;
@@ -63,5 +91,10 @@ void gemm_check_bounds(const GemmDesc& d);
 
 // Enqueue the GEMM on `stream`.
 void gemm(const GemmDesc& d, hipStream_t stream);
+
+// Algorithmic FLOPs of the launch: 8 per complex multiply-add.
+inline double gemm_flops(const GemmDesc& d) {
+  return 8.0 * d.M * (double)d.N * d.K * d.KB * d.nb0 * d.nb1;
+}
 
 }  // namespace ggpu

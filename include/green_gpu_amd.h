@@ -156,6 +156,17 @@ int ggpu_gw_polarization(ggpu_gw_plan* plan, const double* G, int iq, int which,
  * [0] total, [1] G staging, [2] P0, [3] IR + Dyson, [4] Sigma, [5] Sigma staging. */
 int ggpu_gw_last_timings(ggpu_gw_plan* plan, double* ms6);
 
+/* Per-call-site timing of the GEMM launches of ggpu_gw_iterate*: a pair of
+ * device events on the plan's stream around every launch (the reference
+ * reports only an iteration-level FLOP estimate, gw_gpu_kernel.cpp:207-247).
+ * Sites: 0 other, 1 P0 G.V, 2 P0 G.V^dagger, 3 P0 contraction, 4 Sigma V.G,
+ * 5 Sigma P.Y1, 6 Sigma contraction, 7 IR transforms, 8 Dyson updates,
+ * 9 HF, 10 symmetry rotations. */
+int ggpu_gw_set_profiling(ggpu_gw_plan* plan, int enable);
+/* Totals of the last profiled iteration for one site: device ms, algorithmic
+ * FLOPs (8 per complex multiply-add), launch count. */
+int ggpu_gw_site_stats(ggpu_gw_plan* plan, int site, double* ms, double* flops, int64_t* launches);
+
 void ggpu_gw_plan_destroy(ggpu_gw_plan* plan);
 
 #ifdef __cplusplus
