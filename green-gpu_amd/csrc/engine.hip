@@ -345,9 +345,24 @@ GwPlan::GwPlan(int nao_, int ns_, int NQ_, int nts_, int nw_b_, const ggpu_symme
   const int64_t n2 = (int64_t)nao * nao, Q2 = (int64_t)NQ * NQ;
   G_host_dev.resize((size_t)nts * ns * sym.ink * n2);
   Gf.resize((size_t)sym.nk * ns * nts * n2);
+  Grev.resize(Gf.size());
+  {
+    std::vector<int64_t> src((size_t)sym.nk * ns * nts), dst(src.size());
+    size_t o = 0;
+    for (int k = 0; k < sym.nk; ++k)
+      for (int sp = 0; sp < ns; ++sp)
+        for (int t = 0; t < nts; ++t, ++o) {
+          src[o] = (((int64_t)k * ns + sp) * nts + (nts - 1 - t)) * n2;
+          dst[o] = (((int64_t)k * ns + sp) * nts + t) * n2;
+        }
+    rev_src.resize(src.size());
+    rev_dst.resize(dst.size());
+    GGPU_HIP(hipMemcpy(rev_src.get(), src.data(), src.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    GGPU_HIP(hipMemcpy(rev_dst.get(), dst.data(), dst.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+  }
   Sig_acc.resize((size_t)sym.ink * ns * nts * n2);
   Sig_host_dev.resize(Sig_acc.size());
-  P0.resize((size_t)half * Q2);
+  P0.resize((size_t)p0_split * half * Q2);
   P0w.resize((size_t)nw_b * Q2);
   Amat.resize((size_t)nw_b * Q2);
   Pt.resize((size_t)nts * Q2);
@@ -423,7 +438,11 @@ GwPlan::~GwPlan() {
   if (stream) (void)hipStreamDestroy(stream);
 }
 
-void GwPlan::stage_G(const z64* G_dev) { rot.run(G_dev, Gf.get(), stream); }
+void GwPlan::stage_G(const z64* G_dev) {
+  rot.run(G_dev, Gf.get(), stream);
+  slab_gather(Gf.get(), rev_src.get(), Grev.get(), rev_dst.get(), (int)rev_src.size(), (int64_t)nao * nao, 1.0,
+              stream);
+}
 
 // P0(q, tau_t)[Q'][Q] += pref * Tr[G(k, beta - tau_t) V^{Q'} G(k+q, tau_t) V^{Q dagger}]
 // for t < nts/2 (cugw_qkpt.cu:202-246), as three GEMMs per pair batch:
@@ -436,7 +455,7 @@ void GwPlan::p0_for_q(int iq) {
   const int64_t q = sym.q_r2f[iq];
   const int npairs = (int)tab.host.size();
   const double pref = (ns == 1) ? -2.0 : -1.0;
-  zero(P0.get(), (int64_t)half * NQ * NQ, stream);
+  zero(P0.get(), (int64_t)P0.size(), stream);
   // full-BZ k of every pair, in table order
   std::vector<int64_t> ks;
   for (int ik = 0; ik < sym.ink; ++ik)
@@ -456,13 +475,14 @@ void GwPlan::p0_for_q(int iq) {
     for (int b = 0; b < n; ++b) {
       const int64_t k = ks[i0 + b], k1 = sym.k1_from_k2q[k * sym.nq + q];
       for (int s = 0; s < ns; ++s) {
+        // rows (t, n) of the tau-reversed G: one tall GEMM for all t < nts/2
         GemmDesc g;
-        g.M = nao; g.N = nao * NQ; g.K = nao; g.nb0 = half;
-        g.A.ptr = Gf.get() + ((k * ns + s) * nts + (nts - 1)) * n2; g.A.rs = nao; g.A.cs = 1; g.A.bs0 = -n2;
+        g.M = half * nao; g.N = nao * NQ; g.K = nao;
+        g.A.ptr = Grev.get() + (k * ns + s) * nts * n2; g.A.rs = nao; g.A.cs = 1;
         g.B.ptr = VpmQ + b * stride; g.B.rs = (int64_t)nao * NQ; g.B.cs = 1;
-        g.C.ptr = X1 + (b * ns + s) * xs; g.C.rs = (int64_t)nao * NQ; g.C.cs = 1; g.C.bs0 = stride;
+        g.C.ptr = X1 + (b * ns + s) * xs; g.C.rs = (int64_t)nao * NQ; g.C.cs = 1;
         g.site = kSiteP0X1;
-        gemm_on(g, Gf, work, work, stream);
+        gemm_on(g, Grev, work, work, stream);
         GemmDesc h;
         h.M = NQ * nao; h.N = nao; h.K = nao; h.nb0 = half;
         h.A.ptr = Vdir + b * stride; h.A.rs = nao; h.A.cs = 1; h.A.conj = 1;
@@ -472,14 +492,22 @@ void GwPlan::p0_for_q(int iq) {
         gemm_on(h, work, Gf, work, stream);
       }
     }
-    GemmDesc p;
-    p.M = NQ; p.N = NQ; p.K = (int)n2; p.KB = n * ns; p.nb0 = half;
-    p.A.ptr = X1; p.A.rs = 1; p.A.cs = NQ; p.A.kbs = xs; p.A.bs0 = stride;
-    p.B.ptr = X2; p.B.rs = 1; p.B.cs = n2; p.B.kbs = xs; p.B.bs0 = stride;
-    p.C.ptr = P0.get(); p.C.rs = NQ; p.C.cs = 1; p.C.bs0 = (int64_t)NQ * NQ;
-    p.alpha_re = pref; p.beta_re = 1.0;
-    p.site = kSiteP0;
-    gemm_on(p, work, work, P0, stream);
+    // P0 partial slice j accumulates (pair, spin) entries j, j + S, j + 2S, ...
+    const int E = n * ns, S = p0_split, full = E / S, rem = E - full * S;
+    const int64_t Q2 = (int64_t)NQ * NQ;
+    for (int part = 0; part < 2; ++part) {
+      const int slices = part == 0 ? (full > 0 ? S : 0) : rem;
+      if (slices == 0) continue;
+      GemmDesc p;
+      p.M = NQ; p.N = NQ; p.K = (int)n2; p.KB = part == 0 ? full : 1; p.nb0 = half; p.nb1 = slices;
+      const int64_t first = part == 0 ? 0 : (int64_t)full * S * xs;
+      p.A.ptr = X1 + first; p.A.rs = 1; p.A.cs = NQ; p.A.kbs = S * xs; p.A.bs0 = stride; p.A.bs1 = xs;
+      p.B.ptr = X2 + first; p.B.rs = 1; p.B.cs = n2; p.B.kbs = S * xs; p.B.bs0 = stride; p.B.bs1 = xs;
+      p.C.ptr = P0.get(); p.C.rs = NQ; p.C.cs = 1; p.C.bs0 = Q2; p.C.bs1 = (int64_t)half * Q2;
+      p.alpha_re = pref; p.beta_re = 1.0;
+      p.site = kSiteP0;
+      gemm_on(p, work, work, P0, stream);
+    }
   }
 }
 
@@ -489,9 +517,9 @@ void GwPlan::dyson_for_q() {
   const int n = NQ;
   {
     GemmDesc g;
-    g.M = nw_b; g.N = (int)Q2; g.K = half - 1;
-    g.A.ptr = Tf_wt.get(); g.A.rs = half - 1; g.A.cs = 1;
-    g.B.ptr = P0.get() + Q2; g.B.rs = Q2; g.B.cs = 1;
+    g.M = nw_b; g.N = (int)Q2; g.K = half - 1; g.KB = p0_split;
+    g.A.ptr = Tf_wt.get(); g.A.rs = half - 1; g.A.cs = 1; g.A.kbs = 0;
+    g.B.ptr = P0.get() + Q2; g.B.rs = Q2; g.B.cs = 1; g.B.kbs = (int64_t)half * Q2;
     g.C.ptr = P0w.get(); g.C.rs = Q2; g.C.cs = 1;
     g.alpha_re = 1.0 / sym.nk;
     g.site = kSiteIR;
@@ -707,14 +735,19 @@ void GwPlan::polarization(const double* G, int iq, int which, double* out) {
   zero_int(info.get(), nw_b, stream);
   p0_for_q(iq);
   if (which == 0) {
-    std::vector<z64> h((size_t)half * Q2);
+    std::vector<z64> h((size_t)p0_split * half * Q2);
     GGPU_HIP(hipMemcpyAsync(h.data(), P0.get(), h.size() * sizeof(z64), hipMemcpyDeviceToHost, stream));
     GGPU_HIP(hipStreamSynchronize(stream));
     z64* o = reinterpret_cast<z64*>(out);
     const double sc = 1.0 / sym.nk;
     for (int t = 0; t < half; ++t)
       for (int64_t e = 0; e < Q2; ++e) {
-        const z64 v = h[(size_t)t * Q2 + e];
+        z64 v{0.0, 0.0};
+        for (int j = 0; j < p0_split; ++j) {
+          const z64 u = h[((size_t)j * half + t) * Q2 + e];
+          v.x += u.x;
+          v.y += u.y;
+        }
         o[(size_t)t * Q2 + e] = z64{v.x * sc, v.y * sc};
         o[(size_t)(nts - 1 - t) * Q2 + e] = z64{v.x * sc, v.y * sc};
       }

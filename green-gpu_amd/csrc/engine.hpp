@@ -100,7 +100,9 @@ struct GwPlan {
 
   DeviceBuffer<z64> Tf_wt;   // folded tau -> Omega transform, (nw_b, half-1)
   DeviceBuffer<z64> T_tw;    // (nts, nw_b)
-  DeviceBuffer<z64> G_host_dev, Gf, Sig_acc, Sig_host_dev;
+  DeviceBuffer<z64> G_host_dev, Gf, Grev, Sig_acc, Sig_host_dev;
+  DeviceBuffer<int64_t> rev_src, rev_dst;   // Gf -> Grev (tau reversed) slab offsets
+  int p0_split = 4;                         // P0 partial sums (split of the pair/spin reduction)
   DeviceBuffer<z64> P0, P0w, Amat, Pt, Prot, Ptmp, Uq_dev;
   DeviceBuffer<int> info;
   DeviceBuffer<z64> work;

@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <type_traits>
 
 namespace ggpu {
 namespace {
@@ -56,21 +57,39 @@ template <class T> __device__ inline int acc_row(int lk, int r);
 template <> __device__ inline int acc_row<double>(int lk, int r) { return lk + 4 * r; }
 template <> __device__ inline int acc_row<float>(int lk, int r) { return 4 * lk + r; }
 
+// Workgroup -> tile mapping.  Workgroups are dispatched round-robin over the
+// 8 XCDs (each with its own L2); the linear id is remapped so that every XCD
+// receives a contiguous range of logical tiles, ordered m-tile fastest, then
+// n-tile, then batch: the workgroups that share a B panel (same n-tile) and an
+// A panel (same batch) run on the same XCD at the same time.
+constexpr int NXCD = 8;
+__device__ inline int64_t xcd_logical(int64_t id, int64_t total) {
+  const int64_t xcd = id % NXCD, slot = id / NXCD;
+  const int64_t per = total / NXCD, rem = total % NXCD;
+  return (xcd < rem) ? xcd * (per + 1) + slot : rem * (per + 1) + (xcd - rem) * per + slot;
+}
+
 // T: arithmetic type of operands and MFMA; TO: complex element type of C.
 // AKC: A's k index is the contiguous one (else its m index).
 // BKC: B's k index is the contiguous one (else its n index).
-template <class T, class TO, bool AKC, bool BKC, int SITE>
+// GAUSS: 3 real products per complex product (t1 = ar br, t2 = ai bi,
+//   t3 = (ar + ai)(br + bi); re = t1 - t2, im = t3 - t1 - t2) instead of 4.
+template <class T, class TO, bool AKC, bool BKC, bool GAUSS, int SITE>
 __global__ __launch_bounds__(NTHREADS) void gemm_mfma_kernel(const GemmDesc d) {
   using CT = typename Cplx<T>::type;
   using AV = typename Acc<T>::type;
-  __shared__ T sA[2][BKT][BM + PAD];   // [re/im][k][m]
-  __shared__ T sB[2][BKT][BN + PAD];   // [re/im][k][n]
+  __shared__ T sA[2][2][BKT][BM + PAD];   // [buffer][re/im][k][m]
+  __shared__ T sB[2][2][BKT][BN + PAD];   // [buffer][re/im][k][n]
 
   const int tid = threadIdx.x;
+  const int tiles_m = (d.M + BM - 1) / BM;
   const int tiles_n = (d.N + BN - 1) / BN;
-  const int m0 = (blockIdx.x / tiles_n) * BM;
-  const int n0 = (blockIdx.x % tiles_n) * BN;
-  const int64_t b1 = blockIdx.y, b0 = blockIdx.z;
+  int64_t lg = xcd_logical(blockIdx.x, gridDim.x);
+  const int m0 = (int)(lg % tiles_m) * BM;
+  lg /= tiles_m;
+  const int n0 = (int)(lg % tiles_n) * BN;
+  lg /= tiles_n;
+  const int64_t b1 = lg % d.nb1, b0 = lg / d.nb1;
 
   const CT* A = static_cast<const CT*>(d.A.ptr) + b0 * d.A.bs0 + (d.A.off1 ? d.A.off1[b1] : b1 * d.A.bs1);
   const CT* B = static_cast<const CT*>(d.B.ptr) + b0 * d.B.bs0 + (d.B.off1 ? d.B.off1[b1] : b1 * d.B.bs1);
@@ -103,51 +122,76 @@ __global__ __launch_bounds__(NTHREADS) void gemm_mfma_kernel(const GemmDesc d) {
       rb[j] = (gk2 < d.K && gn < d.N) ? Bb[gk2 * d.B.rs + gn * d.B.cs] : CT{0, 0};
     }
   };
-  auto sstore = [&]() {
+  auto sstore = [&](int buf) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      sA[0][a_k[j]][a_m[j]] = ra[j].x;
-      sA[1][a_k[j]][a_m[j]] = d.A.conj ? -ra[j].y : ra[j].y;
-      sB[0][b_k[j]][b_n[j]] = rb[j].x;
-      sB[1][b_k[j]][b_n[j]] = d.B.conj ? -rb[j].y : rb[j].y;
+      sA[buf][0][a_k[j]][a_m[j]] = ra[j].x;
+      sA[buf][1][a_k[j]][a_m[j]] = d.A.conj ? -ra[j].y : ra[j].y;
+      sB[buf][0][b_k[j]][b_n[j]] = rb[j].x;
+      sB[buf][1][b_k[j]][b_n[j]] = d.B.conj ? -rb[j].y : rb[j].y;
     }
   };
 
   const int lane = tid & 63, w = tid >> 6;
   const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
   const int lr = lane & 15, lk = lane >> 4;
-  AV cr[2][2], ci[2][2];
+  AV acc[GAUSS ? 3 : 2][2][2];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int a = 0; a < (GAUSS ? 3 : 2); ++a)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) { cr[i][j] = AV{0, 0, 0, 0}; ci[i][j] = AV{0, 0, 0, 0}; }
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[a][i][j] = AV{0, 0, 0, 0};
 
-  if (niter > 0) gload(0);
+  if (niter > 0) {
+    gload(0);
+    sstore(0);
+  }
+  __syncthreads();
   for (int it = 0; it < niter; ++it) {
-    sstore();
-    __syncthreads();
+    const int buf = it & 1;
     if (it + 1 < niter) gload(it + 1);
+    const int k0 = (it % ktiles) * BKT;
 #pragma unroll
     for (int kk = 0; kk < BKT; kk += 4) {
-      T ar[2], ai[2], nai[2], br[2], bi[2];
+      if (k0 + kk < d.K) {           // uniform: skip MFMA steps past the end of K
+        T ar[2], ai[2], br[2], bi[2];
 #pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        ar[i] = sA[0][kk + lk][wm + 16 * i + lr];
-        ai[i] = sA[1][kk + lk][wm + 16 * i + lr];
-        nai[i] = -ai[i];
-        br[i] = sB[0][kk + lk][wn + 16 * i + lr];
-        bi[i] = sB[1][kk + lk][wn + 16 * i + lr];
-      }
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          cr[i][j] = mfma<T, AV>(ar[i], br[j], cr[i][j]);
-          ci[i][j] = mfma<T, AV>(ar[i], bi[j], ci[i][j]);
-          cr[i][j] = mfma<T, AV>(nai[i], bi[j], cr[i][j]);
-          ci[i][j] = mfma<T, AV>(ai[i], br[j], ci[i][j]);
+        for (int i = 0; i < 2; ++i) {
+          ar[i] = sA[buf][0][kk + lk][wm + 16 * i + lr];
+          ai[i] = sA[buf][1][kk + lk][wm + 16 * i + lr];
+          br[i] = sB[buf][0][kk + lk][wn + 16 * i + lr];
+          bi[i] = sB[buf][1][kk + lk][wn + 16 * i + lr];
         }
+        if (GAUSS) {
+          T as[2], bs[2];
+#pragma unroll
+          for (int i = 0; i < 2; ++i) { as[i] = ar[i] + ai[i]; bs[i] = br[i] + bi[i]; }
+#pragma unroll
+          for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+              acc[0][i][j] = mfma<T, AV>(ar[i], br[j], acc[0][i][j]);
+              acc[1][i][j] = mfma<T, AV>(ai[i], bi[j], acc[1][i][j]);
+              acc[GAUSS ? 2 : 0][i][j] = mfma<T, AV>(as[i], bs[j], acc[GAUSS ? 2 : 0][i][j]);
+            }
+        } else {
+          T nai[2];
+#pragma unroll
+          for (int i = 0; i < 2; ++i) nai[i] = -ai[i];
+#pragma unroll
+          for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+              acc[0][i][j] = mfma<T, AV>(ar[i], br[j], acc[0][i][j]);
+              acc[1][i][j] = mfma<T, AV>(ar[i], bi[j], acc[1][i][j]);
+              acc[0][i][j] = mfma<T, AV>(nai[i], bi[j], acc[0][i][j]);
+              acc[1][i][j] = mfma<T, AV>(ai[i], br[j], acc[1][i][j]);
+            }
+        }
+      }
     }
+    if (it + 1 < niter) sstore(buf ^ 1);
     __syncthreads();
   }
 
@@ -161,7 +205,15 @@ __global__ __launch_bounds__(NTHREADS) void gemm_mfma_kernel(const GemmDesc d) {
         const int row = m0 + wm + 16 * i + acc_row<T>(lk, r);
         const int col = n0 + wn + 16 * j + lr;
         if (row < d.M && col < d.N) {
-          const double xr = cr[i][j][r], xi = ci[i][j][r];
+          double xr, xi;
+          if (GAUSS) {
+            const double t1 = acc[0][i][j][r], t2 = acc[1][i][j][r], t3 = acc[GAUSS ? 2 : 0][i][j][r];
+            xr = t1 - t2;
+            xi = t3 - t1 - t2;
+          } else {
+            xr = acc[0][i][j][r];
+            xi = acc[1][i][j][r];
+          }
           double vr = d.alpha_re * xr - d.alpha_im * xi;
           double vi = d.alpha_re * xi + d.alpha_im * xr;
           TO* p = C + row * d.C.rs + col * d.C.cs;
@@ -176,29 +228,35 @@ __global__ __launch_bounds__(NTHREADS) void gemm_mfma_kernel(const GemmDesc d) {
       }
 }
 
-template <class T, class TO, int SITE>
+template <class T, class TO, bool GAUSS, int SITE>
 void launch_site(const GemmDesc& d, hipStream_t stream) {
-  const int tiles = ((d.M + BM - 1) / BM) * ((d.N + BN - 1) / BN);
-  dim3 grid(tiles, d.nb1, d.nb0), block(NTHREADS);
+  const int64_t tiles = (int64_t)((d.M + BM - 1) / BM) * ((d.N + BN - 1) / BN);
+  const int64_t total = tiles * d.nb0 * d.nb1;
+  require(total < (int64_t)1 << 31, "gemm: grid too large");
+  dim3 grid((unsigned)total), block(NTHREADS);
   const bool akc = (d.A.cs == 1) || (d.A.rs != 1);
   const bool bkc = (d.B.rs == 1) || (d.B.cs != 1);
-  if (akc && bkc) hipLaunchKernelGGL((gemm_mfma_kernel<T, TO, true, true, SITE>), grid, block, 0, stream, d);
-  else if (akc) hipLaunchKernelGGL((gemm_mfma_kernel<T, TO, true, false, SITE>), grid, block, 0, stream, d);
-  else if (bkc) hipLaunchKernelGGL((gemm_mfma_kernel<T, TO, false, true, SITE>), grid, block, 0, stream, d);
-  else hipLaunchKernelGGL((gemm_mfma_kernel<T, TO, false, false, SITE>), grid, block, 0, stream, d);
+  if (akc && bkc) hipLaunchKernelGGL((gemm_mfma_kernel<T, TO, true, true, GAUSS, SITE>), grid, block, 0, stream, d);
+  else if (akc) hipLaunchKernelGGL((gemm_mfma_kernel<T, TO, true, false, GAUSS, SITE>), grid, block, 0, stream, d);
+  else if (bkc) hipLaunchKernelGGL((gemm_mfma_kernel<T, TO, false, true, GAUSS, SITE>), grid, block, 0, stream, d);
+  else hipLaunchKernelGGL((gemm_mfma_kernel<T, TO, false, false, GAUSS, SITE>), grid, block, 0, stream, d);
   GGPU_HIP(hipGetLastError());
 }
 
+// FP64 uses the 3-product (Gauss) complex scheme; FP32 keeps 4 products
+// (the Gauss recombination t3 - t1 - t2 loses relative accuracy when the
+// result is small against |A||B|, which FP32 cannot afford).
 template <class T, class TO>
 void launch_typed(const GemmDesc& d, hipStream_t stream) {
+  constexpr bool G = std::is_same<T, double>::value;
   switch (d.site) {
-    case kSiteP0X1: launch_site<T, TO, kSiteP0X1>(d, stream); break;
-    case kSiteP0X2: launch_site<T, TO, kSiteP0X2>(d, stream); break;
-    case kSiteP0: launch_site<T, TO, kSiteP0>(d, stream); break;
-    case kSiteSigY1: launch_site<T, TO, kSiteSigY1>(d, stream); break;
-    case kSiteSigY2: launch_site<T, TO, kSiteSigY2>(d, stream); break;
-    case kSiteSigma: launch_site<T, TO, kSiteSigma>(d, stream); break;
-    default: launch_site<T, TO, kSiteOther>(d, stream); break;
+    case kSiteP0X1: launch_site<T, TO, G, kSiteP0X1>(d, stream); break;
+    case kSiteP0X2: launch_site<T, TO, G, kSiteP0X2>(d, stream); break;
+    case kSiteP0: launch_site<T, TO, G, kSiteP0>(d, stream); break;
+    case kSiteSigY1: launch_site<T, TO, G, kSiteSigY1>(d, stream); break;
+    case kSiteSigY2: launch_site<T, TO, G, kSiteSigY2>(d, stream); break;
+    case kSiteSigma: launch_site<T, TO, G, kSiteSigma>(d, stream); break;
+    default: launch_site<T, TO, G, kSiteOther>(d, stream); break;
   }
 }
 

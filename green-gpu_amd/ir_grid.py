@@ -129,7 +129,12 @@ class IRBasis:
         roots = np.array(sorted(roots))
         if roots.size != self.L:
             raise RuntimeError(f"found {roots.size} sampling points, expected {self.L}")
-        return roots
+        # u_L has definite parity under tau -> beta - tau, so its roots are
+        # symmetric; the Nystrom evaluation of u_L carries noise ~ eps / s_L,
+        # so the roots of the lower half are mirrored exactly (the GW kernel
+        # relies on tau_{nts-1-t} = beta - tau_t, cugw_qpt.cu:184-191).
+        lower = roots[: self.L // 2]
+        return np.concatenate([lower, self.beta - lower[::-1]])
 
     def boson_frequencies(self):
         """Bosonic Matsubara sampling indices: n = 0 plus +-n at the sign changes
