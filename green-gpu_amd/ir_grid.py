@@ -167,6 +167,35 @@ class IRBasis:
         return np.concatenate([-pos[::-1], [0], pos])
 
 
+def transforms_on_grid(beta, lam, tau, eps=1e-14):
+    """T_wt / T_tw for a given symmetric tau grid (endpoints included), e.g.
+    the sampling points of another IR basis of the same Lambda.
+
+    The bosonic coefficients are fitted by least squares from the nts-2
+    interior points (which may outnumber the L basis functions) and from the
+    Matsubara sampling frequencies; any basis that represents the bosonic
+    functions to round-off gives the same tau -> tau map up to that
+    round-off, whatever its own sampling points are."""
+    tau = np.asarray(tau, dtype=np.float64)
+    if tau.ndim != 1 or tau.size % 2 or tau[0] != 0.0 or tau[-1] != beta:
+        raise ValueError("tau grid must have an even number of points from 0 to beta")
+    if np.abs(tau + tau[::-1] - beta).max() > 1e-12 * beta:
+        raise ValueError("tau grid must be symmetric about beta/2")
+    b = IRBasis(beta, lam, eps)
+    L = b.L
+    if tau.size - 2 < L:
+        raise ValueError(f"{tau.size - 2} interior tau points cannot fix {L} coefficients")
+    n = b.boson_frequencies()
+    U_int = b.u(tau[1:-1])[:, :L]
+    U_all = b.u(tau)[:, :L]
+    Uhat = b.uhat_boson(n)[:, :L]
+    T_wt = Uhat @ np.linalg.pinv(U_int)
+    T_tw = U_all @ np.linalg.pinv(Uhat)
+    return {"beta": b.beta, "lam": b.lam, "tau": tau, "n_boson": n,
+            "T_wt": T_wt.astype(np.complex128), "T_tw": T_tw.astype(np.complex128),
+            "singular_values": b.s}
+
+
 def make_transforms(beta, lam, eps=1e-12):
     """Return dict(tau (nts,), n_boson (nw_b,), T_wt (nw_b, nts-2), T_tw (nts, nw_b))."""
     b = IRBasis(beta, lam, eps)
