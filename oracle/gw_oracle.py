@@ -280,11 +280,14 @@ def p0_pair(Gm, G1, V, pref):
     return pref * np.matmul(A, np.swapaxes(B, -1, -2))
 
 
-def gw_p0(Gf, V_stored, sym, q_ir, ns, nts):
+def gw_p0(Gf, V_stored, sym, q_ir, ns, nts, Gm_src=None):
     """P0(q, tau) for all nts points: k-sum over every full-BZ k (k-stars of the
     IBZ), t < nts/2 computed, scaled by 1/nk and mirrored
-    (cugw_utils.cu:121-150, cugw_qkpt.cu:202-246, cugw_qpt.cu:174-191)."""
+    (cugw_utils.cu:121-150, cugw_qkpt.cu:202-246, cugw_qpt.cu:174-191).
+    ``Gm_src`` supplies the G(k, beta - tau) factor (default ``Gf``); the
+    two-component path passes the spin-block-swapped copy."""
     pref = -2.0 if ns == 1 else -1.0
+    Gm_src = Gf if Gm_src is None else Gm_src
     half = nts // 2
     NQ = None
     P0 = None
@@ -297,7 +300,7 @@ def gw_p0(Gf, V_stored, sym, q_ir, ns, nts):
                 NQ = V.shape[0]
                 P0 = np.zeros((nts, NQ, NQ), dtype=np.complex128)
             for s in range(ns):
-                Gm = Gf[k, s, nts - 1 - np.arange(half)]
+                Gm = Gm_src[k, s, nts - 1 - np.arange(half)]
                 G1 = Gf[k1, s, :half]
                 P0[:half] += p0_pair(Gm, G1, V, pref)
     P0[:half] *= 1.0 / sym.nk
@@ -361,19 +364,15 @@ def sigma_pair(Pt, G1, V):
     return -np.matmul(Y2.reshape(nt, nao, NQ * nao), VH.reshape(NQ * nao, nao))
 
 
-def gw_solve(G, V_stored, sym, T_wt, T_tw, solver="LU", return_parts=False):
-    """One GW iteration: Sigma(tau) (nts, ns, ink, nao, nao) from G(tau) of the
-    same shape (gw_gpu_kernel.cpp:163-205, cugw_utils.cu:103-202)."""
-    nts, ns, ink, nao, _ = G.shape
-    nw_b = T_wt.shape[0]
-    check_gw_dims(nts, nw_b)
-    assert T_wt.shape == (nw_b, nts - 2) and T_tw.shape == (nts, nw_b)
-    Gf = g_full(G, sym)
-    Sigma = np.zeros_like(G)
+def _gw_core(Gf, Gm_src, ns, nts, sym, V_stored, T_wt, T_tw, solver, return_parts):
+    """Sigma(tau) in the (nts, ns, ink, nao, nao) layout for full-BZ G blocks
+    ``Gf`` (nk, ns, nts, nao, nao) (cugw_utils.cu:103-202)."""
+    nao = Gf.shape[-1]
+    Sigma = np.zeros((nts, ns, sym.ink, nao, nao), dtype=np.complex128)
     parts = {"P0": [], "P0w": [], "Pw": [], "Pt": []}
     for iq in range(sym.inq):
         q_ir = int(sym.q_reduced_to_full[iq])
-        P0 = gw_p0(Gf, V_stored, sym, q_ir, ns, nts)
+        P0 = gw_p0(Gf, V_stored, sym, q_ir, ns, nts, Gm_src)
         P0w = tau_to_omega(P0, T_wt)
         Pw = gw_dyson(P0w, solver)
         Pt = omega_to_tau(Pw, T_tw)
@@ -392,9 +391,126 @@ def gw_solve(G, V_stored, sym, T_wt, T_tw, solver="LU", return_parts=False):
                 for s in range(ns):
                     Sigma[:, s, ik] += sigma_pair(Pq, Gf[k1, s], V)
     Sigma /= sym.nk
-    if return_parts:
-        return Sigma, parts
-    return Sigma
+    return Sigma, parts
+
+
+def _check_transforms(nts, T_wt, T_tw):
+    nw_b = T_wt.shape[0]
+    check_gw_dims(nts, nw_b)
+    assert T_wt.shape == (nw_b, nts - 2) and T_tw.shape == (nts, nw_b)
+
+
+def gw_solve(G, V_stored, sym, T_wt, T_tw, solver="LU", return_parts=False):
+    """One GW iteration: Sigma(tau) (nts, ns, ink, nao, nao) from G(tau) of the
+    same shape (gw_gpu_kernel.cpp:163-205, cugw_utils.cu:103-202)."""
+    nts, ns = G.shape[0], G.shape[1]
+    _check_transforms(nts, T_wt, T_tw)
+    Gf = g_full(G, sym)
+    Sigma, parts = _gw_core(Gf, Gf, ns, nts, sym, V_stored, T_wt, T_tw, solver, return_parts)
+    return (Sigma, parts) if return_parts else Sigma
+
+
+# ----------------------------------------------------------------------------
+# Two-component (X2C) variants: gw_gpu_kernel.cpp:457-551, cugw_qkpt.cu:381-403,
+# hf_gpu_kernel.cpp:229-338, cuda_common.cu:176-205
+# ----------------------------------------------------------------------------
+# Pseudo-spin order of the spinor blocks (row block, column block): aa, bb, ab, ba
+X2C_BLOCKS = ((0, 0), (1, 1), (0, 1), (1, 0))
+
+
+def _x2c_check(sym):
+    # the rotation of a spinor matrix onto a time-reversed k-point (value_AO's
+    # spin flip) lives in green-symmetry and is not restated here
+    if np.any(np.asarray(sym.k_tr_conj) != 0):
+        raise ValueError("x2c: time-reversal images in the k-point tables are not supported")
+
+
+def spinor_blocks(X, nao, blocks):
+    """(..., 2 nao, 2 nao) -> (..., len(blocks), nao, nao) on a new axis -3."""
+    return np.stack([X[..., a * nao:(a + 1) * nao, b * nao:(b + 1) * nao] for a, b in blocks], axis=-3)
+
+
+def gw_solve_x2c(G, V_stored, sym, T_wt, T_tw, solver="LU", return_parts=False):
+    """Two-component GW: G, Sigma (nts, 1, ink, nso, nso), nso = 2 nao.  The
+    scalar contractions run on pseudo-spins (aa, bb, ab, ba) with prefactor -1;
+    G(k, beta - tau) enters with its spin blocks swapped (the ab pseudo-spin
+    pairs G_ba(beta - tau) with G_ab(tau)), and every Sigma block is
+    accumulated from its own pseudo-spin (gw_gpu_kernel.cpp:525-551,
+    cugw_qkpt.cu:381-403)."""
+    _x2c_check(sym)
+    nts, nso = G.shape[0], G.shape[-1]
+    assert G.shape[1] == 1 and nso % 2 == 0
+    nao = nso // 2
+    _check_transforms(nts, T_wt, T_tw)
+    Gf2 = g_full(G, sym)[:, 0]                          # (nk, nts, nso, nso)
+    G4 = np.ascontiguousarray(np.swapaxes(spinor_blocks(Gf2, nao, X2C_BLOCKS), 1, 2))   # (nk, 4, nts, n, n)
+    Gm4 = np.ascontiguousarray(G4[:, [0, 1, 3, 2]])
+    S4, parts = _gw_core(G4, Gm4, 4, nts, sym, V_stored, T_wt, T_tw, solver, return_parts)
+    Sigma = np.zeros_like(G)
+    for ss, (a, b) in enumerate(X2C_BLOCKS):
+        Sigma[:, 0, :, a * nao:(a + 1) * nao, b * nao:(b + 1) * nao] = S4[:, ss]
+    return (Sigma, parts) if return_parts else Sigma
+
+
+def _exchange_fbz(dm_fbz, V_stored, sym, px):
+    """K[s, ik] = (px/nk) sum_{k2,Q} V^Q(k,k2) D_s(k2) V^Q(k,k2)^dagger from
+    full-BZ density blocks (ns', nk, n, n)."""
+    ns, n = dm_fbz.shape[0], dm_fbz.shape[-1]
+    K = np.zeros((ns, sym.ink, n, n), dtype=np.complex128)
+    for ik in range(sym.ink):
+        k = int(sym.k_reduced_to_full[ik])
+        for k2 in range(sym.nk):
+            V = v_pair(V_stored, sym, k, k2)
+            VH = np.conj(np.swapaxes(V, -1, -2))
+            for s in range(ns):
+                K[s, ik] += (px / sym.nk) * np.einsum("Qia,ab,Qbj->ij", V, dm_fbz[s, k2], VH)
+    return K
+
+
+def hf_exchange_x2c(dm, V_stored, sym):
+    """Two-component exchange (1, ink, nso, nso): the aa, bb and ab blocks are
+    the scalar exchange of the matching density blocks with prefactor -1
+    (pseudo-spin 3), and K_ba = K_ab^dagger (hf_gpu_kernel.cpp:229-249,
+    cuda_common.cu:176-205)."""
+    _x2c_check(sym)
+    nso = dm.shape[-1]
+    nao = nso // 2
+    dm3 = spinor_blocks(ibz_to_full(dm[0], sym), nao, X2C_BLOCKS[:3])   # (nk, 3, n, n)
+    K3 = _exchange_fbz(np.swapaxes(dm3, 0, 1), V_stored, sym, -1.0)     # (3, ink, n, n)
+    K = np.zeros((1, sym.ink, nso, nso), dtype=np.complex128)
+    for ss, (a, b) in enumerate(X2C_BLOCKS[:3]):
+        K[0, :, a * nao:(a + 1) * nao, b * nao:(b + 1) * nao] = K3[ss]
+    K[0, :, nao:, :nao] = np.conj(np.swapaxes(K3[2], -1, -2))
+    return K
+
+
+def hf_solve_x2c(dm, V_stored, sym, S_k, madelung):
+    """Two-component Fock = J + K - Ewald (hf_gpu_kernel.cpp:251-338):
+    J from the aa + bb density on both diagonal blocks; the Ewald term uses
+    the AO overlap S_aa on both sides of every density block (aa, bb, ab;
+    ba as the adjoint of ab)."""
+    _x2c_check(sym)
+    nso = dm.shape[-1]
+    nao = nso // 2
+    F = hf_exchange_x2c(dm, V_stored, sym)
+    upper = 0
+    for kp in range(sym.nk):
+        V = v_pair(V_stored, sym, kp, kp)
+        D = value_ao(dm[0], sym, kp)
+        upper = upper + np.einsum("Qab,ba->Q", V, D[:nao, :nao] + D[nao:, nao:])
+    upper = upper / sym.nk
+    for ik in range(sym.ink):
+        k = int(sym.k_reduced_to_full[ik])
+        J = np.einsum("Q,Qij->ij", upper, v_pair(V_stored, sym, k, k))
+        F[0, ik, :nao, :nao] += J
+        F[0, ik, nao:, nao:] += J
+        S_aa = S_k[0, ik, :nao, :nao]
+        for a, b in X2C_BLOCKS[:3]:
+            buf = madelung * S_aa @ dm[0, ik, a * nao:(a + 1) * nao, b * nao:(b + 1) * nao] @ S_aa
+            F[0, ik, a * nao:(a + 1) * nao, b * nao:(b + 1) * nao] -= buf
+            if a != b:
+                F[0, ik, nao:, :nao] -= np.conj(buf.T)
+    return F
 
 
 # ----------------------------------------------------------------------------

@@ -19,6 +19,10 @@ What is extracted (all paths relative to ``test/data`` of the reference):
   ``meta.h5:chunk_size``.
 * ``GW/data.h5`` + ``GW/df_int``: ``G_tau``, expected ``result/Sigma_tau``
   and the integral chunk used by the GW regression.
+* the two-component (X2C) system of ``HF_X2C`` / ``GW_X2C`` (one k-point,
+  nao = 10, nso = 20, NQ = 154): the same tables, ``Sigma1`` and
+  ``Sigma_tau`` (test/cu_solver_test.cpp:317-336), into
+  ``ref_fixtures_x2c.npz``.
 
 Run:  python tests/golden/make_fixtures.py  (writes tests/golden/fixtures/)
 """
@@ -73,9 +77,9 @@ def complex_interleaved(a):
     return np.ascontiguousarray(a).view(np.complex128)
 
 
-def main():
-    os.makedirs(OUT, exist_ok=True)
-    inp = os.path.join(REF, "HF/input.h5")
+def read_system(inp):
+    """Dimensions, overlap/Fock matrices and every symmetry table of an
+    ``input.h5``."""
     out = {}
     for key in ("nao", "nso", "NQ", "ns", "nk"):
         out["params_" + key] = h5read(inp, "/params/" + key)
@@ -95,24 +99,35 @@ def main():
     for name in ("conj_pairs_list", "trans_pairs_list", "kpair_irre_list", "kpair_idx",
                  "num_kpair_stored"):
         out["pairs_" + name] = h5read(inp, "/symmetry/pairs/" + name)
+    return out
 
-    hf_data = os.path.join(REF, "HF/data.h5")
-    out["hf_G_tau"] = h5read(hf_data, "/G_tau")
-    out["hf_Sigma1"] = h5read(hf_data, "/result/Sigma1")
-    hf_int = os.path.join(REF, "HF/df_hf_int")
-    out["hf_chunk_size"] = h5read(os.path.join(hf_int, "meta.h5"), "/chunk_size")
-    out["hf_VQ_0"] = complex_interleaved(h5read(os.path.join(hf_int, "VQ_0.h5"), "/0"))
 
-    gw_data = os.path.join(REF, "GW/data.h5")
-    out["gw_G_tau"] = h5read(gw_data, "/G_tau")
-    out["gw_Sigma_tau"] = h5read(gw_data, "/result/Sigma_tau")
-    gw_int = os.path.join(REF, "GW/df_int")
-    out["gw_chunk_size"] = h5read(os.path.join(gw_int, "meta.h5"), "/chunk_size")
-    out["gw_VQ_0"] = complex_interleaved(h5read(os.path.join(gw_int, "VQ_0.h5"), "/0"))
+def read_run(out, prefix, data, integrals, result):
+    out[prefix + "_G_tau"] = h5read(data, "/G_tau")
+    out[prefix + "_" + result.split("/")[-1]] = h5read(data, result)
+    out[prefix + "_chunk_size"] = h5read(os.path.join(integrals, "meta.h5"), "/chunk_size")
+    out[prefix + "_VQ_0"] = complex_interleaved(h5read(os.path.join(integrals, "VQ_0.h5"), "/0"))
 
-    np.savez_compressed(os.path.join(OUT, "ref_fixtures.npz"), **out)
+
+def write(name, out):
+    np.savez_compressed(os.path.join(OUT, name), **out)
     for k, v in out.items():
         print(f"{k:28s} {str(v.dtype):10s} {v.shape}")
+
+
+def main():
+    os.makedirs(OUT, exist_ok=True)
+    # scalar (ns = 2) system: HF/input.h5 and GW/input.h5 are byte-identical
+    out = read_system(os.path.join(REF, "HF/input.h5"))
+    read_run(out, "hf", os.path.join(REF, "HF/data.h5"), os.path.join(REF, "HF/df_hf_int"), "/result/Sigma1")
+    read_run(out, "gw", os.path.join(REF, "GW/data.h5"), os.path.join(REF, "GW/df_int"), "/result/Sigma_tau")
+    write("ref_fixtures.npz", out)
+    # two-component (X2C) system: HF_X2C and GW_X2C share input.h5 and integrals
+    x2c = read_system(os.path.join(REF, "HF_X2C/input.h5"))
+    read_run(x2c, "hf", os.path.join(REF, "HF_X2C/data.h5"), os.path.join(REF, "HF_X2C/df_hf_int"), "/result/Sigma1")
+    read_run(x2c, "gw", os.path.join(REF, "GW_X2C/data.h5"), os.path.join(REF, "GW_X2C/df_hf_int"),
+             "/result/Sigma_tau")
+    write("ref_fixtures_x2c.npz", x2c)
 
 
 if __name__ == "__main__":

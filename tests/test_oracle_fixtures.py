@@ -115,3 +115,53 @@ def test_gw_fixture_is_sensitive(oracle, fx):
     T_half = T_tw.copy()
     T_half[T_tw.shape[0] // 2:] = 0
     assert max_abs(solve(ttw=T_half), ref) > 1e-6
+
+
+X2C = os.path.join(ROOT, "tests", "golden", "fixtures", "ref_fixtures_x2c.npz")
+
+
+@pytest.fixture(scope="module")
+def fx2c():
+    return dict(np.load(X2C))
+
+
+def test_hf_x2c_fixture_sigma1(oracle, fx2c):
+    """test/cu_solver_test.cpp:331-334 (HF_X2C): two-component HF on one
+    k-point (nao 10, nso 20, NQ 154); solver output times -1 equals Sigma1."""
+    sym = oracle.symmetry_from_fixture(fx2c, int(fx2c["params_NQ"]))
+    S_ibz = fx2c["S_k"][:, sym.k_reduced_to_full]
+    F = oracle.hf_solve_x2c(fx2c["hf_G_tau"][-1], fx2c["hf_VQ_0"], sym, S_ibz, float(fx2c["madelung"]))
+    err = max_abs(-F, fx2c["hf_Sigma1"])
+    print(f"HF X2C fixture: max-abs {err:.2e}")
+    assert err < 1e-8
+
+
+def test_gw_x2c_fixture_sigma_tau(oracle, fx2c):
+    """test/cu_solver_test.cpp:317-328 (GW_X2C): two-component GW with the
+    beta = 100 / Lambda = 1e4 grid recovered from the scalar GW fixture."""
+    g = np.load(GRID)
+    sym = oracle.symmetry_from_fixture(fx2c, int(fx2c["params_NQ"]))
+    S = oracle.gw_solve_x2c(fx2c["gw_G_tau"], fx2c["gw_VQ_0"], sym, g["T_wt"], g["T_tw"], solver="lu")
+    err = max_abs(S, fx2c["gw_Sigma_tau"])
+    print(f"GW X2C fixture: max-abs {err:.2e} (|Sigma| max {np.abs(fx2c['gw_Sigma_tau']).max():.2e})")
+    assert err < 1e-6
+    assert err < 1e-8
+
+
+def test_x2c_fixtures_are_sensitive(oracle, fx2c, monkeypatch):
+    """Without the spin-block swap of G(beta - tau), or with K_ba = K_ab
+    instead of its adjoint, the X2C results leave the fixture tolerances."""
+    sym = oracle.symmetry_from_fixture(fx2c, int(fx2c["params_NQ"]))
+    g = np.load(GRID)
+    G, V = fx2c["gw_G_tau"], fx2c["gw_VQ_0"]
+    nao = G.shape[-1] // 2
+    Gf2 = oracle.g_full(G, sym)[:, 0]
+    G4 = np.ascontiguousarray(np.swapaxes(oracle.spinor_blocks(Gf2, nao, oracle.X2C_BLOCKS), 1, 2))
+    S4, _ = oracle._gw_core(G4, G4, 4, G.shape[0], sym, V, g["T_wt"], g["T_tw"], "lu", False)
+    S = np.zeros_like(G)
+    for ss, (a, b) in enumerate(oracle.X2C_BLOCKS):
+        S[:, 0, :, a * nao:(a + 1) * nao, b * nao:(b + 1) * nao] = S4[:, ss]
+    assert max_abs(S, fx2c["gw_Sigma_tau"]) > 1e-6
+    K = oracle.hf_exchange_x2c(fx2c["hf_G_tau"][-1], fx2c["hf_VQ_0"], sym)
+    assert max_abs(K[0, :, nao:, :nao], K[0, :, :nao, nao:]) > 1e-6
+    assert max_abs(K[0, :, nao:, :nao], np.conj(np.swapaxes(K[0, :, :nao, nao:], -1, -2))) < 1e-14
