@@ -98,6 +98,8 @@ def main():
     ap.add_argument("--solver", default="cholesky", choices=["cholesky", "lu"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--cpu-pairs", type=int, default=2)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run one plain 1 GiB device-to-device copy kernel (scale for HBM counters)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
@@ -144,6 +146,14 @@ def main():
     kern.set_profiling(False)
     stats = kern.site_stats()
     phases = kern.timings()
+    if a.calibrate_copy:
+        src = torch.ones(1 << 27, dtype=torch.float64, device=dev)      # 1 GiB
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy: 1073741824 bytes read, 1073741824 bytes written")
+        del src, dst
 
     fl = gw_flops(nk, nao, NQ, ns, nts, nw_b)
     tflops = fl["total"] / dt / 1e12

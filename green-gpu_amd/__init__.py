@@ -8,8 +8,10 @@ declared in include/green_gpu_amd.h; there is no Python or CPU fallback - if
 the shared library is missing, importing the kernels raises.
 
 Tensor layouts are the reference's host layouts (complex128, row-major):
-G, Sigma (nts, ns, ink, nao, nao); dm, S_k, Fock (ns, ink, nao, nao);
-stored integrals (num_kpair_stored, NQ, nao, nao).
+G, Sigma (nts, ns, ink, nso, nso); dm, S_k, Fock (ns, ink, nso, nso);
+stored integrals (num_kpair_stored, NQ, nao, nao).  nso = nao for the
+scalar kernels; nso = 2 nao (with ns = 1) selects the two-component (X2C)
+kernels, as custom_hf_kernel / custom_gw_kernel do (gpu_factory.h:47-89).
 """
 import ctypes as C
 import os
@@ -45,14 +47,14 @@ def lib():
         L.ggpu_last_error.restype = C.c_char_p
         L.ggpu_device_count.argtypes = [C.POINTER(C.c_int)]
         L.ggpu_device_memory.argtypes = [C.POINTER(sz), C.POINTER(sz)]
-        L.ggpu_hf_plan_create.argtypes = [C.POINTER(vp), i32, i32, i32, C.POINTER(SymmetryC)]
+        L.ggpu_hf_plan_create.argtypes = [C.POINTER(vp), i32, i32, i32, i32, C.POINTER(SymmetryC)]
         L.ggpu_hf_load_integrals.argtypes = [vp, pd]
         L.ggpu_hf_synth_integrals.argtypes = [vp, u64, f64]
         L.ggpu_hf_exchange.argtypes = [vp, pd, pd]
         L.ggpu_hf_solve.argtypes = [vp, pd, pd, f64, pd]
         L.ggpu_hf_plan_destroy.argtypes = [vp]
         L.ggpu_hf_plan_destroy.restype = None
-        L.ggpu_gw_plan_create.argtypes = [C.POINTER(vp), i32, i32, i32, i32, i32, C.POINTER(SymmetryC),
+        L.ggpu_gw_plan_create.argtypes = [C.POINTER(vp), i32, i32, i32, i32, i32, i32, C.POINTER(SymmetryC),
                                           pd, pd, i32, i32, sz]
         L.ggpu_gw_load_integrals.argtypes = [vp, pd]
         L.ggpu_gw_synth_integrals.argtypes = [vp, u64, f64]
@@ -225,17 +227,23 @@ def device_memory():
 # Kernels
 # ---------------------------------------------------------------------------
 class HfKernel:
-    """HF kernel on the GPU (reference: scalar_hf_gpu_kernel, hf_gpu_kernel.h:67-84)."""
+    """HF kernel on the GPU (reference: scalar_hf_gpu_kernel, hf_gpu_kernel.h:67-84;
+    x2c_hf_gpu_kernel, hf_gpu_kernel.h:87-100, when nso = 2 nao)."""
 
-    def __init__(self, nao, ns, NQ, sym, madelung=0.0, S_k=None):
+    def __init__(self, nao, ns, NQ, sym, madelung=0.0, S_k=None, nso=None):
         self.nao, self.ns, self.NQ = int(nao), int(ns), int(NQ)
+        self.nso = self.nao if nso is None else int(nso)
         self.sym = sym if isinstance(sym, Symmetry) else Symmetry.like(sym)
         self.madelung = float(madelung)
-        self.S_k = None if S_k is None else _c128(S_k, (self.ns, self.sym.ink, self.nao, self.nao))
+        self.S_k = None if S_k is None else _c128(S_k, self.mat_shape)
         h = C.c_void_p()
         self._csym = self.sym.to_c()
-        _check(lib().ggpu_hf_plan_create(C.byref(h), self.nao, self.ns, self.NQ, C.byref(self._csym)))
+        _check(lib().ggpu_hf_plan_create(C.byref(h), self.nao, self.nso, self.ns, self.NQ, C.byref(self._csym)))
         self._h = h
+
+    @property
+    def mat_shape(self):
+        return (self.ns, self.sym.ink, self.nso, self.nso)
 
     def load_integrals(self, V_stored):
         V = _c128(V_stored)
@@ -245,14 +253,14 @@ class HfKernel:
         _check(lib().ggpu_hf_synth_integrals(self._h, int(seed), float(sigma)))
 
     def exchange(self, dm):
-        dm = _c128(dm, (self.ns, self.sym.ink, self.nao, self.nao))
+        dm = _c128(dm, self.mat_shape)
         K = np.empty_like(dm)
         _check(lib().ggpu_hf_exchange(self._h, _dptr(dm), _dptr(K)))
         return K
 
     def solve(self, dm):
         """J + K - Ewald for the IBZ density matrix (hf_gpu_kernel.cpp:66-104)."""
-        dm = _c128(dm, (self.ns, self.sym.ink, self.nao, self.nao))
+        dm = _c128(dm, self.mat_shape)
         S = self.S_k if self.S_k is not None else np.zeros_like(dm)
         F = np.empty_like(dm)
         _check(lib().ggpu_hf_solve(self._h, _dptr(dm), _dptr(S), self.madelung, _dptr(F)))
@@ -268,24 +276,26 @@ class HfKernel:
 
 
 class GwKernel:
-    """GW kernel on the GPU (reference: scalar_gw_gpu_kernel, gw_gpu_kernel.h:139-172)."""
+    """GW kernel on the GPU (reference: scalar_gw_gpu_kernel, gw_gpu_kernel.h:139-172;
+    x2c_gw_gpu_kernel, gw_gpu_kernel.h:169-206, when nso = 2 nao)."""
 
-    def __init__(self, nao, ns, NQ, T_wt, T_tw, sym, precision="fp64", solver="lu", work_bytes=0):
+    def __init__(self, nao, ns, NQ, T_wt, T_tw, sym, precision="fp64", solver="lu", work_bytes=0, nso=None):
         T_wt = _c128(T_wt)
         T_tw = _c128(T_tw)
         self.nao, self.ns, self.NQ = int(nao), int(ns), int(NQ)
+        self.nso = self.nao if nso is None else int(nso)
         self.nw_b, self.nts = T_wt.shape[0], T_tw.shape[0]
         self.sym = sym if isinstance(sym, Symmetry) else Symmetry.like(sym)
         h = C.c_void_p()
         self._csym = self.sym.to_c()
-        _check(lib().ggpu_gw_plan_create(C.byref(h), self.nao, self.ns, self.NQ, self.nts, self.nw_b,
+        _check(lib().ggpu_gw_plan_create(C.byref(h), self.nao, self.nso, self.ns, self.NQ, self.nts, self.nw_b,
                                          C.byref(self._csym), _dptr(T_wt), _dptr(T_tw),
                                          PRECISION[precision], SOLVER[solver.lower()], int(work_bytes)))
         self._h = h
 
     @property
     def shape(self):
-        return (self.nts, self.ns, self.sym.ink, self.nao, self.nao)
+        return (self.nts, self.ns, self.sym.ink, self.nso, self.nso)
 
     def load_integrals(self, V_stored):
         V = _c128(V_stored)

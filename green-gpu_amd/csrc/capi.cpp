@@ -51,13 +51,13 @@ int ggpu_device_memory(size_t* free_bytes, size_t* total_bytes) {
   });
 }
 
-int ggpu_hf_plan_create(ggpu_hf_plan** plan, int nao, int ns, int NQ, const ggpu_symmetry* sym) {
+int ggpu_hf_plan_create(ggpu_hf_plan** plan, int nao, int nso, int ns, int NQ, const ggpu_symmetry* sym) {
   return guarded([&] {
     ggpu::require(plan && sym, "NULL plan or symmetry");
     *plan = nullptr;
     auto* p = new ggpu_hf_plan{nullptr};
     try {
-      p->impl = new ggpu::HfPlan(nao, ns, NQ, *sym);
+      p->impl = new ggpu::HfPlan(nao, nso, ns, NQ, *sym);
     } catch (...) {
       delete p;
       throw;
@@ -102,14 +102,14 @@ void ggpu_hf_plan_destroy(ggpu_hf_plan* plan) {
   delete plan;
 }
 
-int ggpu_gw_plan_create(ggpu_gw_plan** plan, int nao, int ns, int NQ, int nts, int nw_b, const ggpu_symmetry* sym,
+int ggpu_gw_plan_create(ggpu_gw_plan** plan, int nao, int nso, int ns, int NQ, int nts, int nw_b, const ggpu_symmetry* sym,
                         const double* T_wt, const double* T_tw, int precision, int solver, size_t work_bytes) {
   return guarded([&] {
     ggpu::require(plan && sym, "NULL plan or symmetry");
     *plan = nullptr;
     auto* p = new ggpu_gw_plan{nullptr};
     try {
-      p->impl = new ggpu::GwPlan(nao, ns, NQ, nts, nw_b, *sym, T_wt, T_tw, precision, solver, work_bytes);
+      p->impl = new ggpu::GwPlan(nao, nso, ns, NQ, nts, nw_b, *sym, T_wt, T_tw, precision, solver, work_bytes);
     } catch (...) {
       delete p;
       throw;

@@ -180,13 +180,14 @@ void PairTable::build(const std::vector<PairView>& views, int64_t stride, hipStr
 }
 
 void unfold_pairs(const IntegralStore& st, const PairTable& tab, int i0, int n, const int64_t* out_off_dev,
-                  z64* dst, int NQ, int nao, hipStream_t s) {
+                  void* dst, int out_f32, int NQ, int nao, hipStream_t s) {
   require(i0 >= 0 && i0 + n <= (int)tab.host.size(), "unfold_pairs: range");
   for (int i = i0; i < i0 + n; ++i)
     require((tab.host[i].pos + 1) * (int64_t)NQ * nao * nao <= (int64_t)st.V.size(), "unfold_pairs: pair outside store");
   BlockCopy c;
   c.in = st.V.get();
   c.out = dst;
+  c.out_f32 = out_f32;
   c.rows = nao;
   c.cols = nao;
   c.in_ld = nao;
@@ -307,23 +308,93 @@ void SiteProfiler::collect() {
 }
 
 // ---------------------------------------------------------------------------
+// Spinor <-> spin-block tables (two-component plans)
+// ---------------------------------------------------------------------------
+// Spin blocks in pseudo-spin order: (row block, column block) of aa, bb, ab, ba
+// (x2c_gw_gpu_kernel::copy_Gk_2c, gw_gpu_kernel.cpp:513-523).
+static const int kX2cBlock[4][2] = {{0, 0}, {1, 1}, {0, 1}, {1, 0}};
+
+void SpinorTables::build(const std::vector<int64_t>& in, const std::vector<int64_t>& out, const std::vector<int>& tr,
+                         const std::vector<int>& cj) {
+  n = (int)in.size();
+  h_in = in;
+  h_out = out;
+  in_off.resize(in.size());
+  out_off.resize(out.size());
+  trans.resize(tr.size());
+  conj.resize(cj.size());
+  GGPU_HIP(hipMemcpy(in_off.get(), in.data(), in.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+  GGPU_HIP(hipMemcpy(out_off.get(), out.data(), out.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+  GGPU_HIP(hipMemcpy(trans.get(), tr.data(), tr.size() * sizeof(int), hipMemcpyHostToDevice));
+  GGPU_HIP(hipMemcpy(conj.get(), cj.data(), cj.size() * sizeof(int), hipMemcpyHostToDevice));
+}
+
+// Split: for every matrix m < count, block b < nblk of the spinor matrix
+// in[m] (nso x nso, slab stride in_ms) goes to out[m * nblk + b] (nao x nao,
+// slab stride out_ms; inner batch of n_inner slabs handled by the copy).
+static void build_split(SpinorTables& t, int count, int nblk, int nao, int64_t in_ms, int64_t out_ms) {
+  const int nso = 2 * nao;
+  std::vector<int64_t> in, out;
+  std::vector<int> zero;
+  for (int m = 0; m < count; ++m)
+    for (int b = 0; b < nblk; ++b) {
+      in.push_back(m * in_ms + (int64_t)kX2cBlock[b][0] * nao * nso + (int64_t)kX2cBlock[b][1] * nao);
+      out.push_back(((int64_t)m * nblk + b) * out_ms);
+      zero.push_back(0);
+    }
+  t.build(in, out, zero, zero);
+}
+
+// Copies nao x nao blocks; in_elems / out_elems are the buffer sizes the
+// offsets are checked against on the host before the launch.
+static void spinor_copy(const SpinorTables& t, const z64* in, size_t in_elems, int64_t in_ld, int64_t in_bs,
+                        z64* out, size_t out_elems, int64_t out_ld, int64_t out_bs, int nao, int n_inner,
+                        double scale, hipStream_t s) {
+  const int64_t span_in = (int64_t)(n_inner - 1) * in_bs + (int64_t)(nao - 1) * in_ld + nao;
+  const int64_t span_out = (int64_t)(n_inner - 1) * out_bs + (int64_t)(nao - 1) * out_ld + nao;
+  for (int i = 0; i < t.n; ++i)
+    require(t.h_in[i] >= 0 && t.h_in[i] + span_in <= (int64_t)in_elems && t.h_out[i] >= 0 &&
+                t.h_out[i] + span_out <= (int64_t)out_elems,
+            "spinor block copy out of bounds");
+  BlockCopy c;
+  c.in = in; c.out = out;
+  c.rows = nao; c.cols = nao;
+  c.in_ld = in_ld; c.out_ld = out_ld;
+  c.n_inner = n_inner; c.in_bs = in_bs; c.out_bs = out_bs;
+  c.n_outer = t.n;
+  c.in_off = t.in_off.get(); c.out_off = t.out_off.get();
+  c.trans = t.trans.get(); c.conj = t.conj.get();
+  c.scale = scale;
+  block_copy(c, s);
+}
+
+// ---------------------------------------------------------------------------
 // GW plan
 // ---------------------------------------------------------------------------
-GwPlan::GwPlan(int nao_, int ns_, int NQ_, int nts_, int nw_b_, const ggpu_symmetry& s, const double* T_wt,
+GwPlan::GwPlan(int nao_, int nso_, int ns_, int NQ_, int nts_, int nw_b_, const ggpu_symmetry& s, const double* T_wt,
                const double* T_tw_h, int precision, int solver_, size_t work_bytes)
-    : nao(nao_), ns(ns_), NQ(NQ_), nts(nts_), nw_b(nw_b_), half(nts_ / 2), solver(solver_) {
+    : nao(nao_), nso(nso_), ns(ns_), NQ(NQ_), nts(nts_), nw_b(nw_b_), half(nts_ / 2), solver(solver_) {
   require(nao > 0 && NQ > 0, "gw: nao and NQ must be positive");
+  require(nso == nao || nso == 2 * nao, "gw: nso must be nao or 2 nao");
   require(ns == 1 || ns == 2, "gw: ns must be 1 or 2");
+  x2c = nso == 2 * nao;
+  require(!x2c || ns == 1, "gw: a two-component (nso = 2 nao) plan needs ns = 1");
+  if (x2c) ns = 4;   // pseudo-spins aa, bb, ab, ba
   require(nts % 2 == 0, "gw: nts must be even");
   require(nts >= 4, "gw: nts must be at least 4");
   require(nts >= nw_b, "gw: Nt is not greater than or equal to Nw_b");
   require(nw_b > 0, "gw: nw_b must be positive");
   require(solver == GGPU_SOLVER_CHOLESKY || solver == GGPU_SOLVER_LU, "gw: unknown linear solver");
-  require(precision == GGPU_PRECISION_FP64, "gw: only the FP64 precision is available in this build",
-          kErrArgument);
+  require(precision == GGPU_PRECISION_FP64 || precision == GGPU_PRECISION_FP32 ||
+              precision == GGPU_PRECISION_BF16X3,
+          "gw: unknown precision");
   require(T_wt && T_tw_h, "gw: NULL transform matrices");
-  compute = Compute::F64;
-  sym.load(s, nao, NQ);
+  compute = precision == GGPU_PRECISION_FP64 ? Compute::F64
+            : precision == GGPU_PRECISION_FP32 ? Compute::F32 : Compute::BF16X3;
+  sym.load(s, nso, NQ);
+  if (x2c)
+    for (auto t : sym.k_tr)
+      require(t == 0, "gw: two-component plans do not support time-reversal k images");
   GGPU_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   for (auto& e : ev) GGPU_HIP(hipEventCreate(&e));
 
@@ -346,13 +417,20 @@ GwPlan::GwPlan(int nao_, int ns_, int NQ_, int nts_, int nw_b_, const ggpu_symme
   G_host_dev.resize((size_t)nts * ns * sym.ink * n2);
   Gf.resize((size_t)sym.nk * ns * nts * n2);
   Grev.resize(Gf.size());
+  if (lowp()) {
+    Gf32.resize(Gf.size());
+    Grev32.resize(Gf.size());
+  }
   {
     std::vector<int64_t> src((size_t)sym.nk * ns * nts), dst(src.size());
     size_t o = 0;
     for (int k = 0; k < sym.nk; ++k)
       for (int sp = 0; sp < ns; ++sp)
         for (int t = 0; t < nts; ++t, ++o) {
-          src[o] = (((int64_t)k * ns + sp) * nts + (nts - 1 - t)) * n2;
+          // X2C: the -tau factor of pseudo-spin ab is the ba block and vice
+          // versa (copy_Gk_2c with minus_t, gw_gpu_kernel.cpp:517-521)
+          const int sp_src = (x2c && sp >= 2) ? 5 - sp : sp;
+          src[o] = (((int64_t)k * ns + sp_src) * nts + (nts - 1 - t)) * n2;
           dst[o] = (((int64_t)k * ns + sp) * nts + t) * n2;
         }
     rev_src.resize(src.size());
@@ -366,6 +444,7 @@ GwPlan::GwPlan(int nao_, int ns_, int NQ_, int nts_, int nw_b_, const ggpu_symme
   P0w.resize((size_t)nw_b * Q2);
   Amat.resize((size_t)nw_b * Q2);
   Pt.resize((size_t)nts * Q2);
+  if (lowp()) Pt32.resize((size_t)nts * Q2);
   if (!sym.q_identity()) {
     Prot.resize((size_t)nts * Q2);
     Ptmp.resize((size_t)nts * Q2);
@@ -399,7 +478,7 @@ GwPlan::GwPlan(int nao_, int ns_, int NQ_, int nts_, int nw_b_, const ggpu_symme
   // workspace for pair batches
   const int64_t per_pair_p0 = (2 + 2LL * ns * half) * stride;
   const int64_t per_pair_sig = (1 + 2LL * ns * nts) * stride;
-  const int64_t per_pair = std::max(per_pair_p0, per_pair_sig) * (int64_t)sizeof(z64);
+  const int64_t per_pair = std::max(per_pair_p0, per_pair_sig) * (int64_t)(lowp() ? sizeof(c32) : sizeof(z64));
   size_t budget = work_bytes;
   if (budget == 0) {
     size_t fr = 0, tot = 0;
@@ -408,12 +487,29 @@ GwPlan::GwPlan(int nao_, int ns_, int NQ_, int nts_, int nw_b_, const ggpu_symme
   }
   kb = (int)std::max<int64_t>(1, std::min<int64_t>(budget / per_pair, std::max(sym.nk, sym.ink)));
   kb = std::min(kb, 4096);
-  work.resize((size_t)kb * (per_pair / sizeof(z64)));
+  work.resize(((size_t)kb * per_pair + sizeof(z64) - 1) / sizeof(z64));
   std::vector<int64_t> bo(kb);
   for (int b = 0; b < kb; ++b) bo[b] = b * stride;
   bidx_off.resize(kb);
   GGPU_HIP(hipMemcpy(bidx_off.get(), bo.data(), bo.size() * sizeof(int64_t), hipMemcpyHostToDevice));
 
+  if (x2c) {
+    // G: rotated spinor slabs (k, t) -> pseudo-spin blocks (k, ss, t);
+    // Sigma: pseudo-spin blocks (ik, ss, t) -> spinor layout (t, ik)
+    const int64_t so2 = (int64_t)nso * nso;
+    build_split(x2c_split, sym.nk, 4, nao, (int64_t)nts * so2, (int64_t)nts * n2);
+    std::vector<int64_t> in, out;
+    std::vector<int> z;
+    for (int ik = 0; ik < sym.ink; ++ik)
+      for (int b = 0; b < 4; ++b) {
+        in.push_back(((int64_t)ik * 4 + b) * nts * n2);
+        out.push_back(ik * so2 + (int64_t)kX2cBlock[b][0] * nao * nso + (int64_t)kX2cBlock[b][1] * nao);
+        z.push_back(0);
+      }
+    x2c_merge.build(in, out, z, z);
+    rot.init(sym, nso, 1, nts, stream);
+    return;
+  }
   // Sigma (ik, s, t) -> host layout (t, s, ik)
   std::vector<int64_t> src((size_t)sym.ink * ns * nts), dst(src.size());
   size_t o = 0;
@@ -439,9 +535,20 @@ GwPlan::~GwPlan() {
 }
 
 void GwPlan::stage_G(const z64* G_dev) {
-  rot.run(G_dev, Gf.get(), stream);
+  if (x2c) {
+    // spinor G on the full BZ into Grev (used as scratch), then its blocks
+    rot.run(G_dev, Grev.get(), stream);
+    spinor_copy(x2c_split, Grev.get(), Grev.size(), nso, (int64_t)nso * nso, Gf.get(), Gf.size(), nao,
+                (int64_t)nao * nao, nao, nts, 1.0, stream);
+  } else {
+    rot.run(G_dev, Gf.get(), stream);
+  }
   slab_gather(Gf.get(), rev_src.get(), Grev.get(), rev_dst.get(), (int)rev_src.size(), (int64_t)nao * nao, 1.0,
               stream);
+  if (lowp()) {
+    to_c32(Gf.get(), Gf32.get(), (int64_t)Gf.size(), stream);
+    to_c32(Grev.get(), Grev32.get(), (int64_t)Grev.size(), stream);
+  }
 }
 
 // P0(q, tau_t)[Q'][Q] += pref * Tr[G(k, beta - tau_t) V^{Q'} G(k+q, tau_t) V^{Q dagger}]
@@ -455,21 +562,28 @@ void GwPlan::p0_for_q(int iq) {
   const int64_t q = sym.q_r2f[iq];
   const int npairs = (int)tab.host.size();
   const double pref = (ns == 1) ? -2.0 : -1.0;
+  const bool lp = lowp();
+  const int64_t es = lp ? (int64_t)sizeof(c32) : (int64_t)sizeof(z64);
   zero(P0.get(), (int64_t)P0.size(), stream);
   // full-BZ k of every pair, in table order
   std::vector<int64_t> ks;
   for (int ik = 0; ik < sym.ink; ++ik)
     for (auto k : sym.k_stars[ik]) ks.push_back(k);
-  z64* Vdir = work.get();
-  z64* VpmQ = Vdir + kb * stride;
-  z64* X1 = VpmQ + kb * stride;
-  z64* X2 = X1 + (int64_t)kb * ns * half * stride;
-  const int64_t xs = (int64_t)half * stride;  // one (pair, spin) slab of X1 / X2
+  char* Vdir = reinterpret_cast<char*>(work.get());
+  char* VpmQ = Vdir + kb * stride * es;
+  char* X1 = VpmQ + kb * stride * es;
+  char* X2 = X1 + (int64_t)kb * ns * half * stride * es;
+  const int64_t xs = (int64_t)half * stride;  // one (pair, spin) slab of X1 / X2 (elements)
+  const Span wk{work.get(), work.bytes() / (size_t)es};
+  const void* Gr = lp ? (const void*)Grev32.get() : (const void*)Grev.get();
+  const void* Gp = lp ? (const void*)Gf32.get() : (const void*)Gf.get();
+  const Span gsp{Gp, Gf.size()}, grsp{Gr, Grev.size()};
   for (int i0 = 0; i0 < npairs; i0 += kb) {
     const int n = std::min(kb, npairs - i0);
-    unfold_pairs(store, tab, i0, n, bidx_off.get(), Vdir, NQ, nao, stream);
+    unfold_pairs(store, tab, i0, n, lp ? bidx_off.get() : bidx_off.get(), Vdir, lp, NQ, nao, stream);
     BlockCopy tr;
-    tr.in = Vdir; tr.out = VpmQ; tr.rows = (int)n2; tr.cols = NQ; tr.in_ld = n2; tr.out_ld = NQ;
+    tr.in = Vdir; tr.out = VpmQ; tr.in_f32 = lp; tr.out_f32 = lp;
+    tr.rows = (int)n2; tr.cols = NQ; tr.in_ld = n2; tr.out_ld = NQ;
     tr.n_inner = n; tr.in_bs = stride; tr.out_bs = stride; tr.trans_all = 1;
     block_copy(tr, stream);
     for (int b = 0; b < n; ++b) {
@@ -477,36 +591,40 @@ void GwPlan::p0_for_q(int iq) {
       for (int s = 0; s < ns; ++s) {
         // rows (t, n) of the tau-reversed G: one tall GEMM for all t < nts/2
         GemmDesc g;
+        g.compute = compute; g.out_f32 = lp;
         g.M = half * nao; g.N = nao * NQ; g.K = nao;
-        g.A.ptr = Grev.get() + (k * ns + s) * nts * n2; g.A.rs = nao; g.A.cs = 1;
-        g.B.ptr = VpmQ + b * stride; g.B.rs = (int64_t)nao * NQ; g.B.cs = 1;
-        g.C.ptr = X1 + (b * ns + s) * xs; g.C.rs = (int64_t)nao * NQ; g.C.cs = 1;
+        g.A.ptr = (const char*)Gr + (k * ns + s) * nts * n2 * es; g.A.rs = nao; g.A.cs = 1;
+        g.B.ptr = VpmQ + b * stride * es; g.B.rs = (int64_t)nao * NQ; g.B.cs = 1;
+        g.C.ptr = X1 + (b * ns + s) * xs * es; g.C.rs = (int64_t)nao * NQ; g.C.cs = 1;
         g.site = kSiteP0X1;
-        gemm_on(g, Grev, work, work, stream);
+        gemm_on(g, grsp, wk, wk, stream);
         GemmDesc h;
+        h.compute = compute; h.out_f32 = lp;
         h.M = NQ * nao; h.N = nao; h.K = nao; h.nb0 = half;
-        h.A.ptr = Vdir + b * stride; h.A.rs = nao; h.A.cs = 1; h.A.conj = 1;
-        h.B.ptr = Gf.get() + (k1 * ns + s) * nts * n2; h.B.rs = 1; h.B.cs = nao; h.B.bs0 = n2;
-        h.C.ptr = X2 + (b * ns + s) * xs; h.C.rs = nao; h.C.cs = 1; h.C.bs0 = stride;
+        h.A.ptr = Vdir + b * stride * es; h.A.rs = nao; h.A.cs = 1; h.A.conj = 1;
+        h.B.ptr = (const char*)Gp + (k1 * ns + s) * nts * n2 * es; h.B.rs = 1; h.B.cs = nao; h.B.bs0 = n2;
+        h.C.ptr = X2 + (b * ns + s) * xs * es; h.C.rs = nao; h.C.cs = 1; h.C.bs0 = stride;
         h.site = kSiteP0X2;
-        gemm_on(h, work, Gf, work, stream);
+        gemm_on(h, wk, gsp, wk, stream);
       }
     }
     // P0 partial slice j accumulates (pair, spin) entries j, j + S, j + 2S, ...
+    // (FP64 partials also in the reduced-precision modes)
     const int E = n * ns, S = p0_split, full = E / S, rem = E - full * S;
     const int64_t Q2 = (int64_t)NQ * NQ;
     for (int part = 0; part < 2; ++part) {
       const int slices = part == 0 ? (full > 0 ? S : 0) : rem;
       if (slices == 0) continue;
       GemmDesc p;
+      p.compute = compute; p.out_f32 = 0;
       p.M = NQ; p.N = NQ; p.K = (int)n2; p.KB = part == 0 ? full : 1; p.nb0 = half; p.nb1 = slices;
       const int64_t first = part == 0 ? 0 : (int64_t)full * S * xs;
-      p.A.ptr = X1 + first; p.A.rs = 1; p.A.cs = NQ; p.A.kbs = S * xs; p.A.bs0 = stride; p.A.bs1 = xs;
-      p.B.ptr = X2 + first; p.B.rs = 1; p.B.cs = n2; p.B.kbs = S * xs; p.B.bs0 = stride; p.B.bs1 = xs;
+      p.A.ptr = X1 + first * es; p.A.rs = 1; p.A.cs = NQ; p.A.kbs = S * xs; p.A.bs0 = stride; p.A.bs1 = xs;
+      p.B.ptr = X2 + first * es; p.B.rs = 1; p.B.cs = n2; p.B.kbs = S * xs; p.B.bs0 = stride; p.B.bs1 = xs;
       p.C.ptr = P0.get(); p.C.rs = NQ; p.C.cs = 1; p.C.bs0 = Q2; p.C.bs1 = (int64_t)half * Q2;
       p.alpha_re = pref; p.beta_re = 1.0;
       p.site = kSiteP0;
-      gemm_on(p, work, work, P0, stream);
+      gemm_on(p, wk, wk, P0, stream);
     }
   }
 }
@@ -585,6 +703,7 @@ void GwPlan::dyson_for_q() {
     g.site = kSiteIR;
     gemm_on(g, T_tw, P0w, Pt, stream);
   }
+  if (lowp()) to_c32(Pt.get(), Pt32.get(), (int64_t)Pt.size(), stream);
 }
 
 // Sigma(k, tau) -= sum_{Q,Q'} P~(q, tau)[Q'][Q] V^{Q'}(k,k-q) G(k-q, tau) V^{Q dagger}(k,k-q)
@@ -594,16 +713,21 @@ void GwPlan::dyson_for_q() {
 //   Sigma[t][i][j] -= sum_{Q,m} Y2[Q][t][i][m] conj(V^Q[j][m])
 void GwPlan::sigma_for_q(int iq) {
   const int64_t n2 = (int64_t)nao * nao, stride = (int64_t)NQ * n2, Q2 = (int64_t)NQ * NQ;
-  z64* Vdir = work.get();
-  z64* Y1 = Vdir + kb * stride;
-  z64* Y2 = Y1 + (int64_t)kb * ns * nts * stride;
-  const int64_t ys = (int64_t)nts * stride;   // one (pair, spin) slab of Y1 / Y2
+  const bool lp = lowp();
+  const int64_t es = lp ? (int64_t)sizeof(c32) : (int64_t)sizeof(z64);
+  char* Vdir = reinterpret_cast<char*>(work.get());
+  char* Y1 = Vdir + kb * stride * es;
+  char* Y2 = Y1 + (int64_t)kb * ns * nts * stride * es;
+  const int64_t ys = (int64_t)nts * stride;   // one (pair, spin) slab of Y1 / Y2 (elements)
+  const Span wk{work.get(), work.bytes() / (size_t)es};
+  const void* Gp = lp ? (const void*)Gf32.get() : (const void*)Gf.get();
+  const Span gsp{Gp, Gf.size()};
   for (size_t j = 0; j < sym.q_stars[iq].size(); ++j) {
     const int64_t qd = sym.q_stars[iq][j];
     const int tr = sym.q_tr[qd] != 0;
-    const z64* Pq = Pt.get();
+    const void* Pq = lp ? (const void*)Pt32.get() : (const void*)Pt.get();
     if (!sym.q_identity()) {
-      // P~ = U_q P U_q^dagger (conjugated at use for a time-reversal image)
+      // P~ = U_q P U_q^dagger in FP64 (conjugated at use for a time-reversal image)
       GemmDesc a;
       a.M = NQ; a.N = NQ; a.K = NQ; a.nb0 = nts;
       a.A.ptr = Uq_dev.get() + qd * Q2; a.A.rs = NQ; a.A.cs = 1;
@@ -618,34 +742,42 @@ void GwPlan::sigma_for_q(int iq) {
       c.C.ptr = Prot.get(); c.C.rs = NQ; c.C.cs = 1; c.C.bs0 = Q2;
       c.site = kSiteRotate;
       gemm_on(c, Ptmp, Uq_dev, Prot, stream);
-      Pq = Prot.get();
+      if (lp) {
+        to_c32(Prot.get(), Pt32.get(), (int64_t)Prot.size(), stream);
+        Pq = Pt32.get();
+      } else {
+        Pq = Prot.get();
+      }
     }
-    const DeviceBuffer<z64>& Pbuf = sym.q_identity() ? Pt : Prot;
+    const Span psp{Pq, Pt.size()};
     const PairTable& tab = sig_tables[iq][j];
     for (int i0 = 0; i0 < sym.ink; i0 += kb) {
       const int n = std::min(kb, sym.ink - i0);
-      unfold_pairs(store, tab, i0, n, bidx_off.get(), Vdir, NQ, nao, stream);
+      unfold_pairs(store, tab, i0, n, bidx_off.get(), Vdir, lp, NQ, nao, stream);
       for (int b = 0; b < n; ++b) {
         const int64_t k = sym.k_r2f[i0 + b];
         const int64_t k1 = sym.k2_from_k1q[k * sym.nq + qd];
         for (int s = 0; s < ns; ++s) {
           GemmDesc g;
+          g.compute = compute; g.out_f32 = lp;
           g.M = NQ * nao; g.N = nao; g.K = nao; g.nb0 = nts;
-          g.A.ptr = Vdir + b * stride; g.A.rs = nao; g.A.cs = 1;
-          g.B.ptr = Gf.get() + (k1 * ns + s) * nts * n2; g.B.rs = nao; g.B.cs = 1; g.B.bs0 = n2;
-          g.C.ptr = Y1 + (b * ns + s) * ys; g.C.rs = nao; g.C.cs = 1; g.C.bs0 = stride;
+          g.A.ptr = Vdir + b * stride * es; g.A.rs = nao; g.A.cs = 1;
+          g.B.ptr = (const char*)Gp + (k1 * ns + s) * nts * n2 * es; g.B.rs = nao; g.B.cs = 1; g.B.bs0 = n2;
+          g.C.ptr = Y1 + (b * ns + s) * ys * es; g.C.rs = nao; g.C.cs = 1; g.C.bs0 = stride;
           g.site = kSiteSigY1;
-          gemm_on(g, work, Gf, work, stream);
+          gemm_on(g, wk, gsp, wk, stream);
           GemmDesc h;
+          h.compute = compute; h.out_f32 = lp;
           h.M = NQ; h.N = (int)n2; h.K = NQ; h.nb0 = nts;
           h.A.ptr = Pq; h.A.rs = 1; h.A.cs = NQ; h.A.bs0 = Q2; h.A.conj = tr;
-          h.B.ptr = Y1 + (b * ns + s) * ys; h.B.rs = n2; h.B.cs = 1; h.B.bs0 = stride;
-          h.C.ptr = Y2 + (b * ns + s) * ys; h.C.rs = (int64_t)nts * n2; h.C.cs = 1; h.C.bs0 = n2;
+          h.B.ptr = Y1 + (b * ns + s) * ys * es; h.B.rs = n2; h.B.cs = 1; h.B.bs0 = stride;
+          h.C.ptr = Y2 + (b * ns + s) * ys * es; h.C.rs = (int64_t)nts * n2; h.C.cs = 1; h.C.bs0 = n2;
           h.site = kSiteSigY2;
-          gemm_on(h, Pbuf, work, work, stream);
+          gemm_on(h, psp, wk, wk, stream);
         }
       }
       GemmDesc f;
+      f.compute = compute; f.out_f32 = 0;       // Sigma accumulates in FP64
       f.M = nts * nao; f.N = nao; f.K = nao; f.KB = NQ; f.nb0 = n; f.nb1 = ns;
       f.A.ptr = Y2; f.A.rs = nao; f.A.cs = 1; f.A.kbs = (int64_t)nts * n2; f.A.bs0 = ns * ys; f.A.bs1 = ys;
       f.B.ptr = Vdir; f.B.rs = 1; f.B.cs = nao; f.B.kbs = n2; f.B.bs0 = stride; f.B.bs1 = 0; f.B.conj = 1;
@@ -653,7 +785,7 @@ void GwPlan::sigma_for_q(int iq) {
       f.C.bs0 = (int64_t)ns * nts * n2; f.C.bs1 = (int64_t)nts * n2;
       f.alpha_re = -1.0; f.beta_re = 1.0;
       f.site = kSiteSigma;
-      gemm_on(f, work, work, Sig_acc, stream);
+      gemm_on(f, wk, wk, Sig_acc, stream);
     }
   }
 }
@@ -696,7 +828,11 @@ void GwPlan::iterate_device(const z64* G_dev, z64* S_dev) {
     GGPU_HIP(hipEventRecord(qev[4 * iq + 3], stream));
   }
   GGPU_HIP(hipEventRecord(ev[5], stream));
-  slab_gather(Sig_acc.get(), sig_src.get(), S_dev, sig_dst.get(), (int)sig_src.size(), n2, 1.0 / sym.nk, stream);
+  if (x2c)
+    spinor_copy(x2c_merge, Sig_acc.get(), Sig_acc.size(), nao, n2, S_dev, Sig_host_dev.size(), nso,
+                (int64_t)sym.ink * nso * nso, nao, nts, 1.0 / sym.nk, stream);
+  else
+    slab_gather(Sig_acc.get(), sig_src.get(), S_dev, sig_dst.get(), (int)sig_src.size(), n2, 1.0 / sym.nk, stream);
   GGPU_HIP(hipEventRecord(ev[6], stream));
   GGPU_HIP(hipStreamSynchronize(stream));
   double t_p0 = 0, t_dy = 0, t_sig = 0;
@@ -764,12 +900,20 @@ void GwPlan::polarization(const double* G, int iq, int which, double* out) {
 // ---------------------------------------------------------------------------
 // HF plan
 // ---------------------------------------------------------------------------
-HfPlan::HfPlan(int nao_, int ns_, int NQ_, const ggpu_symmetry& s) : nao(nao_), ns(ns_), NQ(NQ_) {
+HfPlan::HfPlan(int nao_, int nso_, int ns_, int NQ_, const ggpu_symmetry& s)
+    : nao(nao_), nso(nso_), ns(ns_), nps(ns_), NQ(NQ_) {
   require(nao > 0 && NQ > 0, "hf: nao and NQ must be positive");
+  require(nso == nao || nso == 2 * nao, "hf: nso must be nao or 2 nao");
   require(ns == 1 || ns == 2, "hf: Invalid value of \"ns\"");
-  sym.load(s, nao, NQ);
+  x2c = nso == 2 * nao;
+  require(!x2c || ns == 1, "hf: a two-component (nso = 2 nao) plan needs ns = 1");
+  if (x2c) nps = 3;   // density blocks aa, bb, ab (cuhf_utils with pseudo spin 3)
+  sym.load(s, nso, NQ);
+  if (x2c)
+    for (auto t : sym.k_tr)
+      require(t == 0, "hf: two-component plans do not support time-reversal k images");
   GGPU_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-  const int64_t n2 = (int64_t)nao * nao, stride = (int64_t)NQ * n2;
+  const int64_t n2 = (int64_t)nao * nao, stride = (int64_t)NQ * n2, so2 = (int64_t)nso * nso;
   tables.resize(sym.ink);
   for (int ik = 0; ik < sym.ink; ++ik) {
     const int64_t k = sym.k_r2f[ik];
@@ -777,20 +921,20 @@ HfPlan::HfPlan(int nao_, int ns_, int NQ_, const ggpu_symmetry& s) : nao(nao_), 
     for (int k2 = 0; k2 < sym.nk; ++k2) v.push_back(sym.view(k, k2));
     tables[ik].build(v, stride, stream);
   }
-  dm_host_dev.resize((size_t)ns * sym.ink * n2);
-  dm_fbz.resize((size_t)sym.nk * ns * n2);
-  const int64_t per_pair = (1 + ns) * stride * (int64_t)sizeof(z64);
+  dm_host_dev.resize((size_t)ns * sym.ink * so2);
+  dm_fbz.resize((size_t)sym.nk * nps * n2);
+  const int64_t per_pair = (1 + nps) * stride * (int64_t)sizeof(z64);
   size_t fr = 0, tot = 0;
   GGPU_HIP(hipMemGetInfo(&fr, &tot));
   const size_t budget = std::min<size_t>((size_t)(0.4 * fr), (size_t)16 << 30);
   kb = (int)std::max<int64_t>(1, std::min<int64_t>(budget / per_pair, sym.nk));
   work.resize((size_t)kb * (per_pair / sizeof(z64)));
   nsplit = 64;
-  Kpart.resize((size_t)nsplit * ns * n2);
-  Kout.resize((size_t)sym.ink * ns * n2);
+  Kpart.resize((size_t)nsplit * nps * n2);
+  Kout.resize((size_t)sym.ink * nps * n2);
   jvec.resize(NQ);
-  S_dev.resize((size_t)ns * sym.ink * n2);
-  Ftmp.resize((size_t)ns * sym.ink * n2);
+  S_dev.resize((size_t)ns * sym.ink * so2);
+  Ftmp.resize((size_t)nps * sym.ink * n2);
   diag_tables.resize(2);
   {
     std::vector<PairView> all, ibz;
@@ -803,7 +947,26 @@ HfPlan::HfPlan(int nao_, int ns_, int NQ_, const ggpu_symmetry& s) : nao(nao_), 
   for (int b = 0; b < kb; ++b) bo[b] = b * stride;
   bidx_off.resize(kb);
   GGPU_HIP(hipMemcpy(bidx_off.get(), bo.data(), bo.size() * sizeof(int64_t), hipMemcpyHostToDevice));
-  rot.init(sym, nao, ns, 1, stream);
+  if (x2c) {
+    dm2c_fbz.resize((size_t)sym.nk * so2);
+    F2c.resize((size_t)sym.ink * so2);
+    build_split(x2c_split, sym.nk, 3, nao, so2, n2);
+    // Fock blocks aa, bb, ab from Kout[ik][0..2]; ba = ab^dagger
+    // (copy_2c_Fock_from_device_to_host, cuda_common.cu:176-205)
+    std::vector<int64_t> in, out;
+    std::vector<int> tr, cj;
+    for (int ik = 0; ik < sym.ink; ++ik)
+      for (int b = 0; b < 4; ++b) {
+        in.push_back(((int64_t)ik * 3 + (b == 3 ? 2 : b)) * n2);
+        out.push_back(ik * so2 + (int64_t)kX2cBlock[b][0] * nao * nso + (int64_t)kX2cBlock[b][1] * nao);
+        tr.push_back(b == 3);
+        cj.push_back(b == 3);
+      }
+    x2c_merge.build(in, out, tr, cj);
+    rot.init(sym, nso, 1, 1, stream);
+  } else {
+    rot.init(sym, nao, ns, 1, stream);
+  }
 }
 
 HfPlan::~HfPlan() {
@@ -820,7 +983,7 @@ HfPlan::~HfPlan() {
 // buffers, so each launch has nsplit * ns workgroups instead of ns.
 void HfPlan::exchange_device() {
   const int64_t n2 = (int64_t)nao * nao, stride = (int64_t)NQ * n2;
-  const double px = (ns == 1) ? -0.5 : -1.0;
+  const double px = (nps == 1) ? -0.5 : -1.0;   // -1 for ns = 2 and the X2C blocks
   z64* Vdir = work.get();
   z64* X = Vdir + kb * stride;
   const int64_t xs = (int64_t)kb * stride;          // one spin slab of X
@@ -828,13 +991,13 @@ void HfPlan::exchange_device() {
     zero(Kpart.get(), Kpart.size(), stream);
     for (int i0 = 0; i0 < sym.nk; i0 += kb) {
       const int n = std::min(kb, sym.nk - i0);
-      unfold_pairs(store, tables[ik], i0, n, bidx_off.get(), Vdir, NQ, nao, stream);
+      unfold_pairs(store, tables[ik], i0, n, bidx_off.get(), Vdir, 0, NQ, nao, stream);
       for (int b = 0; b < n; ++b) {
         const int64_t k2 = i0 + b;
         GemmDesc g;
-        g.M = NQ * nao; g.N = nao; g.K = nao; g.nb0 = ns;
+        g.M = NQ * nao; g.N = nao; g.K = nao; g.nb0 = nps;
         g.A.ptr = Vdir + b * stride; g.A.rs = nao; g.A.cs = 1;
-        g.B.ptr = dm_fbz.get() + k2 * ns * n2; g.B.rs = nao; g.B.cs = 1; g.B.bs0 = n2;
+        g.B.ptr = dm_fbz.get() + k2 * nps * n2; g.B.rs = nao; g.B.cs = 1; g.B.bs0 = n2;
         g.C.ptr = X + b * stride; g.C.rs = nao; g.C.cs = 1; g.C.bs0 = xs;
         gemm_on(g, work, dm_fbz, work, stream, kSiteHF);
       }
@@ -847,17 +1010,18 @@ void HfPlan::exchange_device() {
         const int kbat = part == 0 ? ch : rem;
         const int64_t first = part == 0 ? 0 : (int64_t)full * ch;
         GemmDesc f;
-        f.M = nao; f.N = nao; f.K = nao; f.KB = kbat; f.nb0 = ns; f.nb1 = slices;
+        f.M = nao; f.N = nao; f.K = nao; f.KB = kbat; f.nb0 = nps; f.nb1 = slices;
         f.A.ptr = X + first * n2; f.A.rs = nao; f.A.cs = 1; f.A.kbs = n2; f.A.bs0 = xs; f.A.bs1 = (int64_t)ch * n2;
         f.B.ptr = Vdir + first * n2; f.B.rs = 1; f.B.cs = nao; f.B.kbs = n2; f.B.bs0 = 0; f.B.bs1 = (int64_t)ch * n2;
         f.B.conj = 1;
-        f.C.ptr = Kpart.get() + (part == 0 ? 0 : (int64_t)full * ns * n2); f.C.rs = nao; f.C.cs = 1;
-        f.C.bs0 = n2; f.C.bs1 = (int64_t)ns * n2;
+        f.C.ptr = Kpart.get() + (part == 0 ? 0 : (int64_t)full * nps * n2); f.C.rs = nao; f.C.cs = 1;
+        f.C.bs0 = n2; f.C.bs1 = (int64_t)nps * n2;
         f.alpha_re = px / sym.nk; f.beta_re = 1.0;
         gemm_on(f, work, work, Kpart, stream, kSiteHF);
       }
     }
-    sum_partials(Kpart.get(), (int64_t)ns * n2, nsplit, Kout.get() + (int64_t)ik * ns * n2, (int64_t)ns * n2, stream);
+    sum_partials(Kpart.get(), (int64_t)nps * n2, nsplit, Kout.get() + (int64_t)ik * nps * n2, (int64_t)nps * n2,
+                 stream);
   }
 }
 
@@ -868,61 +1032,105 @@ static void ibz_to_host_layout(const std::vector<z64>& h, double* out, int ink, 
       std::memcpy(o + ((int64_t)s * ink + ik) * n2, h.data() + ((int64_t)ik * ns + s) * n2, n2 * sizeof(z64));
 }
 
+void HfPlan::stage_dm() {
+  if (x2c) {
+    rot.run(dm_host_dev.get(), dm2c_fbz.get(), stream);   // (nk, 1, 1, nso, nso)
+    spinor_copy(x2c_split, dm2c_fbz.get(), dm2c_fbz.size(), nso, 0, dm_fbz.get(), dm_fbz.size(), nao, 0, nao, 1, 1.0,
+                stream);
+  } else {
+    rot.run(dm_host_dev.get(), dm_fbz.get(), stream);     // (nk, ns, 1, nao, nao)
+  }
+}
+
+void HfPlan::fetch(double* out) {
+  const int64_t n2 = (int64_t)nao * nao;
+  if (x2c) {
+    spinor_copy(x2c_merge, Kout.get(), Kout.size(), nao, 0, F2c.get(), F2c.size(), nso, 0, nao, 1, 1.0, stream);
+    GGPU_HIP(hipMemcpyAsync(out, F2c.get(), F2c.bytes(), hipMemcpyDeviceToHost, stream));
+    GGPU_HIP(hipStreamSynchronize(stream));
+    return;
+  }
+  std::vector<z64> h(Kout.size());
+  GGPU_HIP(hipMemcpyAsync(h.data(), Kout.get(), Kout.bytes(), hipMemcpyDeviceToHost, stream));
+  GGPU_HIP(hipStreamSynchronize(stream));
+  ibz_to_host_layout(h, out, sym.ink, ns, n2);
+}
+
 void HfPlan::exchange(const double* dm, double* K) {
   require(store.loaded, "hf: integrals not loaded", kErrState);
   require(dm && K, "hf: NULL pointer");
   GGPU_HIP(hipMemcpyAsync(dm_host_dev.get(), dm, dm_host_dev.bytes(), hipMemcpyHostToDevice, stream));
-  rot.run(dm_host_dev.get(), dm_fbz.get(), stream);   // (nk, ns, 1, nao, nao)
+  stage_dm();
   exchange_device();
-  std::vector<z64> h(Kout.size());
-  GGPU_HIP(hipMemcpyAsync(h.data(), Kout.get(), Kout.bytes(), hipMemcpyDeviceToHost, stream));
-  GGPU_HIP(hipStreamSynchronize(stream));
-  ibz_to_host_layout(h, K, sym.ink, ns, (int64_t)nao * nao);
+  fetch(K);
 }
 
-// F = J + K - c madelung S D S  (hf_gpu_kernel.cpp:66-104, :141-201)
+// F = J + K - c madelung S D S  (hf_gpu_kernel.cpp:66-104, :141-201); the
+// two-component variant follows hf_gpu_kernel.cpp:251-320.
 void HfPlan::solve(const double* dm, const double* S_k, double madelung, double* F) {
   require(store.loaded, "hf: integrals not loaded", kErrState);
   require(dm && S_k && F, "hf: NULL pointer");
-  const int64_t n2 = (int64_t)nao * nao, stride = (int64_t)NQ * n2;
+  const int64_t n2 = (int64_t)nao * nao, stride = (int64_t)NQ * n2, so2 = (int64_t)nso * nso;
   GGPU_HIP(hipMemcpyAsync(dm_host_dev.get(), dm, dm_host_dev.bytes(), hipMemcpyHostToDevice, stream));
   GGPU_HIP(hipMemcpyAsync(S_dev.get(), S_k, S_dev.bytes(), hipMemcpyHostToDevice, stream));
-  rot.run(dm_host_dev.get(), dm_fbz.get(), stream);
+  stage_dm();
   exchange_device();                                   // Kout[ik][s]
   // Hartree: jvec[Q] = (1/nk) sum_{k', s} sum_{a,b} V^Q(k',k')[a][b] D_s(k')[b][a]
+  // (X2C: the density is D_aa + D_bb, blocks 0 and 1)
+  const int nj = x2c ? 2 : ns;
   z64* Vdir = work.get();
   zero(jvec.get(), jvec.size(), stream);
   for (int i0 = 0; i0 < sym.nk; i0 += kb) {
     const int n = std::min(kb, sym.nk - i0);
-    unfold_pairs(store, diag_tables[0], i0, n, bidx_off.get(), Vdir, NQ, nao, stream);
+    unfold_pairs(store, diag_tables[0], i0, n, bidx_off.get(), Vdir, 0, NQ, nao, stream);
     for (int b = 0; b < n; ++b)
-      for (int s = 0; s < ns; ++s) {
+      for (int s = 0; s < nj; ++s) {
         GemmDesc g;   // (NQ x 1) = sum_a sum_b V[Q][a][b] D[b][a]: K index b, K-batch a
         g.M = NQ; g.N = 1; g.K = nao; g.KB = nao;
         g.A.ptr = Vdir + b * stride; g.A.rs = n2; g.A.cs = 1; g.A.kbs = nao;
-        g.B.ptr = dm_fbz.get() + ((int64_t)(i0 + b) * ns + s) * n2; g.B.rs = nao; g.B.cs = 0; g.B.kbs = 1;
+        g.B.ptr = dm_fbz.get() + ((int64_t)(i0 + b) * nps + s) * n2; g.B.rs = nao; g.B.cs = 0; g.B.kbs = 1;
         g.C.ptr = jvec.get(); g.C.rs = 1; g.C.cs = 1;
         g.alpha_re = 1.0 / sym.nk; g.beta_re = 1.0;
         gemm_on(g, work, dm_fbz, jvec, stream, kSiteHF);
       }
   }
   // F[ik][s] = K + sum_Q jvec[Q] V^Q(k,k) - c madelung S D S
-  const double c = (ns == 2) ? 1.0 : 0.5;
   for (int i0 = 0; i0 < sym.ink; i0 += kb) {
     const int n = std::min(kb, sym.ink - i0);
-    unfold_pairs(store, diag_tables[1], i0, n, bidx_off.get(), Vdir, NQ, nao, stream);
+    unfold_pairs(store, diag_tables[1], i0, n, bidx_off.get(), Vdir, 0, NQ, nao, stream);
     for (int b = 0; b < n; ++b) {
-      GemmDesc g;   // (1 x nao^2) += jvec^T (1 x NQ) . V (NQ x nao^2), for both spins
-      g.M = 1; g.N = (int)n2; g.K = NQ; g.nb0 = ns;
+      GemmDesc g;   // (1 x nao^2) += jvec^T (1 x NQ) . V (NQ x nao^2), for every spin (X2C: aa, bb)
+      g.M = 1; g.N = (int)n2; g.K = NQ; g.nb0 = nj;
       g.A.ptr = jvec.get(); g.A.rs = 0; g.A.cs = 1;
       g.B.ptr = Vdir + b * stride; g.B.rs = n2; g.B.cs = 1;
-      g.C.ptr = Kout.get() + (int64_t)(i0 + b) * ns * n2; g.C.rs = 0; g.C.cs = 1; g.C.bs0 = n2;
+      g.C.ptr = Kout.get() + (int64_t)(i0 + b) * nps * n2; g.C.rs = 0; g.C.cs = 1; g.C.bs0 = n2;
       g.beta_re = 1.0;
       gemm_on(g, jvec, work, Kout, stream, kSiteHF);
     }
   }
-  // Ewald: Ftmp[s][ik] = S D, then Kout[ik][s] -= c madelung (S D) S
-  {
+  if (x2c) {
+    // Ewald on the aa, bb, ab blocks with the AO overlap S_aa on both sides
+    // (x2c_hf_gpu_kernel::add_Ewald, hf_gpu_kernel.cpp:300-320); ba follows
+    // as the adjoint of ab when the blocks are merged
+    for (int b = 0; b < 3; ++b) {
+      GemmDesc g;
+      g.M = nao; g.N = nao; g.K = nao; g.nb0 = sym.ink;
+      g.A.ptr = S_dev.get(); g.A.rs = nso; g.A.cs = 1; g.A.bs0 = so2;
+      g.B.ptr = dm_host_dev.get() + (int64_t)kX2cBlock[b][0] * nao * nso + (int64_t)kX2cBlock[b][1] * nao;
+      g.B.rs = nso; g.B.cs = 1; g.B.bs0 = so2;
+      g.C.ptr = Ftmp.get() + b * n2; g.C.rs = nao; g.C.cs = 1; g.C.bs0 = 3 * n2;
+      gemm_on(g, S_dev, dm_host_dev, Ftmp, stream, kSiteHF);
+    }
+    GemmDesc h;
+    h.M = nao; h.N = nao; h.K = nao; h.nb0 = sym.ink; h.nb1 = 3;
+    h.A.ptr = Ftmp.get(); h.A.rs = nao; h.A.cs = 1; h.A.bs0 = 3 * n2; h.A.bs1 = n2;
+    h.B.ptr = S_dev.get(); h.B.rs = nso; h.B.cs = 1; h.B.bs0 = so2; h.B.bs1 = 0;
+    h.C.ptr = Kout.get(); h.C.rs = nao; h.C.cs = 1; h.C.bs0 = 3 * n2; h.C.bs1 = n2;
+    h.alpha_re = -madelung; h.beta_re = 1.0;
+    gemm_on(h, Ftmp, S_dev, Kout, stream, kSiteHF);
+  } else {
+    // Ewald: Ftmp[s][ik] = S D, then Kout[ik][s] -= c madelung (S D) S
+    const double c = (ns == 2) ? 1.0 : 0.5;
     GemmDesc g;
     g.M = nao; g.N = nao; g.K = nao; g.nb0 = ns * sym.ink;
     g.A.ptr = S_dev.get(); g.A.rs = nao; g.A.cs = 1; g.A.bs0 = n2;
@@ -939,10 +1147,7 @@ void HfPlan::solve(const double* dm, const double* S_k, double madelung, double*
       gemm_on(h, Ftmp, S_dev, Kout, stream, kSiteHF);
     }
   }
-  std::vector<z64> h(Kout.size());
-  GGPU_HIP(hipMemcpyAsync(h.data(), Kout.get(), Kout.bytes(), hipMemcpyDeviceToHost, stream));
-  GGPU_HIP(hipStreamSynchronize(stream));
-  ibz_to_host_layout(h, F, sym.ink, ns, n2);
+  fetch(F);
 }
 
 }  // namespace ggpu

@@ -53,7 +53,7 @@ struct PairTable {
 
 // Unfold pairs [i0, i0+n) of a PairTable into dst[b] (NQ x nao x nao each).
 void unfold_pairs(const IntegralStore& st, const PairTable& tab, int i0, int n, const int64_t* out_off_dev,
-                  z64* dst, int NQ, int nao, hipStream_t s);
+                  void* dst, int out_f32, int NQ, int nao, hipStream_t s);
 
 // Rotate IBZ matrices into the full BZ: out[k][s][t] = U_k X[t][s][ibz(k)] U_k^dagger
 // (conjugated for time-reversal images).  X has host layout (nt, ns, ink, n, n),
@@ -89,8 +89,23 @@ class SiteProfiler : public GemmProfiler {
   size_t used_ = 0;
 };
 
+// Tables of a BlockCopy between spinor matrices (nso = 2 nao) and their
+// nao x nao spin blocks, one entry per outer batch.
+struct SpinorTables {
+  DeviceBuffer<int64_t> in_off, out_off;
+  DeviceBuffer<int> trans, conj;
+  std::vector<int64_t> h_in, h_out;   // host copies for the bounds check
+  int n = 0;
+  void build(const std::vector<int64_t>& in, const std::vector<int64_t>& out, const std::vector<int>& tr,
+             const std::vector<int>& cj);
+};
+
 struct GwPlan {
-  int nao, ns, NQ, nts, nw_b, half;
+  // ns: spin blocks the contractions run over (the two-component plan runs
+  // over the four pseudo-spins aa, bb, ab, ba); nso: size of the caller's
+  // matrices (nao, or 2 nao for X2C)
+  int nao, nso, ns, NQ, nts, nw_b, half;
+  bool x2c = false;
   Compute compute;
   int solver;
   SymTables sym;
@@ -102,6 +117,7 @@ struct GwPlan {
   DeviceBuffer<z64> T_tw;    // (nts, nw_b)
   DeviceBuffer<z64> G_host_dev, Gf, Grev, Sig_acc, Sig_host_dev;
   DeviceBuffer<int64_t> rev_src, rev_dst;   // Gf -> Grev (tau reversed) slab offsets
+  DeviceBuffer<c32> Gf32, Grev32, Pt32;     // complex64 copies for the reduced-precision modes
   int p0_split = 4;                         // P0 partial sums (split of the pair/spin reduction)
   DeviceBuffer<z64> P0, P0w, Amat, Pt, Prot, Ptmp, Uq_dev;
   DeviceBuffer<int> info;
@@ -117,10 +133,13 @@ struct GwPlan {
   SiteProfiler prof;
   std::vector<hipEvent_t> qev;          // per-q phase events
 
-  GwPlan(int nao, int ns, int NQ, int nts, int nw_b, const ggpu_symmetry& s, const double* T_wt,
+  SpinorTables x2c_split, x2c_merge;   // G spinor -> pseudo-spin blocks, Sigma back
+
+  GwPlan(int nao, int nso, int ns, int NQ, int nts, int nw_b, const ggpu_symmetry& s, const double* T_wt,
          const double* T_tw, int precision, int solver, size_t work_bytes);
   ~GwPlan();
 
+  bool lowp() const { return compute != Compute::F64; }
   void iterate_device(const z64* G_dev_hostlayout, z64* Sigma_dev_hostlayout);
   void iterate(const double* G, double* Sigma);
   void polarization(const double* G, int iq, int which, double* out);
@@ -134,7 +153,10 @@ struct GwPlan {
 };
 
 struct HfPlan {
-  int nao, ns, NQ;
+  // ns: the caller's spins; nps: density blocks the exchange runs over
+  // (3 for X2C: aa, bb, ab); nso: the caller's matrix size
+  int nao, nso, ns, nps, NQ;
+  bool x2c = false;
   SymTables sym;
   IntegralStore store;
   hipStream_t stream = nullptr;
@@ -144,15 +166,18 @@ struct HfPlan {
   std::vector<PairTable> tables;   // per IBZ k: pairs (k, k2) for all k2
   int kb = 1, nsplit = 1;
 
-  HfPlan(int nao, int ns, int NQ, const ggpu_symmetry& s);
+  HfPlan(int nao, int nso, int ns, int NQ, const ggpu_symmetry& s);
   ~HfPlan();
   void exchange(const double* dm, double* K);
   void solve(const double* dm, const double* S_k, double madelung, double* F);
 
  private:
+  void stage_dm();          // dm_host_dev -> dm_fbz (full BZ, density blocks)
   void exchange_device();   // Kout[ik][s] from dm_fbz
+  void fetch(double* out);  // Kout -> caller layout (ns, ink, nso, nso)
   std::vector<PairTable> diag_tables;  // (k, k) pairs: [0] all full k, [1] IBZ representatives
-  DeviceBuffer<z64> jvec, S_dev, Ftmp;
+  DeviceBuffer<z64> jvec, S_dev, Ftmp, dm2c_fbz, F2c;
+  SpinorTables x2c_split, x2c_merge;
 };
 
 }  // namespace ggpu

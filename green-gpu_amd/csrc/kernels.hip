@@ -1,6 +1,7 @@
 // Bandwidth-bound and small-matrix kernels of the HF/GW path (gfx950).
 #include "kernels.hpp"
 
+#include <algorithm>
 #include <cmath>
 
 namespace ggpu {
@@ -12,14 +13,23 @@ __device__ inline z64 cj(z64 a) { return z64{a.x, -a.y}; }
 __device__ inline z64 cmul(z64 a, z64 b) { return z64{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
 __device__ inline z64 csub(z64 a, z64 b) { return z64{a.x - b.x, a.y - b.y}; }
 
+__device__ inline c32 narrow(z64 v) { return c32{(float)v.x, (float)v.y}; }
+__device__ inline c32 narrow(c32 v) { return v; }
+template <class T> __device__ inline T cjt(T a) { return T{a.x, -a.y}; }
+template <class T> __device__ inline T scl(T a, double f) {
+  using R = decltype(a.x);
+  return T{(R)(f * a.x), (R)(f * a.y)};
+}
+
 // 32 x 32 tiles through LDS; block = 32 x 8 threads.
+template <class TI, class TO>
 __global__ __launch_bounds__(256) void block_copy_kernel(const BlockCopy a) {
-  __shared__ z64 tile[TILE][TILE + 1];
+  __shared__ TO tile[TILE][TILE + 1];
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
   const int c0 = blockIdx.x * TILE, r0 = blockIdx.y * TILE;
   const int outer = blockIdx.z / a.n_inner, inner = blockIdx.z - outer * a.n_inner;
-  const z64* in = a.in + (a.in_off ? a.in_off[outer] : 0) + inner * a.in_bs;
-  z64* out = a.out + (a.out_off ? a.out_off[outer] : 0) + inner * a.out_bs;
+  const TI* in = static_cast<const TI*>(a.in) + (a.in_off ? a.in_off[outer] : 0) + inner * a.in_bs;
+  TO* out = static_cast<TO*>(a.out) + (a.out_off ? a.out_off[outer] : 0) + inner * a.out_bs;
   const bool tr = a.trans ? a.trans[outer] != 0 : a.trans_all != 0;
   const bool cg = a.conj ? a.conj[outer] != 0 : a.conj_all != 0;
   if (tr) {
@@ -27,15 +37,19 @@ __global__ __launch_bounds__(256) void block_copy_kernel(const BlockCopy a) {
 #pragma unroll
     for (int j = 0; j < TILE; j += 8) {
       const int c = c0 + ty + j, r = r0 + tx;
-      if (c < a.cols && r < a.rows) tile[ty + j][tx] = in[c * a.in_ld + r];
+      if (c < a.cols && r < a.rows) {
+        if constexpr (sizeof(TI) == sizeof(TO)) tile[ty + j][tx] = in[c * a.in_ld + r];
+        else tile[ty + j][tx] = narrow(in[c * a.in_ld + r]);
+      }
     }
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < TILE; j += 8) {
       const int r = r0 + ty + j, c = c0 + tx;
       if (r < a.rows && c < a.cols) {
-        z64 v = tile[tx][ty + j];
-        out[r * a.out_ld + c] = cg ? cj(v) : v;
+        TO v = tile[tx][ty + j];
+        if (a.scale != 1.0) v = scl(v, a.scale);
+        out[r * a.out_ld + c] = cg ? cjt(v) : v;
       }
     }
   } else {
@@ -43,11 +57,19 @@ __global__ __launch_bounds__(256) void block_copy_kernel(const BlockCopy a) {
     for (int j = 0; j < TILE; j += 8) {
       const int r = r0 + ty + j, c = c0 + tx;
       if (r < a.rows && c < a.cols) {
-        z64 v = in[r * a.in_ld + c];
-        out[r * a.out_ld + c] = cg ? cj(v) : v;
+        TO v;
+        if constexpr (sizeof(TI) == sizeof(TO)) v = in[r * a.in_ld + c];
+        else v = narrow(in[r * a.in_ld + c]);
+        if (a.scale != 1.0) v = scl(v, a.scale);
+        out[r * a.out_ld + c] = cg ? cjt(v) : v;
       }
     }
   }
+}
+
+__global__ void to_c32_kernel(const z64* in, c32* out, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = c32{(float)in[i].x, (float)in[i].y};
 }
 
 __global__ void slab_gather_kernel(const z64* in, const int64_t* src_off, z64* out,
@@ -286,12 +308,24 @@ inline int grid_for(int64_t n, int block) {
 }  // namespace
 
 void block_copy(const BlockCopy& a, hipStream_t s) {
-  if (a.rows <= 0 || a.cols <= 0) return;
-  require(a.n_outer * (int64_t)a.n_inner <= 2147483647, "block_copy: batch too large");
-  dim3 grid((a.cols + TILE - 1) / TILE, (a.rows + TILE - 1) / TILE, a.n_outer * a.n_inner);
-  require(grid.z <= 65535u, "block_copy: batch above 65535");
-  hipLaunchKernelGGL(block_copy_kernel, grid, dim3(256), 0, s, a);
-  GGPU_HIP(hipGetLastError());
+  if (a.rows <= 0 || a.cols <= 0 || a.n_outer <= 0 || a.n_inner <= 0) return;
+  require(a.n_inner <= 65535, "block_copy: inner batch above 65535");
+  require(!(a.in_f32 && !a.out_f32), "block_copy: complex64 -> complex128 not supported");
+  // grid.z is limited to 65535: launch the outer batches in chunks
+  const int per = 65535 / a.n_inner;
+  for (int o0 = 0; o0 < a.n_outer; o0 += per) {
+    BlockCopy c = a;
+    c.n_outer = std::min(per, a.n_outer - o0);
+    if (c.in_off) c.in_off += o0;
+    if (c.out_off) c.out_off += o0;
+    if (c.trans) c.trans += o0;
+    if (c.conj) c.conj += o0;
+    dim3 grid((c.cols + TILE - 1) / TILE, (c.rows + TILE - 1) / TILE, c.n_outer * c.n_inner);
+    if (!c.in_f32 && !c.out_f32) hipLaunchKernelGGL((block_copy_kernel<z64, z64>), grid, dim3(256), 0, s, c);
+    else if (!c.in_f32) hipLaunchKernelGGL((block_copy_kernel<z64, c32>), grid, dim3(256), 0, s, c);
+    else hipLaunchKernelGGL((block_copy_kernel<c32, c32>), grid, dim3(256), 0, s, c);
+    GGPU_HIP(hipGetLastError());
+  }
 }
 
 void slab_gather(const z64* in, const int64_t* src_off, z64* out, const int64_t* dst_off, int n_slabs,
@@ -307,6 +341,12 @@ void herm_one_minus(z64* M, z64* A, int n, int nw, hipStream_t s) {
   const int T = (n + TILE - 1) / TILE;
   require(nw <= 65535, "herm_one_minus: too many matrices");
   hipLaunchKernelGGL(herm_one_minus_kernel, dim3(T * (T + 1) / 2, nw), dim3(256), 0, s, M, A, n);
+  GGPU_HIP(hipGetLastError());
+}
+
+void to_c32(const z64* in, c32* out, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(to_c32_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, in, out, n);
   GGPU_HIP(hipGetLastError());
 }
 

@@ -9,9 +9,11 @@ namespace ggpu {
 // (trans = 1, reads in[b][c][r]) of an rows x cols block, optionally
 // complex-conjugated.  Batch b = (outer, inner): element offsets are
 // in_off[outer] + inner * in_bs (same for out); flags are per outer batch.
+// Elements are multiplied by `scale`; a null in_off/out_off means offset 0.
 struct BlockCopy {
-  const z64* in = nullptr;
-  z64* out = nullptr;
+  const void* in = nullptr;          // complex128, or complex64 if in_f32
+  void* out = nullptr;               // complex128, or complex64 if out_f32
+  int in_f32 = 0, out_f32 = 0;
   int rows = 0, cols = 0;          // shape of the OUTPUT block
   int64_t in_ld = 0, out_ld = 0;   // leading dimensions (row strides)
   int n_inner = 1;
@@ -22,6 +24,7 @@ struct BlockCopy {
   const int* trans = nullptr;        // device, n_outer entries (nullptr: trans_all)
   const int* conj = nullptr;         // device, n_outer entries (nullptr: conj_all)
   int trans_all = 0, conj_all = 0;
+  double scale = 1.0;                // applied to every element
 };
 void block_copy(const BlockCopy& a, hipStream_t s);
 
@@ -34,6 +37,9 @@ void slab_gather(const z64* in, const int64_t* src_off, z64* out, const int64_t*
 // For every w: H = (M + M^dagger)/2 (real diagonal), written back to M, and
 // A = I - H written to A (if A != nullptr).  M, A: nw x n x n.
 void herm_one_minus(z64* M, z64* A, int n, int nw, hipStream_t s);
+
+// out[i] = (complex64) in[i]
+void to_c32(const z64* in, c32* out, int64_t n, hipStream_t s);
 
 // Fill n complex elements with zero.
 void zero(z64* p, int64_t n, hipStream_t s);

@@ -228,6 +228,193 @@ __global__ __launch_bounds__(NTHREADS) void gemm_mfma_kernel(const GemmDesc d) {
       }
 }
 
+// ---------------------------------------------------------------------------
+// Split-bf16 x3 kernel.  Operands are complex64 in memory; every element x is
+// split while it is staged into LDS as hi = bf16(x), lo = bf16(x - hi), and
+// each real product is formed as hi*hi + hi*lo + lo*hi on
+// v_mfma_f32_16x16x32_bf16 with FP32 accumulation (the lo*lo term, ~2^-16
+// relative, is dropped).  A complex 16x16x32 block costs 12 MFMAs:
+//   re += ar br - ai bi,  im += ar bi + ai br  (3 products each).
+// LDS holds 4 planes per operand (re_hi, im_hi, re_lo, im_lo) with k
+// contiguous per row, so every MFMA fragment is one 16-byte LDS read.
+// ---------------------------------------------------------------------------
+constexpr int XBK = 32;          // k per MFMA step
+constexpr int XPAD = 8;          // bf16 elements of row padding
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+
+__device__ inline unsigned short bf16_rn(float x) {
+  const unsigned u = __float_as_uint(x);
+  return (unsigned short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ inline float bf16_f(unsigned short h) { return __uint_as_float(((unsigned)h) << 16); }
+
+template <class TO, bool AKC, bool BKC, int SITE>
+__global__ __launch_bounds__(NTHREADS) void gemm_bf16x3_kernel(const GemmDesc d) {
+  __shared__ __attribute__((aligned(16))) unsigned short sA[4][BM][XBK + XPAD];
+  __shared__ __attribute__((aligned(16))) unsigned short sB[4][BN][XBK + XPAD];
+
+  const int tid = threadIdx.x;
+  const int tiles_m = (d.M + BM - 1) / BM;
+  const int tiles_n = (d.N + BN - 1) / BN;
+  int64_t lg = xcd_logical(blockIdx.x, gridDim.x);
+  const int m0 = (int)(lg % tiles_m) * BM;
+  lg /= tiles_m;
+  const int n0 = (int)(lg % tiles_n) * BN;
+  lg /= tiles_n;
+  const int64_t b1 = lg % d.nb1, b0 = lg / d.nb1;
+  const float2* A = static_cast<const float2*>(d.A.ptr) + b0 * d.A.bs0 + (d.A.off1 ? d.A.off1[b1] : b1 * d.A.bs1);
+  const float2* B = static_cast<const float2*>(d.B.ptr) + b0 * d.B.bs0 + (d.B.off1 ? d.B.off1[b1] : b1 * d.B.bs1);
+  TO* C = static_cast<TO*>(d.C.ptr) + b0 * d.C.bs0 + (d.C.off1 ? d.C.off1[b1] : b1 * d.C.bs1);
+
+  // loader: 8 complex elements of A and of B per thread per k-tile
+  int am[8], ak[8], bn[8], bk[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    if (AKC) { am[j] = tid >> 2; ak[j] = ((tid & 3) << 3) + j; }
+    else     { ak[j] = tid >> 3; am[j] = ((tid & 7) << 3) + j; }
+    if (BKC) { bn[j] = tid >> 2; bk[j] = ((tid & 3) << 3) + j; }
+    else     { bk[j] = tid >> 3; bn[j] = ((tid & 7) << 3) + j; }
+  }
+  const int ktiles = (d.K + XBK - 1) / XBK;
+  const int niter = d.KB * ktiles;
+  float2 ra[8], rb[8];
+  auto gload = [&](int it) {
+    const int kb = it / ktiles;
+    const int k0 = (it - kb * ktiles) * XBK;
+    const float2* Ab = A + kb * d.A.kbs;
+    const float2* Bb = B + kb * d.B.kbs;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int gm = m0 + am[j], gk = k0 + ak[j];
+      ra[j] = (gm < d.M && gk < d.K) ? Ab[gm * d.A.rs + gk * d.A.cs] : float2{0.f, 0.f};
+      const int gk2 = k0 + bk[j], gn = n0 + bn[j];
+      rb[j] = (gk2 < d.K && gn < d.N) ? Bb[gk2 * d.B.rs + gn * d.B.cs] : float2{0.f, 0.f};
+    }
+  };
+  auto split_store = [&](unsigned short (*S)[BM][XBK + XPAD], int row, int k, float re, float im) {
+    const unsigned short rh = bf16_rn(re), ih = bf16_rn(im);
+    S[0][row][k] = rh;
+    S[1][row][k] = ih;
+    S[2][row][k] = bf16_rn(re - bf16_f(rh));
+    S[3][row][k] = bf16_rn(im - bf16_f(ih));
+  };
+  auto sstore = [&]() {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      split_store(sA, am[j], ak[j], ra[j].x, d.A.conj ? -ra[j].y : ra[j].y);
+      split_store(sB, bn[j], bk[j], rb[j].x, d.B.conj ? -rb[j].y : rb[j].y);
+    }
+  };
+
+  const int lane = tid & 63, w = tid >> 6;
+  const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+  const int lr = lane & 15, lk = lane >> 4;
+  v4f cr[2][2], ci[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) { cr[i][j] = v4f{0, 0, 0, 0}; ci[i][j] = v4f{0, 0, 0, 0}; }
+
+  auto frag = [&](unsigned short (*S)[BM][XBK + XPAD], int p, int row) -> v8bf {
+    const v4u u = *reinterpret_cast<const v4u*>(&S[p][row][8 * lk]);
+    return __builtin_bit_cast(v8bf, u);
+  };
+  auto negf = [&](unsigned short (*S)[BM][XBK + XPAD], int p, int row) -> v8bf {
+    v4u u = *reinterpret_cast<const v4u*>(&S[p][row][8 * lk]);
+    u ^= v4u{0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
+    return __builtin_bit_cast(v8bf, u);
+  };
+
+  if (niter > 0) gload(0);
+  for (int it = 0; it < niter; ++it) {
+    sstore();
+    __syncthreads();
+    if (it + 1 < niter) gload(it + 1);
+    v8bf arh[2], aih[2], arl[2], ail[2], naih[2], nail[2], brh[2], bih[2], brl[2], bil[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int ra_ = wm + 16 * i + lr, rb_ = wn + 16 * i + lr;
+      arh[i] = frag(sA, 0, ra_); aih[i] = frag(sA, 1, ra_); arl[i] = frag(sA, 2, ra_); ail[i] = frag(sA, 3, ra_);
+      naih[i] = negf(sA, 1, ra_); nail[i] = negf(sA, 3, ra_);
+      brh[i] = frag(sB, 0, rb_); bih[i] = frag(sB, 1, rb_); brl[i] = frag(sB, 2, rb_); bil[i] = frag(sB, 3, rb_);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        v4f r = cr[i][j], m = ci[i][j];
+        r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arh[i], brh[j], r, 0, 0, 0);
+        m = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arh[i], bih[j], m, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arh[i], brl[j], r, 0, 0, 0);
+        m = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arh[i], bil[j], m, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arl[i], brh[j], r, 0, 0, 0);
+        m = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arl[i], bih[j], m, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(naih[i], bih[j], r, 0, 0, 0);
+        m = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aih[i], brh[j], m, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(naih[i], bil[j], r, 0, 0, 0);
+        m = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aih[i], brl[j], m, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nail[i], bih[j], r, 0, 0, 0);
+        m = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ail[i], brh[j], m, 0, 0, 0);
+        cr[i][j] = r;
+        ci[i][j] = m;
+      }
+    __syncthreads();
+  }
+
+  const bool use_beta = (d.beta_re != 0.0 || d.beta_im != 0.0);
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm + 16 * i + 4 * lk + r;
+        const int col = n0 + wn + 16 * j + lr;
+        if (row < d.M && col < d.N) {
+          const double xr = cr[i][j][r], xi = ci[i][j][r];
+          double vr = d.alpha_re * xr - d.alpha_im * xi;
+          double vi = d.alpha_re * xi + d.alpha_im * xr;
+          TO* p = C + row * d.C.rs + col * d.C.cs;
+          if (use_beta) {
+            const TO o = *p;
+            vr += d.beta_re * o.x - d.beta_im * o.y;
+            vi += d.beta_re * o.y + d.beta_im * o.x;
+          }
+          using TS = decltype(TO{}.x);
+          *p = TO{static_cast<TS>(vr), static_cast<TS>(vi)};
+        }
+      }
+}
+
+template <class TO, int SITE>
+void launch_bf16x3_site(const GemmDesc& d, hipStream_t stream) {
+  const int64_t tiles = (int64_t)((d.M + BM - 1) / BM) * ((d.N + BN - 1) / BN);
+  const int64_t total = tiles * d.nb0 * d.nb1;
+  require(total < (int64_t)1 << 31, "gemm: grid too large");
+  dim3 grid((unsigned)total), block(NTHREADS);
+  const bool akc = (d.A.cs == 1) || (d.A.rs != 1);
+  const bool bkc = (d.B.rs == 1) || (d.B.cs != 1);
+  if (akc && bkc) hipLaunchKernelGGL((gemm_bf16x3_kernel<TO, true, true, SITE>), grid, block, 0, stream, d);
+  else if (akc) hipLaunchKernelGGL((gemm_bf16x3_kernel<TO, true, false, SITE>), grid, block, 0, stream, d);
+  else if (bkc) hipLaunchKernelGGL((gemm_bf16x3_kernel<TO, false, true, SITE>), grid, block, 0, stream, d);
+  else hipLaunchKernelGGL((gemm_bf16x3_kernel<TO, false, false, SITE>), grid, block, 0, stream, d);
+  GGPU_HIP(hipGetLastError());
+}
+
+template <class TO>
+void launch_bf16x3(const GemmDesc& d, hipStream_t stream) {
+  switch (d.site) {
+    case kSiteP0X1: launch_bf16x3_site<TO, kSiteP0X1>(d, stream); break;
+    case kSiteP0X2: launch_bf16x3_site<TO, kSiteP0X2>(d, stream); break;
+    case kSiteP0: launch_bf16x3_site<TO, kSiteP0>(d, stream); break;
+    case kSiteSigY1: launch_bf16x3_site<TO, kSiteSigY1>(d, stream); break;
+    case kSiteSigY2: launch_bf16x3_site<TO, kSiteSigY2>(d, stream); break;
+    case kSiteSigma: launch_bf16x3_site<TO, kSiteSigma>(d, stream); break;
+    default: launch_bf16x3_site<TO, kSiteOther>(d, stream); break;
+  }
+}
+
 template <class T, class TO, bool GAUSS, int SITE>
 void launch_site(const GemmDesc& d, hipStream_t stream) {
   const int64_t tiles = (int64_t)((d.M + BM - 1) / BM) * ((d.N + BN - 1) / BN);
@@ -329,6 +516,10 @@ void gemm(const GemmDesc& d, hipStream_t stream) {
     case Compute::F32:
       if (d.out_f32) launch_typed<float, float2>(d, stream);
       else launch_typed<float, double2>(d, stream);
+      break;
+    case Compute::BF16X3:
+      if (d.out_f32) launch_bf16x3<float2>(d, stream);
+      else launch_bf16x3<double2>(d, stream);
       break;
     default:
       throw Error(kErrArgument, "gemm: compute mode not available");

@@ -72,7 +72,7 @@ typedef struct {
   const int64_t* q_star_members;     /* [q_star_offsets[inq]]                  */
   const int64_t* k1_from_k2q;        /* [nk*nq]  k1 = k2 + q, index k2*nq + q  */
   const int64_t* k2_from_k1q;        /* [nk*nq]  k2 = k1 - q, index k1*nq + q  */
-  const double* U_k;                 /* [nk][nao][nao] complex, or NULL        */
+  const double* U_k;                 /* [nk][nso][nso] complex, or NULL        */
   const double* U_q;                 /* [nq][NQ][NQ] complex, or NULL          */
   const int64_t* conj_kpair;         /* [nk(nk+1)/2] or NULL                   */
   const int64_t* trans_kpair;        /* [nk(nk+1)/2] or NULL                   */
@@ -94,9 +94,16 @@ int ggpu_device_memory(size_t* free_bytes, size_t* total_bytes);
 
 /* ------------------------------------------------------------------------
  * HF exchange  (replaces scalar_hf_gpu_kernel::compute_exchange_selfenergy,
- * hf_gpu_kernel.cpp:106-139, and cuhf_utils, cuhf_utils.cu:32-159)
+ * hf_gpu_kernel.cpp:106-139, x2c_hf_gpu_kernel, hf_gpu_kernel.cpp:229-338,
+ * and cuhf_utils, cuhf_utils.cu:32-159)
+ *
+ * nso = nao: scalar plan, ns = 1 or 2.
+ * nso = 2 nao: two-component (X2C) plan, ns = 1; density and Fock matrices
+ * are nso x nso spinor matrices (custom_hf_kernel(X2C = nso != nao, ...),
+ * gpu_factory.h:47-62).  The k-point tables must not contain time-reversal
+ * images for X2C (their spinor rotation is not part of this library).
  * ------------------------------------------------------------------------ */
-int ggpu_hf_plan_create(ggpu_hf_plan** plan, int nao, int ns, int NQ, const ggpu_symmetry* sym);
+int ggpu_hf_plan_create(ggpu_hf_plan** plan, int nao, int nso, int ns, int NQ, const ggpu_symmetry* sym);
 
 /* Copy the stored integral pairs (num_kpair_stored x NQ x nao x nao complex,
  * the buffer df_integral_t::read_entire fills, df_integral_t.h:96-129) to the
@@ -108,16 +115,21 @@ int ggpu_hf_load_integrals(ggpu_hf_plan* plan, const double* V_stored);
 int ggpu_hf_synth_integrals(ggpu_hf_plan* plan, uint64_t seed, double sigma);
 
 /* K_s(k) for the IBZ k:  K = (px/nk) sum_{k2,Q} V^Q(k,k2) D_s(k2) V^Q(k,k2)^dagger,
- * px = -0.5 (ns = 1) or -1 (ns = 2).  dm: (ns, ink, nao, nao) on the IBZ (the
+ * px = -0.5 (ns = 1) or -1 (ns = 2).  dm: (ns, ink, nso, nso) on the IBZ (the
  * library unfolds it to the full BZ, hf_gpu_kernel.cpp:203-209).
- * K_out: (ns, ink, nao, nao), overwritten. */
+ * K_out: (ns, ink, nso, nso), overwritten.
+ * X2C: the aa, bb and ab blocks are the exchange of the matching density
+ * blocks with px = -1 and K_ba = K_ab^dagger (cuda_common.cu:176-205). */
 int ggpu_hf_exchange(ggpu_hf_plan* plan, const double* dm, double* K_out);
 
 /* Full HF self-energy, as scalar_hf_gpu_kernel::solve returns it
  * (hf_gpu_kernel.cpp:66-104, without the MPI reduction):
  *   F = J + K - c * madelung * S D S,   c = 1 (ns = 2) or 0.5 (ns = 1),
  *   J(k) = sum_Q V^Q(k,k) (1/nk) sum_{s,k'} Tr[V^Q(k',k') D_s(k')]  (hf_gpu_kernel.cpp:141-186).
- * dm, S_k, F_out: (ns, ink, nao, nao) on the IBZ. */
+ * dm, S_k, F_out: (ns, ink, nso, nso) on the IBZ.
+ * X2C (hf_gpu_kernel.cpp:251-320): J from D_aa + D_bb on both diagonal
+ * blocks; Ewald madelung * S_aa D_xy S_aa on the aa, bb, ab blocks and its
+ * adjoint on ba. */
 int ggpu_hf_solve(ggpu_hf_plan* plan, const double* dm, const double* S_k, double madelung, double* F_out);
 
 void ggpu_hf_plan_destroy(ggpu_hf_plan* plan);
@@ -130,8 +142,12 @@ void ggpu_hf_plan_destroy(ggpu_hf_plan* plan);
 /* T_wt: (nw_b, nts-2) complex  (transformer_t::Tnt_BF, cugw_qpt.cu:329-339)
  * T_tw: (nts, nw_b)   complex  (transformer_t::Ttn_FB, cugw_qpt.cu:341-358)
  * nts must be even and nts >= nw_b (gw_gpu_kernel.h:67, cugw_qpt.cu:79).
- * work_bytes: device workspace budget for pair batches (0 = automatic). */
-int ggpu_gw_plan_create(ggpu_gw_plan** plan, int nao, int ns, int NQ, int nts, int nw_b,
+ * work_bytes: device workspace budget for pair batches (0 = automatic).
+ * nso = nao: scalar plan (ns = 1 or 2).  nso = 2 nao: two-component plan
+ * (ns = 1), whose contractions run on the pseudo-spins aa, bb, ab, ba with
+ * G(k, beta - tau) spin-swapped (x2c_gw_gpu_kernel, gw_gpu_kernel.cpp:457-551,
+ * cugw_qkpt.cu:381-403); no time-reversal k images. */
+int ggpu_gw_plan_create(ggpu_gw_plan** plan, int nao, int nso, int ns, int NQ, int nts, int nw_b,
                         const ggpu_symmetry* sym, const double* T_wt, const double* T_tw,
                         int precision, int solver, size_t work_bytes);
 
@@ -139,7 +155,7 @@ int ggpu_gw_load_integrals(ggpu_gw_plan* plan, const double* V_stored);
 int ggpu_gw_synth_integrals(ggpu_gw_plan* plan, uint64_t seed, double sigma);
 
 /* One GW iteration: Sigma(tau) from G(tau).
- * G, Sigma_out: (nts, ns, ink, nao, nao) complex, Sigma_out overwritten
+ * G, Sigma_out: (nts, ns, ink, nso, nso) complex, Sigma_out overwritten
  * (already divided by nk, as gw_gpu_kernel.cpp:188 leaves it). */
 int ggpu_gw_iterate(ggpu_gw_plan* plan, const double* G, double* Sigma_out);
 

@@ -75,6 +75,20 @@ def test_bad_spin_count_rejected(ggpu, synth):
     expect_error(lambda: ggpu.GwKernel(4, 3, 6, T_wt, T_tw, sym), 1, "ns must be 1 or 2")
 
 
+def test_bad_spinor_dimensions_rejected(ggpu, synth):
+    """nso must be nao (scalar) or 2 nao (X2C, gpu_factory.h:47-89 pick the
+    X2C kernels when nso != nao); X2C runs with ns = 1."""
+    tau, T_wt, T_tw = ir(synth)
+    sym = small_tables(ggpu)
+    expect_error(lambda: ggpu.HfKernel(4, 1, 6, sym, nso=6), 1, "nso must be nao or 2 nao")
+    expect_error(lambda: ggpu.GwKernel(4, 1, 6, T_wt, T_tw, sym, nso=5), 1, "nso must be nao or 2 nao")
+    expect_error(lambda: ggpu.HfKernel(4, 2, 6, sym, nso=8), 1, "needs ns = 1")
+    expect_error(lambda: ggpu.GwKernel(4, 2, 6, T_wt, T_tw, sym, nso=8), 1, "needs ns = 1")
+    tr = small_tables(ggpu)
+    tr.k_tr_conj = np.array([0, 1])
+    expect_error(lambda: ggpu.GwKernel(4, 1, 6, T_wt, T_tw, tr, nso=8), 1, "time-reversal")
+
+
 def test_bad_symmetry_tables_rejected(ggpu, synth):
     tau, T_wt, T_tw = ir(synth)
     sym = small_tables(ggpu)
@@ -94,10 +108,10 @@ def test_unknown_solver_and_null_args(ggpu, synth):
     T_wt = np.ascontiguousarray(T_wt)
     T_tw = np.ascontiguousarray(T_tw)
     pd = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
-    rc = L.ggpu_gw_plan_create(C.byref(h), 4, 1, 6, T_tw.shape[0], T_wt.shape[0], C.byref(cs),
+    rc = L.ggpu_gw_plan_create(C.byref(h), 4, 4, 1, 6, T_tw.shape[0], T_wt.shape[0], C.byref(cs),
                                pd(T_wt), pd(T_tw), 0, 7, 0)
     assert rc == 1 and b"unknown linear solver" in L.ggpu_last_error()
-    rc = L.ggpu_gw_plan_create(C.byref(h), 4, 1, 6, T_tw.shape[0], T_wt.shape[0], None,
+    rc = L.ggpu_gw_plan_create(C.byref(h), 4, 4, 1, 6, T_tw.shape[0], T_wt.shape[0], None,
                                pd(T_wt), pd(T_tw), 0, 0, 0)
     assert rc == 1
     rc = L.ggpu_gw_iterate(None, None, None)

@@ -5,10 +5,12 @@ Tolerances: the plain FP64 path must match the FP64 oracle to 1e-10 max-abs
 on Sigma(tau) and K(k) (and on the intermediates P0, P(Omega)); the HF
 fixture is checked at the reference's own 1e-8 (test/cu_solver_test.cpp:95).
 """
+import os
+
 import numpy as np
 import pytest
 
-from conftest import max_abs
+from conftest import ROOT, max_abs
 
 pytestmark = pytest.mark.gpu
 
@@ -100,6 +102,94 @@ def test_gw_symmetry_fixture_system(ggpu, synth, oracle, fx):
     assert max_abs(Sig, Sig_or) < TOL
 
 
+# The reference holds its double and single-precision runs to 1e-6
+# (test/cu_solver_test.cpp:287).  The split-bf16 x3 mode has no reference
+# counterpart: each operand keeps ~16 significant bits (hi + lo), so a
+# product carries up to ~1e-5 relative error and the fixture's Sigma
+# (|Sigma| <= 0.054) lands near 1e-5; it is held to 3e-5.
+FIXTURE_TOL = {"fp64": 1e-6, "fp32": 1e-6, "bf16x3": 3e-5}
+
+
+@pytest.mark.parametrize("solver,precision", [("lu", "fp64"), ("cholesky", "fp64"),
+                                              ("lu", "fp32"), ("cholesky", "bf16x3")])
+def test_gw_reference_fixture(ggpu, oracle, fx, solver, precision):
+    """The reference's GW regression (test/cu_solver_test.cpp:226-315): one GW
+    iteration on GW/data.h5:/G_tau must reproduce /result/Sigma_tau within
+    1e-6, for LU and Cholesky and for the single-precision variant.  The tau
+    grid and transforms are the ones recovered from the fixture
+    (tests/golden/make_gw_grid.py); FP64 must also match the oracle at 1e-10."""
+    g = np.load(os.path.join(ROOT, "tests", "golden", "fixtures", "gw_grid_beta100_lambda1e4.npz"))
+    NQ, ns, nao = int(fx["params_NQ"]), int(fx["params_ns"]), int(fx["params_nao"])
+    sym = oracle.symmetry_from_fixture(fx, NQ)
+    G, V, ref = fx["gw_G_tau"], fx["gw_VQ_0"], fx["gw_Sigma_tau"]
+    print(f"GW reference fixture solver={solver} precision={precision}", flush=True)
+    kern = ggpu.GwKernel(nao, ns, NQ, g["T_wt"], g["T_tw"], sym, precision=precision, solver=solver)
+    kern.load_integrals(V)
+    Sig = kern.solve(G)
+    err = max_abs(Sig, ref)
+    print(f"max-abs vs Sigma_tau {err:.2e}", flush=True)
+    assert err < FIXTURE_TOL[precision]
+    if precision == "fp64":
+        assert max_abs(Sig, oracle.gw_solve(G, V, sym, g["T_wt"], g["T_tw"], solver=solver)) < TOL
+
+
+@pytest.fixture(scope="module")
+def fx2c():
+    return dict(np.load(os.path.join(ROOT, "tests", "golden", "fixtures", "ref_fixtures_x2c.npz")))
+
+
+def test_hf_x2c_reference_fixture(ggpu, oracle, fx2c):
+    """The reference's HF_X2C regression (test/cu_solver_test.cpp:331-334):
+    two-component HF (nao 10, nso 20, NQ 154, one k-point); -F == Sigma1."""
+    nao, nso, NQ = int(fx2c["params_nao"]), int(fx2c["params_nso"]), int(fx2c["params_NQ"])
+    sym = oracle.symmetry_from_fixture(fx2c, NQ)
+    S_ibz = fx2c["S_k"][:, sym.k_reduced_to_full]
+    dm = fx2c["hf_G_tau"][-1]
+    print(f"HF X2C fixture nao={nao} nso={nso} NQ={NQ}", flush=True)
+    k = ggpu.HfKernel(nao, 1, NQ, sym, madelung=float(fx2c["madelung"]), S_k=S_ibz, nso=nso)
+    k.load_integrals(fx2c["hf_VQ_0"])
+    F = k.solve(dm)
+    assert max_abs(-F, fx2c["hf_Sigma1"]) < 1e-8
+    assert max_abs(F, oracle.hf_solve_x2c(dm, fx2c["hf_VQ_0"], sym, S_ibz, float(fx2c["madelung"]))) < 1e-12
+    assert max_abs(k.exchange(dm), oracle.hf_exchange_x2c(dm, fx2c["hf_VQ_0"], sym)) < 1e-12
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_gw_x2c_reference_fixture(ggpu, oracle, fx2c, precision):
+    """The reference's GW_X2C regression (test/cu_solver_test.cpp:317-328,
+    LU, double and single precision): Sigma_tau within 1e-6; FP64 also
+    against the oracle at 1e-10."""
+    g = np.load(os.path.join(ROOT, "tests", "golden", "fixtures", "gw_grid_beta100_lambda1e4.npz"))
+    nao, nso, NQ = int(fx2c["params_nao"]), int(fx2c["params_nso"]), int(fx2c["params_NQ"])
+    sym = oracle.symmetry_from_fixture(fx2c, NQ)
+    G, V, ref = fx2c["gw_G_tau"], fx2c["gw_VQ_0"], fx2c["gw_Sigma_tau"]
+    print(f"GW X2C fixture precision={precision}", flush=True)
+    kern = ggpu.GwKernel(nao, 1, NQ, g["T_wt"], g["T_tw"], sym, precision=precision, solver="lu", nso=nso)
+    kern.load_integrals(V)
+    Sig = kern.solve(G)
+    err = max_abs(Sig, ref)
+    print(f"max-abs vs Sigma_tau {err:.2e}", flush=True)
+    assert err < 1e-6
+    if precision == "fp64":
+        assert max_abs(Sig, oracle.gw_solve_x2c(G, V, sym, g["T_wt"], g["T_tw"], solver="lu")) < TOL
+
+
+def test_gw_x2c_synthetic_multi_k(ggpu, synth, oracle):
+    """Two-component GW on a 2x2x1 mesh (several k and q, k +- q maps)
+    with synthetic spinor G: GPU vs oracle at 1e-10."""
+    mesh, nao, NQ = (2, 2, 1), 5, 24
+    sym = ggpu.mesh_symmetry_none(mesh)
+    tau, T_wt, T_tw = synth.load_ir(50.0, 1e3)
+    G, _ = synth.make_green(sym.nk, 2 * nao, 1, 50.0, tau, 77)
+    V = synth.make_integrals(sym.nk, NQ, nao, 78)
+    print("GW X2C synthetic mesh=(2,2,1) nao=5", flush=True)
+    kern = ggpu.GwKernel(nao, 1, NQ, T_wt, T_tw, sym, solver="cholesky", nso=2 * nao)
+    kern.load_integrals(V)
+    Sig = kern.solve(G)
+    Sig_or = oracle.gw_solve_x2c(G, V, oracle.symmetry_none(mesh, 2 * nao, NQ), T_wt, T_tw, solver="cholesky")
+    assert max_abs(Sig, Sig_or) < TOL
+
+
 def test_gw_small_workload(ggpu, synth, oracle):
     """The reference's CPU-runnable workload: nk=2x2x2, nao=26, NQ=100, ns=1,
     beta=50, IR Lambda=1e3; plain FP64 GW iteration vs the oracle at 1e-10."""
@@ -113,3 +203,24 @@ def test_gw_small_workload(ggpu, synth, oracle):
     err = max_abs(Sig, Sig_or)
     print(f"max-abs Sigma error {err:.3e} (|Sigma|max {np.abs(Sig_or).max():.3e})", flush=True)
     assert err < TOL
+
+
+# ---------------------------------------------------------------------------
+# Reduced-precision contractions (FP64 Dyson / accumulation)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("precision,rtol", [("fp32", 2e-5), ("bf16x3", 2e-4)])
+def test_gw_reduced_precision(ggpu, synth, oracle, precision, rtol):
+    """FP32 contractions with FP64 P0/Sigma accumulation and FP64 Dyson: the
+    error against the FP64 oracle stays at the FP32 level relative to |Sigma|."""
+    mesh, nao, NQ, ns = (2, 2, 1), 13, 40, 2
+    sym, tau, T_wt, T_tw, G, D, V = gw_system(ggpu, synth, mesh, nao, NQ, ns, 50.0, 1e3, 31)
+    print(f"GW {precision} mesh={mesh} nao={nao} NQ={NQ}", flush=True)
+    kern = ggpu.GwKernel(nao, ns, NQ, T_wt, T_tw, sym, precision=precision, solver="cholesky")
+    kern.load_integrals(V)
+    Sig = kern.solve(G)
+    osym = oracle.symmetry_none(mesh, nao, NQ)
+    Sig_or = oracle.gw_solve(G, V, osym, T_wt, T_tw, solver="cholesky")
+    err = max_abs(Sig, Sig_or)
+    scale = np.abs(Sig_or).max()
+    print(f"{precision}: max-abs {err:.3e}, |Sigma| max {scale:.3e}, rel {err / scale:.2e}", flush=True)
+    assert err < rtol * scale
