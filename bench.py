@@ -184,9 +184,16 @@ def main():
     dtype = {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16x3"}[head]
     dom_name, (dom_ms, dom_fl, dom_n) = max(r["sites"].items(), key=lambda kv: kv[1][0])
     achieved = dom_fl / (dom_ms * 1e-3) / 1e12
+    # HBM bytes per launch of that kernel from the committed rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE passes of this workload and precision (null when not profiled)
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if os.path.exists(tf):
+        traffic = json.load(open(tf)).get(a.workload, {}).get(head, {}).get(dom_name)
     roofline = {"bound": "mfma", "kernel": f"gemm site {dom_name}", "achieved": round(achieved, 3),
                 "peak": PEAK[dtype], "unit": "TFLOP/s", "frac": round(achieved / PEAK[dtype], 4),
-                "traffic": None, "launches": dom_n, "avg_launch_ms": round(dom_ms / dom_n, 4)}
+                "traffic": None if traffic is None else round(traffic), "launches": dom_n,
+                "avg_launch_ms": round(dom_ms / dom_n, 4)}
     out = {
         "metric": METRIC, "value": round(r["tflops"], 3), "unit": "TFLOP/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(r["s"] * 1e3, 1),
