@@ -1066,7 +1066,7 @@ void HfPlan::exchange(const double* dm, double* K) {
 }
 
 // F = J + K - c madelung S D S  (hf_gpu_kernel.cpp:66-104, :141-201); the
-// two-component variant follows hf_gpu_kernel.cpp:251-320.
+// two-component variant follows hf_gpu_kernel.cpp:253-327.
 void HfPlan::solve(const double* dm, const double* S_k, double madelung, double* F) {
   require(store.loaded, "hf: integrals not loaded", kErrState);
   require(dm && S_k && F, "hf: NULL pointer");
@@ -1110,7 +1110,7 @@ void HfPlan::solve(const double* dm, const double* S_k, double madelung, double*
   }
   if (x2c) {
     // Ewald on the aa, bb, ab blocks with the AO overlap S_aa on both sides
-    // (x2c_hf_gpu_kernel::add_Ewald, hf_gpu_kernel.cpp:300-320); ba follows
+    // (x2c_hf_gpu_kernel::add_Ewald, hf_gpu_kernel.cpp:304-327); ba follows
     // as the adjoint of ab when the blocks are merged
     for (int b = 0; b < 3; ++b) {
       GemmDesc g;

@@ -127,7 +127,7 @@ int ggpu_hf_exchange(ggpu_hf_plan* plan, const double* dm, double* K_out);
  *   F = J + K - c * madelung * S D S,   c = 1 (ns = 2) or 0.5 (ns = 1),
  *   J(k) = sum_Q V^Q(k,k) (1/nk) sum_{s,k'} Tr[V^Q(k',k') D_s(k')]  (hf_gpu_kernel.cpp:141-186).
  * dm, S_k, F_out: (ns, ink, nso, nso) on the IBZ.
- * X2C (hf_gpu_kernel.cpp:251-320): J from D_aa + D_bb on both diagonal
+ * X2C (hf_gpu_kernel.cpp:253-327): J from D_aa + D_bb on both diagonal
  * blocks; Ewald madelung * S_aa D_xy S_aa on the aa, bb, ab blocks and its
  * adjoint on ba. */
 int ggpu_hf_solve(ggpu_hf_plan* plan, const double* dm, const double* S_k, double madelung, double* F_out);
@@ -136,7 +136,7 @@ void ggpu_hf_plan_destroy(ggpu_hf_plan* plan);
 
 /* ------------------------------------------------------------------------
  * GW self-energy  (replaces scalar_gw_gpu_kernel::compute_gw_selfenergy,
- * gw_gpu_kernel.cpp:294-360, and cugw_utils::accumulate_gw_selfenergy_on_device,
+ * gw_gpu_kernel.cpp:295-360, and cugw_utils::accumulate_gw_selfenergy_on_device,
  * cugw_utils.cu:103-202)
  * ------------------------------------------------------------------------ */
 /* T_wt: (nw_b, nts-2) complex  (transformer_t::Tnt_BF, cugw_qpt.cu:329-339)
@@ -145,7 +145,7 @@ void ggpu_hf_plan_destroy(ggpu_hf_plan* plan);
  * work_bytes: device workspace budget for pair batches (0 = automatic).
  * nso = nao: scalar plan (ns = 1 or 2).  nso = 2 nao: two-component plan
  * (ns = 1), whose contractions run on the pseudo-spins aa, bb, ab, ba with
- * G(k, beta - tau) spin-swapped (x2c_gw_gpu_kernel, gw_gpu_kernel.cpp:457-551,
+ * G(k, beta - tau) spin-swapped (x2c_gw_gpu_kernel, gw_gpu_kernel.cpp:458-551,
  * cugw_qkpt.cu:381-403); no time-reversal k images. */
 int ggpu_gw_plan_create(ggpu_gw_plan** plan, int nao, int nso, int ns, int NQ, int nts, int nw_b,
                         const ggpu_symmetry* sym, const double* T_wt, const double* T_tw,

@@ -411,7 +411,7 @@ def gw_solve(G, V_stored, sym, T_wt, T_tw, solver="LU", return_parts=False):
 
 
 # ----------------------------------------------------------------------------
-# Two-component (X2C) variants: gw_gpu_kernel.cpp:457-551, cugw_qkpt.cu:381-403,
+# Two-component (X2C) variants: gw_gpu_kernel.cpp:458-551, cugw_qkpt.cu:381-403,
 # hf_gpu_kernel.cpp:229-338, cuda_common.cu:176-205
 # ----------------------------------------------------------------------------
 # Pseudo-spin order of the spinor blocks (row block, column block): aa, bb, ab, ba
@@ -485,7 +485,7 @@ def hf_exchange_x2c(dm, V_stored, sym):
 
 
 def hf_solve_x2c(dm, V_stored, sym, S_k, madelung):
-    """Two-component Fock = J + K - Ewald (hf_gpu_kernel.cpp:251-338):
+    """Two-component Fock = J + K - Ewald (hf_gpu_kernel.cpp:253-327):
     J from the aa + bb density on both diagonal blocks; the Ewald term uses
     the AO overlap S_aa on both sides of every density block (aa, bb, ab;
     ba as the adjoint of ab)."""
