@@ -132,7 +132,7 @@ def main():
     modes = [m.strip() for m in a.precision.split(",")]
     if "fp64" not in modes:
         modes = ["fp64"] + modes
-    runs, sigmas = {}, {}
+    runs, sigmas, p0s = {}, {}, {}
     for mode in modes:
         kern = pkg.GwKernel(nao, ns, NQ, T_wt, T_tw, sym, precision=mode, solver=a.solver)
         kern.synth_integrals(seed, sigma)
@@ -155,6 +155,8 @@ def main():
         kern.set_profiling(False)
         runs[mode] = {"s": dt, "sites": kern.site_stats(), "phases": kern.timings()}
         sigmas[mode] = Sd
+        # P0(tau) of the first IBZ q, for the per-mode P0 error
+        p0s[mode] = kern.polarization(G, 0, 0)
         kern.close()
         del kern
     S64 = torch.view_as_complex(sigmas["fp64"].view(-1, 2))
@@ -166,6 +168,9 @@ def main():
         r["sigma_max_abs_err"] = err
         r["sigma_rel_err"] = err / scale
         r["tflops"] = fl["total"] / r["s"] / 1e12
+        p0_scale = float(np.abs(p0s["fp64"]).max())
+        r["p0_max_abs_err"] = float(np.abs(p0s[mode] - p0s["fp64"]).max())
+        r["p0_rel_err"] = r["p0_max_abs_err"] / p0_scale
         r["speedup_vs_fp64"] = runs["fp64"]["s"] / r["s"]
         log(f"[{mode}] {r['s']:.2f}s  {r['tflops']:.1f} TFLOP/s  Sigma max-abs err {err:.2e} (|Sigma| max {scale:.2e})")
     if a.calibrate_copy:
@@ -176,6 +181,23 @@ def main():
         torch.cuda.synchronize()
         log("calibration copy: 1073741824 bytes read, 1073741824 bytes written")
         del src, dst
+
+    # HF build (exchange K + direct J; FP64 in every mode) on the same system:
+    # D = -G(beta), synthetic integrals with the same seed
+    hk = pkg.HfKernel(nao, ns, NQ, sym)
+    hk.synth_integrals(seed, sigma)
+    D = -np.ascontiguousarray(G[-1])                                   # (ns, ink, nao, nao)
+    hk.solve(D)                                                        # warm-up
+    t = time.perf_counter()
+    F = hk.solve(D)
+    hf_s = time.perf_counter() - t
+    hk.close()
+    hf_fl = float(nk) * nk * ns * 16.0 * NQ * nao ** 3
+    # E_HF-like scalar, accumulated in FP64 on the host (the reference defines none)
+    e_hf = float(0.5 * np.real(np.einsum("skab,skba->", D, F)) / nk)
+    hf = {"seconds": round(hf_s, 4), "exchange_tflops": round(hf_fl / hf_s / 1e12, 2), "flop": hf_fl,
+          "e_hf_like": e_hf, "precision": "fp64", "includes": "K + J, host-to-host"}
+    log(f"[hf] {hf_s:.3f}s  {hf_fl / hf_s / 1e12:.1f} TFLOP/s (exchange FLOPs)")
 
     # headline: the fastest mode whose Sigma stays within 1e-6 of FP64
     ok = [m for m in modes if runs[m]["sigma_max_abs_err"] <= 1e-6]
@@ -207,12 +229,15 @@ def main():
         "modes": {m: {"gw_iteration_s": round(runs[m]["s"], 3), "tflops": round(runs[m]["tflops"], 2),
                       "speedup_vs_fp64": round(runs[m]["speedup_vs_fp64"], 3),
                       "sigma_max_abs_err": runs[m]["sigma_max_abs_err"],
-                      "sigma_rel_err": runs[m]["sigma_rel_err"]} for m in modes},
+                      "sigma_rel_err": runs[m]["sigma_rel_err"],
+                      "p0_max_abs_err": runs[m]["p0_max_abs_err"],
+                      "p0_rel_err": runs[m]["p0_rel_err"]} for m in modes},
         "sigma_abs_max": scale,
         "phases_ms": {k: round(v, 1) for k, v in r["phases"].items()},
         "sites": {k: {"ms": round(v[0], 1), "tflops": round(v[1] / (v[0] * 1e-3) / 1e12, 2), "launches": v[2]}
                   for k, v in r["sites"].items()},
         "roofline": roofline,
+        "hf": hf,
     }
     if not a.no_cpu:
         pairs = {}

@@ -239,6 +239,7 @@ __global__ __launch_bounds__(NTHREADS) void gemm_mfma_kernel(const GemmDesc d) {
 // contiguous per row, so every MFMA fragment is one 16-byte LDS read.
 // ---------------------------------------------------------------------------
 constexpr int XBK = 32;          // k per MFMA step
+int g_bf16_tiles = -1;           // split-bf16 tile choice: -1 automatic, else a fixed kBf16Tile* value
 constexpr int XPAD = 8;          // bf16 elements of row padding
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef unsigned int v4u __attribute__((ext_vector_type(4)));
@@ -387,8 +388,230 @@ __global__ __launch_bounds__(NTHREADS) void gemm_bf16x3_kernel(const GemmDesc d)
       }
 }
 
+// ---------------------------------------------------------------------------
+// Large-tile split-bf16 x3 kernel.  Workgroup tile (32 WMB) x (32 WNB) of C,
+// 4 waves in 2 x 2, each wave WMB x WNB blocks of 16 x 16.  The bigger tile
+// halves the operand traffic per MFMA against the 64 x 64 kernel above, and
+// the loader is organised in groups of 8 consecutive k of one row:
+//   k contiguous in memory:  group g -> row g / 4, k0 = 8 (g % 4)
+//                            (4 lanes stream one 256-byte row segment)
+//   row contiguous:          group g -> row g % rows, k0 = 8 (g / rows)
+//                            (lanes stream consecutive rows of one k)
+// so every global load instruction is coalesced in both orientations and
+// each group is split into the four bf16 planes with one 16-byte LDS store
+// per plane.
+// ---------------------------------------------------------------------------
+typedef float v2f __attribute__((ext_vector_type(2)));
+typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
+// (a, b) -> packed bf16 hi = rne(a), rne(b) and lo = rne(a - hi_a), rne(b - hi_b),
+// element a in the low half (v_cvt_pk_bf16_f32 does the rounding)
+__device__ inline void split2(float a, float b, unsigned& hi, unsigned& lo) {
+  const unsigned h = __builtin_bit_cast(unsigned, __builtin_convertvector(v2f{a, b}, v2bf));
+  const float ha = __uint_as_float(h << 16), hb = __uint_as_float(h & 0xffff0000u);
+  lo = __builtin_bit_cast(unsigned, __builtin_convertvector(v2f{a - ha, b - hb}, v2bf));
+  hi = h;
+}
+
+template <class TO, bool AKC, bool BKC, int WMB, int WNB, int SITE>
+__global__ __launch_bounds__(NTHREADS) void gemm_bf16x3_big_kernel(const GemmDesc d) {
+  constexpr int TM = 32 * WMB, TN = 32 * WNB, LDK = XBK + XPAD;
+  constexpr int GA = (TM * 4 + NTHREADS - 1) / NTHREADS;   // 8-element groups per thread
+  constexpr int GB = (TN * 4 + NTHREADS - 1) / NTHREADS;
+  __shared__ __attribute__((aligned(16))) unsigned short sA[4][TM][LDK];
+  __shared__ __attribute__((aligned(16))) unsigned short sB[4][TN][LDK];
+
+  const int tid = threadIdx.x;
+  const int tiles_m = (d.M + TM - 1) / TM;
+  const int tiles_n = (d.N + TN - 1) / TN;
+  int64_t lg = xcd_logical(blockIdx.x, gridDim.x);
+  const int m0 = (int)(lg % tiles_m) * TM;
+  lg /= tiles_m;
+  const int n0 = (int)(lg % tiles_n) * TN;
+  lg /= tiles_n;
+  const int64_t b1 = lg % d.nb1, b0 = lg / d.nb1;
+  const float2* A = static_cast<const float2*>(d.A.ptr) + b0 * d.A.bs0 + (d.A.off1 ? d.A.off1[b1] : b1 * d.A.bs1);
+  const float2* B = static_cast<const float2*>(d.B.ptr) + b0 * d.B.bs0 + (d.B.off1 ? d.B.off1[b1] : b1 * d.B.bs1);
+  TO* C = static_cast<TO*>(d.C.ptr) + b0 * d.C.bs0 + (d.C.off1 ? d.C.off1[b1] : b1 * d.C.bs1);
+
+  int arow[GA], ak0[GA], brow[GB], bk0[GB];
+#pragma unroll
+  for (int u = 0; u < GA; ++u) {
+    const int g = tid + u * NTHREADS;
+    arow[u] = AKC ? (g >> 2) : (g % TM);
+    ak0[u] = AKC ? ((g & 3) << 3) : ((g / TM) << 3);
+    if (g >= TM * 4) arow[u] = -1;
+  }
+#pragma unroll
+  for (int u = 0; u < GB; ++u) {
+    const int g = tid + u * NTHREADS;
+    brow[u] = BKC ? (g >> 2) : (g % TN);
+    bk0[u] = BKC ? ((g & 3) << 3) : ((g / TN) << 3);
+    if (g >= TN * 4) brow[u] = -1;
+  }
+  const int ktiles = (d.K + XBK - 1) / XBK;
+  const int niter = d.KB * ktiles;
+  float2 ra[GA][8], rb[GB][8];
+  auto gload = [&](int it) {
+    const int kb = it / ktiles;
+    const int kt = (it - kb * ktiles) * XBK;
+    const float2* Ab = A + kb * d.A.kbs;
+    const float2* Bb = B + kb * d.B.kbs;
+#pragma unroll
+    for (int u = 0; u < GA; ++u) {
+      const int gm = m0 + arow[u];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int gk = kt + ak0[u] + j;
+        ra[u][j] = (arow[u] >= 0 && gm < d.M && gk < d.K) ? Ab[gm * d.A.rs + gk * d.A.cs] : float2{0.f, 0.f};
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < GB; ++u) {
+      const int gn = n0 + brow[u];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int gk = kt + bk0[u] + j;
+        rb[u][j] = (brow[u] >= 0 && gn < d.N && gk < d.K) ? Bb[gk * d.B.rs + gn * d.B.cs] : float2{0.f, 0.f};
+      }
+    }
+  };
+  // split 8 consecutive-k elements into the re_hi, im_hi, re_lo, im_lo planes
+  auto split8 = [&](unsigned short (*S)[LDK], int rows, int row, int k0, const float2* x, bool conj) {
+    unsigned hr[4], hi[4], lr_[4], li[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float s = conj ? -1.f : 1.f;
+      split2(x[2 * q].x, x[2 * q + 1].x, hr[q], lr_[q]);
+      split2(s * x[2 * q].y, s * x[2 * q + 1].y, hi[q], li[q]);
+    }
+    *reinterpret_cast<v4u*>(&S[0 * rows + row][k0]) = v4u{hr[0], hr[1], hr[2], hr[3]};
+    *reinterpret_cast<v4u*>(&S[1 * rows + row][k0]) = v4u{hi[0], hi[1], hi[2], hi[3]};
+    *reinterpret_cast<v4u*>(&S[2 * rows + row][k0]) = v4u{lr_[0], lr_[1], lr_[2], lr_[3]};
+    *reinterpret_cast<v4u*>(&S[3 * rows + row][k0]) = v4u{li[0], li[1], li[2], li[3]};
+  };
+  auto sstore = [&]() {
+#pragma unroll
+    for (int u = 0; u < GA; ++u)
+      if (arow[u] >= 0) split8(&sA[0][0], TM, arow[u], ak0[u], ra[u], d.A.conj != 0);
+#pragma unroll
+    for (int u = 0; u < GB; ++u)
+      if (brow[u] >= 0) split8(&sB[0][0], TN, brow[u], bk0[u], rb[u], d.B.conj != 0);
+  };
+
+  const int lane = tid & 63, w = tid >> 6;
+  const int wm = (w >> 1) * (16 * WMB), wn = (w & 1) * (16 * WNB);
+  const int lr = lane & 15, lk = lane >> 4;
+  v4f cr[WMB][WNB], ci[WMB][WNB];
+#pragma unroll
+  for (int i = 0; i < WMB; ++i)
+#pragma unroll
+    for (int j = 0; j < WNB; ++j) { cr[i][j] = v4f{0, 0, 0, 0}; ci[i][j] = v4f{0, 0, 0, 0}; }
+  const v4u sign = v4u{0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
+
+  if (niter > 0) gload(0);
+  for (int it = 0; it < niter; ++it) {
+    sstore();
+    __syncthreads();
+    if (it + 1 < niter) gload(it + 1);
+    v8bf brh[WNB], bih[WNB], brl[WNB], bil[WNB];
+#pragma unroll
+    for (int j = 0; j < WNB; ++j) {
+      const int rb_ = wn + 16 * j + lr;
+      brh[j] = __builtin_bit_cast(v8bf, *reinterpret_cast<const v4u*>(&sB[0][rb_][8 * lk]));
+      bih[j] = __builtin_bit_cast(v8bf, *reinterpret_cast<const v4u*>(&sB[1][rb_][8 * lk]));
+      brl[j] = __builtin_bit_cast(v8bf, *reinterpret_cast<const v4u*>(&sB[2][rb_][8 * lk]));
+      bil[j] = __builtin_bit_cast(v8bf, *reinterpret_cast<const v4u*>(&sB[3][rb_][8 * lk]));
+    }
+#pragma unroll
+    for (int i = 0; i < WMB; ++i) {
+      const int ra_ = wm + 16 * i + lr;
+      const v4u uih = *reinterpret_cast<const v4u*>(&sA[1][ra_][8 * lk]);
+      const v4u uil = *reinterpret_cast<const v4u*>(&sA[3][ra_][8 * lk]);
+      const v8bf arh = __builtin_bit_cast(v8bf, *reinterpret_cast<const v4u*>(&sA[0][ra_][8 * lk]));
+      const v8bf arl = __builtin_bit_cast(v8bf, *reinterpret_cast<const v4u*>(&sA[2][ra_][8 * lk]));
+      const v8bf aih = __builtin_bit_cast(v8bf, uih), ail = __builtin_bit_cast(v8bf, uil);
+      const v8bf naih = __builtin_bit_cast(v8bf, uih ^ sign), nail = __builtin_bit_cast(v8bf, uil ^ sign);
+      // product-major order: consecutive updates of one accumulator are
+      // 2 WNB - 1 MFMAs apart
+#define GGPU_X3_STEP(acc, a, b)                                                       \
+  _Pragma("unroll") for (int j = 0; j < WNB; ++j) acc[i][j] =                         \
+      __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j], acc[i][j], 0, 0, 0);
+      GGPU_X3_STEP(cr, arh, brh) GGPU_X3_STEP(ci, arh, bih)
+      GGPU_X3_STEP(cr, arh, brl) GGPU_X3_STEP(ci, arh, bil)
+      GGPU_X3_STEP(cr, arl, brh) GGPU_X3_STEP(ci, arl, bih)
+      GGPU_X3_STEP(cr, naih, bih) GGPU_X3_STEP(ci, aih, brh)
+      GGPU_X3_STEP(cr, naih, bil) GGPU_X3_STEP(ci, aih, brl)
+      GGPU_X3_STEP(cr, nail, bih) GGPU_X3_STEP(ci, ail, brh)
+#undef GGPU_X3_STEP
+    }
+    __syncthreads();
+  }
+
+  const bool use_beta = (d.beta_re != 0.0 || d.beta_im != 0.0);
+#pragma unroll
+  for (int i = 0; i < WMB; ++i)
+#pragma unroll
+    for (int j = 0; j < WNB; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm + 16 * i + 4 * lk + r;
+        const int col = n0 + wn + 16 * j + lr;
+        if (row < d.M && col < d.N) {
+          const double xr = cr[i][j][r], xi = ci[i][j][r];
+          double vr = d.alpha_re * xr - d.alpha_im * xi;
+          double vi = d.alpha_re * xi + d.alpha_im * xr;
+          TO* p = C + row * d.C.rs + col * d.C.cs;
+          if (use_beta) {
+            const TO o = *p;
+            vr += d.beta_re * o.x - d.beta_im * o.y;
+            vi += d.beta_re * o.y + d.beta_im * o.x;
+          }
+          using TS = decltype(TO{}.x);
+          *p = TO{static_cast<TS>(vr), static_cast<TS>(vi)};
+        }
+      }
+}
+
+template <class TO, int WMB, int WNB, int SITE>
+void launch_bf16x3_big(const GemmDesc& d, hipStream_t stream) {
+  constexpr int TM = 32 * WMB, TN = 32 * WNB;
+  const int64_t tiles = (int64_t)((d.M + TM - 1) / TM) * ((d.N + TN - 1) / TN);
+  const int64_t total = tiles * d.nb0 * d.nb1;
+  require(total < (int64_t)1 << 31, "gemm: grid too large");
+  dim3 grid((unsigned)total), block(NTHREADS);
+  const bool akc = (d.A.cs == 1) || (d.A.rs != 1);
+  const bool bkc = (d.B.rs == 1) || (d.B.cs != 1);
+  if (akc && bkc) hipLaunchKernelGGL((gemm_bf16x3_big_kernel<TO, true, true, WMB, WNB, SITE>), grid, block, 0, stream, d);
+  else if (akc) hipLaunchKernelGGL((gemm_bf16x3_big_kernel<TO, true, false, WMB, WNB, SITE>), grid, block, 0, stream, d);
+  else if (bkc) hipLaunchKernelGGL((gemm_bf16x3_big_kernel<TO, false, true, WMB, WNB, SITE>), grid, block, 0, stream, d);
+  else hipLaunchKernelGGL((gemm_bf16x3_big_kernel<TO, false, false, WMB, WNB, SITE>), grid, block, 0, stream, d);
+  GGPU_HIP(hipGetLastError());
+}
+
+// Fraction of a tiled extent that is real work.
+inline double tile_use(int n, int t) { return (double)n / (double)(((n + t - 1) / t) * t); }
+
 template <class TO, int SITE>
 void launch_bf16x3_site(const GemmDesc& d, hipStream_t stream) {
+  int mode = g_bf16_tiles;
+  if (mode < 0) {
+    // two waves per SIMD (the 8- and 10-block wave tiles) beat the bigger
+    // 16-block tiles; among those, the one that wastes least of M x N
+    mode = kBf16Tile64x64;
+    if (d.M >= 64 && d.N >= 48 && (d.M >= 128 || d.N >= 128)) {
+      const double u1 = tile_use(d.M, 128) * tile_use(d.N, 64), u4 = tile_use(d.M, 64) * tile_use(d.N, 128),
+                   u5 = tile_use(d.M, 160) * tile_use(d.N, 64);
+      mode = kBf16Tile128x64;
+      if (u4 > u1 + 0.03 && u4 >= u5) mode = kBf16Tile64x128;
+      else if (u5 > u1 + 0.03) mode = kBf16Tile160x64;
+    }
+  }
+  switch (mode) {
+    case kBf16Tile128x64: return launch_bf16x3_big<TO, 4, 2, SITE>(d, stream);
+    case kBf16Tile64x128: return launch_bf16x3_big<TO, 2, 4, SITE>(d, stream);
+    case kBf16Tile160x64: return launch_bf16x3_big<TO, 5, 2, SITE>(d, stream);
+    default: break;
+  }
   const int64_t tiles = (int64_t)((d.M + BM - 1) / BM) * ((d.N + BN - 1) / BN);
   const int64_t total = tiles * d.nb0 * d.nb1;
   require(total < (int64_t)1 << 31, "gemm: grid too large");
@@ -503,6 +726,7 @@ void gemm_check_bounds(const GemmDesc& d) {
 }
 
 void set_gemm_profiler(GemmProfiler* p) { g_profiler = p; }
+void set_bf16_tiles(int mode) { g_bf16_tiles = mode; }
 
 void gemm(const GemmDesc& d, hipStream_t stream) {
   gemm_check_bounds(d);

@@ -72,6 +72,10 @@ struct GemmProfiler {
   virtual void after(const GemmDesc& d, hipStream_t s) = 0;
 };
 void set_gemm_profiler(GemmProfiler* p);
+// Workgroup tile of the split-bf16 x3 GEMMs: chosen per shape (-1, the
+// default) or fixed to one of these for every launch (all give the same sums).
+enum { kBf16Tile64x64 = 0, kBf16Tile128x64 = 1, kBf16Tile64x128 = 4, kBf16Tile160x64 = 5 };
+void set_bf16_tiles(int mode);
 
 struct GemmDesc {
   int site = kSiteOther;
