@@ -101,6 +101,8 @@ def main():
     ap.add_argument("--precision", default="fp64,fp32,bf16x3",
                     help="comma-separated modes; fp64 always runs (the error reference)")
     ap.add_argument("--solver", default="cholesky", choices=["cholesky", "lu"])
+    ap.add_argument("--integrals", default="host", choices=["host", "device"],
+                    help="host: pinned host memory, streamed per pair batch (overlapped); device: resident in HBM")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--cpu-pairs", type=int, default=2)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -134,7 +136,7 @@ def main():
         modes = ["fp64"] + modes
     runs, sigmas, p0s = {}, {}, {}
     for mode in modes:
-        kern = pkg.GwKernel(nao, ns, NQ, T_wt, T_tw, sym, precision=mode, solver=a.solver)
+        kern = pkg.GwKernel(nao, ns, NQ, T_wt, T_tw, sym, precision=mode, solver=a.solver, integrals=a.integrals)
         kern.synth_integrals(seed, sigma)
         Sd = torch.empty_like(Gd)
         torch.cuda.synchronize()
@@ -153,7 +155,8 @@ def main():
             log(f"[{mode}] step {i} done")
         dt = (time.perf_counter() - t0) / a.steps
         kern.set_profiling(False)
-        runs[mode] = {"s": dt, "sites": kern.site_stats(), "phases": kern.timings()}
+        runs[mode] = {"s": dt, "sites": kern.site_stats(), "phases": kern.timings(),
+                      "streamed": kern.streamed_bytes()}
         sigmas[mode] = Sd
         # P0(tau) of the first IBZ q, for the per-mode P0 error
         p0s[mode] = kern.polarization(G, 0, 0)
@@ -223,7 +226,7 @@ def main():
         "data": "synthetic (seeded random H(k) -> G(tau); Gaussian V, sigma 0.1)",
         "config": {"workload": a.workload, "nk": nk, "nao": nao, "NQ": NQ, "ns": ns, "nts": nts,
                    "nw_b": nw_b, "beta": w["beta"], "ir_lambda": w["lam"], "precision": head,
-                   "solver": a.solver},
+                   "solver": a.solver, "integrals": a.integrals},
         "gw_iteration_s": round(r["s"], 3),
         "gw_flop_per_iteration": fl["total"],
         "modes": {m: {"gw_iteration_s": round(runs[m]["s"], 3), "tflops": round(runs[m]["tflops"], 2),
@@ -233,6 +236,9 @@ def main():
                       "p0_max_abs_err": runs[m]["p0_max_abs_err"],
                       "p0_rel_err": runs[m]["p0_rel_err"]} for m in modes},
         "sigma_abs_max": scale,
+        "integrals": {"residence": a.integrals,
+                      "h2d_gb_per_iteration": round(r["streamed"] / 1e9, 2),
+                      "h2d_gb_per_s_average": round(r["streamed"] / 1e9 / r["s"], 2)},
         "phases_ms": {k: round(v, 1) for k, v in r["phases"].items()},
         "sites": {k: {"ms": round(v[0], 1), "tflops": round(v[1] / (v[0] * 1e-3) / 1e12, 2), "launches": v[2]}
                   for k, v in r["sites"].items()},

@@ -24,6 +24,7 @@ LIB_PATH = os.path.join(HERE, "libgreen_gpu_amd.so")
 
 PRECISION = {"fp64": 0, "fp32": 1, "bf16x3": 2}
 SOLVER = {"cholesky": 0, "lu": 1}
+RESIDENCE = {"device": 0, "host": 1}
 
 
 class GgpuError(RuntimeError):
@@ -48,6 +49,7 @@ def lib():
         L.ggpu_device_count.argtypes = [C.POINTER(C.c_int)]
         L.ggpu_device_memory.argtypes = [C.POINTER(sz), C.POINTER(sz)]
         L.ggpu_hf_plan_create.argtypes = [C.POINTER(vp), i32, i32, i32, i32, C.POINTER(SymmetryC)]
+        L.ggpu_hf_set_integral_residence.argtypes = [vp, i32]
         L.ggpu_hf_load_integrals.argtypes = [vp, pd]
         L.ggpu_hf_synth_integrals.argtypes = [vp, u64, f64]
         L.ggpu_hf_exchange.argtypes = [vp, pd, pd]
@@ -56,6 +58,8 @@ def lib():
         L.ggpu_hf_plan_destroy.restype = None
         L.ggpu_gw_plan_create.argtypes = [C.POINTER(vp), i32, i32, i32, i32, i32, i32, C.POINTER(SymmetryC),
                                           pd, pd, i32, i32, sz]
+        L.ggpu_gw_set_integral_residence.argtypes = [vp, i32]
+        L.ggpu_gw_streamed_bytes.argtypes = [vp, pd]
         L.ggpu_gw_load_integrals.argtypes = [vp, pd]
         L.ggpu_gw_synth_integrals.argtypes = [vp, u64, f64]
         L.ggpu_gw_iterate.argtypes = [vp, pd, pd]
@@ -71,9 +75,11 @@ def lib():
 
 
 EXPORTED = ["ggpu_last_error", "ggpu_device_count", "ggpu_device_memory",
-            "ggpu_hf_plan_create", "ggpu_hf_load_integrals", "ggpu_hf_synth_integrals",
+            "ggpu_hf_plan_create", "ggpu_hf_set_integral_residence", "ggpu_hf_load_integrals",
+            "ggpu_hf_synth_integrals",
             "ggpu_hf_exchange", "ggpu_hf_solve", "ggpu_hf_plan_destroy",
-            "ggpu_gw_plan_create", "ggpu_gw_load_integrals", "ggpu_gw_synth_integrals",
+            "ggpu_gw_plan_create", "ggpu_gw_set_integral_residence", "ggpu_gw_streamed_bytes",
+            "ggpu_gw_load_integrals", "ggpu_gw_synth_integrals",
             "ggpu_gw_iterate", "ggpu_gw_iterate_device", "ggpu_gw_polarization",
             "ggpu_gw_last_timings", "ggpu_gw_set_profiling", "ggpu_gw_site_stats",
             "ggpu_gw_plan_destroy"]
@@ -230,7 +236,7 @@ class HfKernel:
     """HF kernel on the GPU (reference: scalar_hf_gpu_kernel, hf_gpu_kernel.h:67-84;
     x2c_hf_gpu_kernel, hf_gpu_kernel.h:87-100, when nso = 2 nao)."""
 
-    def __init__(self, nao, ns, NQ, sym, madelung=0.0, S_k=None, nso=None):
+    def __init__(self, nao, ns, NQ, sym, madelung=0.0, S_k=None, nso=None, integrals="device"):
         self.nao, self.ns, self.NQ = int(nao), int(ns), int(NQ)
         self.nso = self.nao if nso is None else int(nso)
         self.sym = sym if isinstance(sym, Symmetry) else Symmetry.like(sym)
@@ -240,6 +246,7 @@ class HfKernel:
         self._csym = self.sym.to_c()
         _check(lib().ggpu_hf_plan_create(C.byref(h), self.nao, self.nso, self.ns, self.NQ, C.byref(self._csym)))
         self._h = h
+        _check(lib().ggpu_hf_set_integral_residence(self._h, RESIDENCE[integrals]))
 
     @property
     def mat_shape(self):
@@ -279,7 +286,8 @@ class GwKernel:
     """GW kernel on the GPU (reference: scalar_gw_gpu_kernel, gw_gpu_kernel.h:139-172;
     x2c_gw_gpu_kernel, gw_gpu_kernel.h:169-206, when nso = 2 nao)."""
 
-    def __init__(self, nao, ns, NQ, T_wt, T_tw, sym, precision="fp64", solver="lu", work_bytes=0, nso=None):
+    def __init__(self, nao, ns, NQ, T_wt, T_tw, sym, precision="fp64", solver="lu", work_bytes=0, nso=None,
+                 integrals="device"):
         T_wt = _c128(T_wt)
         T_tw = _c128(T_tw)
         self.nao, self.ns, self.NQ = int(nao), int(ns), int(NQ)
@@ -292,6 +300,7 @@ class GwKernel:
                                          C.byref(self._csym), _dptr(T_wt), _dptr(T_tw),
                                          PRECISION[precision], SOLVER[solver.lower()], int(work_bytes)))
         self._h = h
+        _check(lib().ggpu_gw_set_integral_residence(self._h, RESIDENCE[integrals]))
 
     @property
     def shape(self):
@@ -323,6 +332,12 @@ class GwKernel:
         out = np.empty((n, self.NQ, self.NQ), dtype=np.complex128)
         _check(lib().ggpu_gw_polarization(self._h, _dptr(G), int(iq), int(which), _dptr(out)))
         return out
+
+    def streamed_bytes(self):
+        """Integral bytes copied host -> device by the last iteration."""
+        b = C.c_double()
+        _check(lib().ggpu_gw_streamed_bytes(self._h, C.byref(b)))
+        return b.value
 
     def timings(self):
         t = np.zeros(6)

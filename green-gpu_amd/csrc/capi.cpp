@@ -66,6 +66,15 @@ int ggpu_hf_plan_create(ggpu_hf_plan** plan, int nao, int nso, int ns, int NQ, c
   });
 }
 
+int ggpu_hf_set_integral_residence(ggpu_hf_plan* plan, int residence) {
+  return guarded([&] {
+    ggpu::require(plan && plan->impl, "NULL plan");
+    ggpu::require(residence == GGPU_INTEGRALS_DEVICE || residence == GGPU_INTEGRALS_HOST, "unknown integral residence");
+    auto* p = plan->impl;
+    p->store.set_host_resident(residence == GGPU_INTEGRALS_HOST, p->kb, (int64_t)p->NQ * p->nao * p->nao);
+  });
+}
+
 int ggpu_hf_load_integrals(ggpu_hf_plan* plan, const double* V_stored) {
   return guarded([&] {
     ggpu::require(plan && plan->impl, "NULL plan");
@@ -115,6 +124,22 @@ int ggpu_gw_plan_create(ggpu_gw_plan** plan, int nao, int nso, int ns, int NQ, i
       throw;
     }
     *plan = p;
+  });
+}
+
+int ggpu_gw_set_integral_residence(ggpu_gw_plan* plan, int residence) {
+  return guarded([&] {
+    ggpu::require(plan && plan->impl, "NULL plan");
+    ggpu::require(residence == GGPU_INTEGRALS_DEVICE || residence == GGPU_INTEGRALS_HOST, "unknown integral residence");
+    auto* p = plan->impl;
+    p->store.set_host_resident(residence == GGPU_INTEGRALS_HOST, p->kb, (int64_t)p->NQ * p->nao * p->nao);
+  });
+}
+
+int ggpu_gw_streamed_bytes(ggpu_gw_plan* plan, double* bytes) {
+  return guarded([&] {
+    ggpu::require(plan && plan->impl && bytes, "NULL argument");
+    *bytes = plan->impl->streamed_last;
   });
 }
 

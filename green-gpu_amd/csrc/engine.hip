@@ -138,12 +138,56 @@ PairView SymTables::view(int64_t k1, int64_t k2) const {
 // ---------------------------------------------------------------------------
 // Integral store and pair unfolding
 // ---------------------------------------------------------------------------
+IntegralStore::~IntegralStore() {
+  if (copy_) (void)hipStreamSynchronize(copy_);
+  for (int b = 0; b < 2; ++b) {
+    if (ev_copy_[b]) (void)hipEventDestroy(ev_copy_[b]);
+    if (ev_used_[b]) (void)hipEventDestroy(ev_used_[b]);
+  }
+  if (copy_) (void)hipStreamDestroy(copy_);
+  if (host_) (void)hipHostFree(host_);
+}
+
+void IntegralStore::set_host_resident(bool on, int slots, int64_t pair_elems) {
+  require(!loaded_, "integrals: choose the residence before loading integrals", kErrState);
+  host_mode_ = on;
+  if (!on) return;
+  require(slots >= 1 && pair_elems > 0, "integrals: bad staging size");
+  slots_ = slots;
+  pair_elems_ = pair_elems;
+  if (!copy_) {
+    GGPU_HIP(hipStreamCreateWithFlags(&copy_, hipStreamNonBlocking));
+    for (int b = 0; b < 2; ++b) {
+      GGPU_HIP(hipEventCreateWithFlags(&ev_copy_[b], hipEventDisableTiming));
+      GGPU_HIP(hipEventCreateWithFlags(&ev_used_[b], hipEventDisableTiming));
+    }
+  }
+  stage_.resize((size_t)2 * slots * pair_elems);
+  std::vector<int64_t> off((size_t)2 * slots);
+  for (int i = 0; i < 2 * slots; ++i) off[i] = i * pair_elems;
+  stage_off_.resize(off.size());
+  GGPU_HIP(hipMemcpy(stage_off_.get(), off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+}
+
+void IntegralStore::alloc_host(int64_t n_elems) {
+  if (host_ && n_elems_ == n_elems) return;
+  if (host_) GGPU_HIP(hipHostFree(host_));
+  host_ = nullptr;
+  GGPU_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_), (size_t)n_elems * sizeof(z64), hipHostMallocDefault));
+}
+
 void IntegralStore::load_host(const double* V_stored, int64_t n_elems, hipStream_t s) {
   require(V_stored != nullptr, "integrals: NULL pointer");
-  V.resize(n_elems);
-  GGPU_HIP(hipMemcpyAsync(V.get(), V_stored, n_elems * sizeof(z64), hipMemcpyHostToDevice, s));
-  GGPU_HIP(hipStreamSynchronize(s));
-  loaded = true;
+  if (host_mode_) {
+    alloc_host(n_elems);
+    std::memcpy(host_, V_stored, (size_t)n_elems * sizeof(z64));
+  } else {
+    V_.resize(n_elems);
+    GGPU_HIP(hipMemcpyAsync(V_.get(), V_stored, n_elems * sizeof(z64), hipMemcpyHostToDevice, s));
+    GGPU_HIP(hipStreamSynchronize(s));
+  }
+  n_elems_ = n_elems;
+  loaded_ = true;
 }
 
 void IntegralStore::synth(const SymTables& t, int NQ, int nao, uint64_t seed, double sigma, hipStream_t s) {
@@ -155,10 +199,77 @@ void IntegralStore::synth(const SymTables& t, int NQ, int nao, uint64_t seed, do
   }
   DeviceBuffer<int> d(diag.size());
   GGPU_HIP(hipMemcpy(d.get(), diag.data(), diag.size() * sizeof(int), hipMemcpyHostToDevice));
-  V.resize((size_t)t.num_stored * NQ * nao * nao);
-  synth_integrals(V.get(), 0, t.num_stored, d.get(), NQ, nao, seed, sigma, s);
-  GGPU_HIP(hipStreamSynchronize(s));
-  loaded = true;
+  const int64_t per = (int64_t)NQ * nao * nao, n = t.num_stored * per;
+  if (host_mode_) {
+    // generate on the device in chunks of pairs, copy each to pinned memory
+    alloc_host(n);
+    const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(t.num_stored, ((int64_t)1 << 30) / (per * 16)));
+    DeviceBuffer<z64> tmp((size_t)chunk * per);
+    for (int64_t p0 = 0; p0 < t.num_stored; p0 += chunk) {
+      const int64_t np = std::min(chunk, t.num_stored - p0);
+      synth_integrals(tmp.get(), p0, np, d.get() + p0, NQ, nao, seed, sigma, s);
+      GGPU_HIP(hipMemcpyAsync(host_ + p0 * per, tmp.get(), (size_t)np * per * sizeof(z64), hipMemcpyDeviceToHost, s));
+      GGPU_HIP(hipStreamSynchronize(s));
+    }
+  } else {
+    V_.resize((size_t)n);
+    synth_integrals(V_.get(), 0, t.num_stored, d.get(), NQ, nao, seed, sigma, s);
+    GGPU_HIP(hipStreamSynchronize(s));
+  }
+  n_elems_ = n;
+  loaded_ = true;
+}
+
+// Copy pairs [i0, i0 + n) of tab into staging buffer buf, after the last
+// reader of that buffer is done.
+void IntegralStore::issue(const PairTable& tab, int i0, int n, int buf) {
+  require(n <= slots_, "integrals: batch larger than the staging area");
+  GGPU_HIP(hipStreamWaitEvent(copy_, ev_used_[buf], 0));
+  const size_t bytes = (size_t)pair_elems_ * sizeof(z64);
+  for (int i = 0; i < n; ++i) {
+    const int64_t pos = tab.host[i0 + i].pos;
+    require(pos >= 0 && (pos + 1) * pair_elems_ <= n_elems_, "integrals: pair outside store");
+    GGPU_HIP(hipMemcpyAsync(stage_.get() + ((int64_t)buf * slots_ + i) * pair_elems_, host_ + pos * pair_elems_,
+                            bytes, hipMemcpyHostToDevice, copy_));
+  }
+  GGPU_HIP(hipEventRecord(ev_copy_[buf], copy_));
+  streamed_ += (double)bytes * n;
+}
+
+const z64* IntegralStore::view(const PairTable& tab, int i0, int n, hipStream_t s, const int64_t** offsets) {
+  require(i0 >= 0 && i0 + n <= (int)tab.host.size(), "integrals: pair range");
+  if (!host_) {
+    *offsets = tab.in_off.get() + i0;
+    return V_.get();
+  }
+  require(cur_ < 0, "integrals: previous staged batch still in use");
+  int buf;
+  if (pre_.tab == &tab && pre_.i0 == i0 && pre_.n == n) {
+    buf = pre_.buf;
+  } else {
+    buf = next_;
+    issue(tab, i0, n, buf);
+  }
+  GGPU_HIP(hipStreamWaitEvent(s, ev_copy_[buf], 0));
+  cur_ = buf;
+  // prefetch the batch that usually follows into the other buffer
+  const int j0 = i0 + n, total = (int)tab.host.size();
+  if (j0 < total) {
+    const int m = std::min(n, total - j0);
+    issue(tab, j0, m, buf ^ 1);
+    pre_ = Pending{&tab, j0, m, buf ^ 1};
+  } else {
+    pre_ = Pending{};
+  }
+  next_ = buf ^ 1;
+  *offsets = stage_off_.get() + (int64_t)buf * slots_;
+  return stage_.get();
+}
+
+void IntegralStore::done(hipStream_t s) {
+  if (!host_ || cur_ < 0) return;
+  GGPU_HIP(hipEventRecord(ev_used_[cur_], s));
+  cur_ = -1;
 }
 
 void PairTable::build(const std::vector<PairView>& views, int64_t stride, hipStream_t s) {
@@ -179,13 +290,15 @@ void PairTable::build(const std::vector<PairView>& views, int64_t stride, hipStr
   GGPU_HIP(hipMemcpy(conj.get(), cj.data(), cj.size() * sizeof(int), hipMemcpyHostToDevice));
 }
 
-void unfold_pairs(const IntegralStore& st, const PairTable& tab, int i0, int n, const int64_t* out_off_dev,
+void unfold_pairs(IntegralStore& st, const PairTable& tab, int i0, int n, const int64_t* out_off_dev,
                   void* dst, int out_f32, int NQ, int nao, hipStream_t s) {
   require(i0 >= 0 && i0 + n <= (int)tab.host.size(), "unfold_pairs: range");
   for (int i = i0; i < i0 + n; ++i)
-    require((tab.host[i].pos + 1) * (int64_t)NQ * nao * nao <= (int64_t)st.V.size(), "unfold_pairs: pair outside store");
+    require((tab.host[i].pos + 1) * (int64_t)NQ * nao * nao <= st.elems(), "unfold_pairs: pair outside store");
+  const int64_t* offs = nullptr;
+  const z64* src = st.view(tab, i0, n, s, &offs);
   BlockCopy c;
-  c.in = st.V.get();
+  c.in = src;
   c.out = dst;
   c.out_f32 = out_f32;
   c.rows = nao;
@@ -196,11 +309,12 @@ void unfold_pairs(const IntegralStore& st, const PairTable& tab, int i0, int n, 
   c.in_bs = (int64_t)nao * nao;
   c.out_bs = (int64_t)nao * nao;
   c.n_outer = n;
-  c.in_off = tab.in_off.get() + i0;
+  c.in_off = offs;
   c.out_off = out_off_dev;
   c.trans = tab.trans.get() + i0;
   c.conj = tab.conj.get() + i0;
   block_copy(c, s);
+  st.done(s);
 }
 
 // ---------------------------------------------------------------------------
@@ -800,7 +914,7 @@ void GwPlan::check_info() {
 }
 
 void GwPlan::iterate_device(const z64* G_dev, z64* S_dev) {
-  require(store.loaded, "gw: integrals not loaded", kErrState);
+  require(store.loaded(), "gw: integrals not loaded", kErrState);
   const int64_t n2 = (int64_t)nao * nao;
   while (qev.size() < (size_t)4 * sym.inq) {
     hipEvent_t e;
@@ -813,6 +927,7 @@ void GwPlan::iterate_device(const z64* G_dev, z64* S_dev) {
     ~Guard() { set_gemm_profiler(nullptr); }
   } guard(profiling ? &prof : nullptr);
   if (profiling) prof.reset();
+  const double streamed0 = store.bytes_streamed();
   GGPU_HIP(hipEventRecord(ev[0], stream));
   stage_G(G_dev);
   GGPU_HIP(hipEventRecord(ev[1], stream));
@@ -848,6 +963,7 @@ void GwPlan::iterate_device(const z64* G_dev, z64* S_dev) {
   GGPU_HIP(hipEventElapsedTime(&g, ev[0], ev[1]));
   GGPU_HIP(hipEventElapsedTime(&sg, ev[5], ev[6]));
   last_ms[0] = tot; last_ms[1] = g; last_ms[2] = t_p0; last_ms[3] = t_dy; last_ms[4] = t_sig; last_ms[5] = sg;
+  streamed_last = store.bytes_streamed() - streamed0;
   if (profiling) prof.collect();
   check_info();
 }
@@ -861,7 +977,7 @@ void GwPlan::iterate(const double* G, double* Sigma) {
 }
 
 void GwPlan::polarization(const double* G, int iq, int which, double* out) {
-  require(store.loaded, "gw: integrals not loaded", kErrState);
+  require(store.loaded(), "gw: integrals not loaded", kErrState);
   require(iq >= 0 && iq < sym.inq, "gw: iq out of range");
   require(which >= 0 && which <= 2, "gw: which must be 0, 1 or 2");
   require(G && out, "gw: NULL pointer");
@@ -1057,7 +1173,7 @@ void HfPlan::fetch(double* out) {
 }
 
 void HfPlan::exchange(const double* dm, double* K) {
-  require(store.loaded, "hf: integrals not loaded", kErrState);
+  require(store.loaded(), "hf: integrals not loaded", kErrState);
   require(dm && K, "hf: NULL pointer");
   GGPU_HIP(hipMemcpyAsync(dm_host_dev.get(), dm, dm_host_dev.bytes(), hipMemcpyHostToDevice, stream));
   stage_dm();
@@ -1068,7 +1184,7 @@ void HfPlan::exchange(const double* dm, double* K) {
 // F = J + K - c madelung S D S  (hf_gpu_kernel.cpp:66-104, :141-201); the
 // two-component variant follows hf_gpu_kernel.cpp:253-327.
 void HfPlan::solve(const double* dm, const double* S_k, double madelung, double* F) {
-  require(store.loaded, "hf: integrals not loaded", kErrState);
+  require(store.loaded(), "hf: integrals not loaded", kErrState);
   require(dm && S_k && F, "hf: NULL pointer");
   const int64_t n2 = (int64_t)nao * nao, stride = (int64_t)NQ * n2, so2 = (int64_t)nso * nso;
   GGPU_HIP(hipMemcpyAsync(dm_host_dev.get(), dm, dm_host_dev.bytes(), hipMemcpyHostToDevice, stream));

@@ -35,12 +35,44 @@ struct SymTables {
   bool q_identity() const { return U_q.empty(); }
 };
 
-// Device copy of the stored k-pair integrals, (num_stored, NQ, nao, nao).
-struct IntegralStore {
-  DeviceBuffer<z64> V;
-  bool loaded = false;
+struct PairTable;
+
+// The stored k-pair integrals, (num_stored, NQ, nao, nao): either resident in
+// HBM, or in pinned host memory and streamed pair batch by pair batch into a
+// double-buffered device staging area on a copy stream of their own, each
+// batch's copy overlapping the previous batch's compute.
+class IntegralStore {
+ public:
+  ~IntegralStore();
+  void set_host_resident(bool on, int slots, int64_t pair_elems);   // before loading
   void load_host(const double* V_stored, int64_t n_elems, hipStream_t s);
   void synth(const SymTables& t, int NQ, int nao, uint64_t seed, double sigma, hipStream_t s);
+  bool loaded() const { return loaded_; }
+  bool host_resident() const { return host_ != nullptr || host_mode_; }
+  int64_t elems() const { return n_elems_; }
+  double bytes_streamed() const { return streamed_; }
+  // Device view of pairs [i0, i0 + n) of `tab` and the element offset of each
+  // (n entries, device): the store itself when resident in HBM, else the
+  // staging buffer the batch was copied to.  `s` waits for the copy; call
+  // done(s) after the last read of the view to release the buffer.
+  const z64* view(const PairTable& tab, int i0, int n, hipStream_t s, const int64_t** offsets);
+  void done(hipStream_t s);
+
+ private:
+  void alloc_host(int64_t n_elems);
+  void issue(const PairTable& tab, int i0, int n, int buf);
+  DeviceBuffer<z64> V_;
+  z64* host_ = nullptr;
+  bool host_mode_ = false, loaded_ = false;
+  int64_t n_elems_ = 0, pair_elems_ = 0;
+  int slots_ = 0;
+  hipStream_t copy_ = nullptr;
+  hipEvent_t ev_copy_[2] = {nullptr, nullptr}, ev_used_[2] = {nullptr, nullptr};
+  DeviceBuffer<z64> stage_;
+  DeviceBuffer<int64_t> stage_off_;        // [2][slots]: (buf * slots + b) * pair_elems
+  struct Pending { const PairTable* tab = nullptr; int i0 = -1, n = 0, buf = -1; } pre_;
+  int cur_ = -1, next_ = 0;
+  double streamed_ = 0;
 };
 
 // Per-outer-batch tables for BlockCopy (device arrays uploaded once).
@@ -52,7 +84,7 @@ struct PairTable {
 };
 
 // Unfold pairs [i0, i0+n) of a PairTable into dst[b] (NQ x nao x nao each).
-void unfold_pairs(const IntegralStore& st, const PairTable& tab, int i0, int n, const int64_t* out_off_dev,
+void unfold_pairs(IntegralStore& st, const PairTable& tab, int i0, int n, const int64_t* out_off_dev,
                   void* dst, int out_f32, int NQ, int nao, hipStream_t s);
 
 // Rotate IBZ matrices into the full BZ: out[k][s][t] = U_k X[t][s][ibz(k)] U_k^dagger
@@ -129,6 +161,7 @@ struct GwPlan {
   FullBZRotator rot;
   int kb = 1;                           // pairs per batch
   double last_ms[6] = {0, 0, 0, 0, 0, 0};
+  double streamed_last = 0;             // integral bytes copied host -> device by the last iterate
   bool profiling = false;
   SiteProfiler prof;
   std::vector<hipEvent_t> qev;          // per-q phase events

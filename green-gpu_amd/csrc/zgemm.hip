@@ -595,15 +595,26 @@ template <class TO, int SITE>
 void launch_bf16x3_site(const GemmDesc& d, hipStream_t stream) {
   int mode = g_bf16_tiles;
   if (mode < 0) {
-    // two waves per SIMD (the 8- and 10-block wave tiles) beat the bigger
-    // 16-block tiles; among those, the one that wastes least of M x N
+    // Two waves per SIMD (8- and 10-block wave tiles) beat bigger tiles.
+    // Large shapes take the tile measured fastest for their call site on
+    // MI355X (nao = 52, NQ = 300, nts = 110) unless its tile waste is more
+    // than 10 % above the best fit; other shapes take the best fit.
     mode = kBf16Tile64x64;
     if (d.M >= 64 && d.N >= 48 && (d.M >= 128 || d.N >= 128)) {
-      const double u1 = tile_use(d.M, 128) * tile_use(d.N, 64), u4 = tile_use(d.M, 64) * tile_use(d.N, 128),
-                   u5 = tile_use(d.M, 160) * tile_use(d.N, 64);
-      mode = kBf16Tile128x64;
-      if (u4 > u1 + 0.03 && u4 >= u5) mode = kBf16Tile64x128;
-      else if (u5 > u1 + 0.03) mode = kBf16Tile160x64;
+      const int cand[3] = {kBf16Tile128x64, kBf16Tile64x128, kBf16Tile160x64};
+      const double use[3] = {tile_use(d.M, 128) * tile_use(d.N, 64), tile_use(d.M, 64) * tile_use(d.N, 128),
+                             tile_use(d.M, 160) * tile_use(d.N, 64)};
+      int best = 0;
+      for (int c = 1; c < 3; ++c)
+        if (use[c] > use[best] + 0.03) best = c;
+      int pref = -1;
+      switch (SITE) {
+        case kSiteP0X1: pref = 1; break;                   // 64 x 128
+        case kSiteP0X2: case kSiteP0: case kSiteSigY1: pref = 2; break;   // 160 x 64
+        case kSiteSigY2: case kSiteSigma: pref = 0; break;  // 128 x 64
+        default: break;
+      }
+      mode = cand[(pref >= 0 && use[pref] >= 0.9 * use[best]) ? pref : best];
     }
   }
   switch (mode) {

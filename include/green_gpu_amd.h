@@ -48,6 +48,13 @@ typedef enum {
   GGPU_PRECISION_BF16X3 = 2   /* complex64 data, split-bf16 x3 products, FP32 accumulation        */
 } ggpu_precision;
 
+/* Where a plan keeps the stored integrals.  HOST: in pinned host memory,
+ * streamed pair batch by pair batch into a double-buffered device staging
+ * area on a copy stream of the plan's own, each batch's copy overlapping the
+ * previous batch's compute (the reference's low-GPU-memory mode uploads every
+ * (k, q) pair, cugw_qkpt.cu:115-160).  DEVICE (default): loaded once into HBM. */
+typedef enum { GGPU_INTEGRALS_DEVICE = 0, GGPU_INTEGRALS_HOST = 1 } ggpu_integral_residence;
+
 /*
  * Symmetry and integral-storage tables: the plain data green-symmetry
  * provides (cu_symmetry.h:41-66, built at gw_gpu_kernel.cpp:34-91) plus the
@@ -105,6 +112,10 @@ int ggpu_device_memory(size_t* free_bytes, size_t* total_bytes);
  * ------------------------------------------------------------------------ */
 int ggpu_hf_plan_create(ggpu_hf_plan** plan, int nao, int nso, int ns, int NQ, const ggpu_symmetry* sym);
 
+/* Choose where the integrals live (ggpu_integral_residence); call before
+ * loading or synthesising them.  Results do not depend on the choice. */
+int ggpu_hf_set_integral_residence(ggpu_hf_plan* plan, int residence);
+
 /* Copy the stored integral pairs (num_kpair_stored x NQ x nao x nao complex,
  * the buffer df_integral_t::read_entire fills, df_integral_t.h:96-129) to the
  * device.  The host array is not referenced after the call. */
@@ -151,6 +162,9 @@ int ggpu_gw_plan_create(ggpu_gw_plan** plan, int nao, int nso, int ns, int NQ, i
                         const ggpu_symmetry* sym, const double* T_wt, const double* T_tw,
                         int precision, int solver, size_t work_bytes);
 
+int ggpu_gw_set_integral_residence(ggpu_gw_plan* plan, int residence);
+/* Bytes of integrals copied host -> device by the last iterate (0 when resident in HBM). */
+int ggpu_gw_streamed_bytes(ggpu_gw_plan* plan, double* bytes);
 int ggpu_gw_load_integrals(ggpu_gw_plan* plan, const double* V_stored);
 int ggpu_gw_synth_integrals(ggpu_gw_plan* plan, uint64_t seed, double sigma);
 
