@@ -64,6 +64,7 @@ def lib():
         L.ggpu_gw_synth_integrals.argtypes = [vp, u64, f64]
         L.ggpu_gw_iterate.argtypes = [vp, pd, pd]
         L.ggpu_gw_iterate_device.argtypes = [vp, vp, vp]
+        L.ggpu_gw_iterate_many.argtypes = [C.POINTER(vp), i32, C.POINTER(vp), C.POINTER(vp)]
         L.ggpu_gw_polarization.argtypes = [vp, pd, i32, i32, pd]
         L.ggpu_gw_last_timings.argtypes = [vp, pd]
         L.ggpu_gw_set_profiling.argtypes = [vp, i32]
@@ -80,7 +81,7 @@ EXPORTED = ["ggpu_last_error", "ggpu_device_count", "ggpu_device_memory",
             "ggpu_hf_exchange", "ggpu_hf_solve", "ggpu_hf_plan_destroy",
             "ggpu_gw_plan_create", "ggpu_gw_set_integral_residence", "ggpu_gw_streamed_bytes",
             "ggpu_gw_load_integrals", "ggpu_gw_synth_integrals",
-            "ggpu_gw_iterate", "ggpu_gw_iterate_device", "ggpu_gw_polarization",
+            "ggpu_gw_iterate", "ggpu_gw_iterate_device", "ggpu_gw_iterate_many", "ggpu_gw_polarization",
             "ggpu_gw_last_timings", "ggpu_gw_set_profiling", "ggpu_gw_site_stats",
             "ggpu_gw_plan_destroy"]
 
@@ -215,6 +216,109 @@ def mesh_symmetry_none(mesh):
                     q_stars=[[q] for q in range(nk)],
                     k1_from_k2q=k1.astype(np.int64), k2_from_k1q=k2.astype(np.int64),
                     num_kpair_stored=nk * (nk + 1) // 2)
+
+
+def iterate_many(kernels, G_ptrs, Sigma_ptrs):
+    """One GW iteration for several independent systems (GwKernel objects of
+    any sizes) on device-resident G / Sigma; the systems overlap on the GPU."""
+    n = len(kernels)
+    if not (n == len(G_ptrs) == len(Sigma_ptrs)) or n == 0:
+        raise ValueError("one G and one Sigma pointer per kernel")
+    arr = lambda xs: (C.c_void_p * n)(*[C.c_void_p(x) for x in xs])
+    _check(lib().ggpu_gw_iterate_many(arr([k._h.value for k in kernels]), n, arr(G_ptrs), arr(Sigma_ptrs)))
+
+
+_hip = None
+
+
+def _hip_runtime():
+    """The HIP runtime libgreen_gpu_amd.so itself is linked against (already
+    loaded by the library; dlopen resolves the same soname to it).  Device
+    buffers handed to the library must come from this runtime: a second HIP
+    runtime in the process (torch's bundled one, for instance) keeps its own
+    view of the device and does not interoperate with it."""
+    global _hip
+    if _hip is None:
+        lib()
+        H = C.CDLL("libamdhip64.so.7")
+        vp, sz = C.c_void_p, C.c_size_t
+        H.hipMalloc.argtypes = [C.POINTER(vp), sz]
+        H.hipFree.argtypes = [vp]
+        H.hipMemcpy.argtypes = [vp, vp, sz, C.c_int]
+        H.hipMemset.argtypes = [vp, C.c_int, sz]
+        H.hipDeviceSynchronize.argtypes = []
+        H.hipGetErrorString.argtypes = [C.c_int]
+        H.hipGetErrorString.restype = C.c_char_p
+        _hip = H
+    return _hip
+
+
+def _hip_check(rc, what):
+    if rc != 0:
+        raise GgpuError(rc, f"{what}: {_hip_runtime().hipGetErrorString(rc).decode()}")
+
+
+_H2D, _D2H, _D2D = 1, 2, 3
+
+
+class DeviceArray:
+    """A device buffer holding a copy of a numpy array (dtype and shape kept
+    on the host side).  ``ptr`` is what the *_device entry points take."""
+
+    def __init__(self, shape, dtype=np.complex128, zero=False):
+        self.shape, self.dtype = tuple(shape), np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
+        p = C.c_void_p()
+        _hip_check(_hip_runtime().hipMalloc(C.byref(p), max(self.nbytes, 1)), "hipMalloc")
+        self._p = p
+        if zero:
+            _hip_check(_hip_runtime().hipMemset(p, 0, self.nbytes), "hipMemset")
+            synchronize()
+
+    @classmethod
+    def from_numpy(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        _hip_check(_hip_runtime().hipMemcpy(d._p, a.ctypes.data, a.nbytes, _H2D), "hipMemcpy H2D")
+        synchronize()     # the library's streams do not wait on the null stream
+        return d
+
+    @classmethod
+    def empty_like(cls, other, zero=False):
+        return cls(other.shape, other.dtype, zero=zero)
+
+    @property
+    def ptr(self):
+        if self._p is None:
+            raise ValueError("device buffer already freed")
+        return self._p.value
+
+    def copy_from(self, other):
+        """Device-to-device copy of a buffer of the same size."""
+        if other.nbytes != self.nbytes:
+            raise ValueError("size mismatch")
+        _hip_check(_hip_runtime().hipMemcpy(self._p, other._p, self.nbytes, _D2D), "hipMemcpy D2D")
+        synchronize()
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        _hip_check(_hip_runtime().hipMemcpy(out.ctypes.data, self._p, self.nbytes, _D2H), "hipMemcpy D2H")
+        return out
+
+    def free(self):
+        if self._p is not None and self._p.value:
+            _hip_runtime().hipFree(self._p)
+        self._p = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def synchronize():
+    _hip_check(_hip_runtime().hipDeviceSynchronize(), "hipDeviceSynchronize")
 
 
 def device_count():

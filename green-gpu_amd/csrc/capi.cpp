@@ -3,8 +3,11 @@
 #include "../../include/green_gpu_amd.h"
 #include "engine.hpp"
 
+#include <exception>
 #include <new>
 #include <string>
+#include <thread>
+#include <vector>
 
 struct ggpu_hf_plan { ggpu::HfPlan* impl; };
 struct ggpu_gw_plan { ggpu::GwPlan* impl; };
@@ -170,6 +173,42 @@ int ggpu_gw_iterate_device(ggpu_gw_plan* plan, const void* G_device, void* Sigma
   return guarded([&] {
     ggpu::require(plan && plan->impl && G_device && Sigma_device, "NULL argument");
     plan->impl->iterate_device(static_cast<const ggpu::z64*>(G_device), static_cast<ggpu::z64*>(Sigma_device));
+  });
+}
+
+int ggpu_gw_iterate_many(ggpu_gw_plan* const* plans, int n, const void* const* G_device, void* const* Sigma_device) {
+  return guarded([&] {
+    ggpu::require(plans && G_device && Sigma_device && n >= 1, "NULL argument or no systems");
+    for (int i = 0; i < n; ++i) {
+      ggpu::require(plans[i] && plans[i]->impl && G_device[i] && Sigma_device[i], "NULL argument");
+      for (int j = 0; j < i; ++j) ggpu::require(plans[i] != plans[j], "the same plan appears twice");
+    }
+    int dev = 0;
+    GGPU_HIP(hipGetDevice(&dev));
+    // one host thread per system: each plan enqueues on its own stream, so
+    // the systems' kernels overlap on the device
+    std::vector<std::exception_ptr> err(n);
+    std::vector<std::thread> th;
+    th.reserve(n);
+    for (int i = 0; i < n; ++i)
+      th.emplace_back([&, i] {
+        try {
+          GGPU_HIP(hipSetDevice(dev));
+          plans[i]->impl->iterate_device(static_cast<const ggpu::z64*>(G_device[i]),
+                                         static_cast<ggpu::z64*>(Sigma_device[i]));
+        } catch (...) {
+          err[i] = std::current_exception();
+        }
+      });
+    for (auto& t : th) t.join();
+    for (int i = 0; i < n; ++i) {
+      if (!err[i]) continue;
+      try {
+        std::rethrow_exception(err[i]);
+      } catch (const ggpu::Error& e) {
+        throw ggpu::Error(e.code, "system " + std::to_string(i) + ": " + e.what());
+      }
+    }
   });
 }
 

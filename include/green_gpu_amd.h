@@ -176,6 +176,15 @@ int ggpu_gw_iterate(ggpu_gw_plan* plan, const double* G, double* Sigma_out);
 /* Same, with G and Sigma already in device memory (HBM-resident inputs). */
 int ggpu_gw_iterate_device(ggpu_gw_plan* plan, const void* G_device, void* Sigma_device);
 
+/* Several independent systems (plans of any sizes) in one call: each plan
+ * runs its iteration on its own stream from its own host thread, so small
+ * systems overlap on the device.  G_device[i] / Sigma_device[i] as for
+ * ggpu_gw_iterate_device; a plan may appear only once.  On failure the
+ * message names the first failing system.  No counterpart in the reference,
+ * which runs one system per process (gw_gpu_kernel::solve,
+ * gw_gpu_kernel.cpp:163-205, called once per iteration). */
+int ggpu_gw_iterate_many(ggpu_gw_plan* const* plans, int n, const void* const* G_device, void* const* Sigma_device);
+
 /* Intermediates of one IBZ q for error reports and tests:
  * which = 0: P0(tau)   (nts, NQ, NQ)   bare bubble, scaled by 1/nk and mirrored
  * which = 1: P(Omega)  (nw_b, NQ, NQ)  after the Dyson solve
