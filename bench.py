@@ -37,7 +37,14 @@ WORKLOADS = {
     # six independent systems of uneven size in one batched call
     "gw_batch6_nk27": dict(mesh=(3, 3, 3), ns=2, beta=100.0, lam=1e4,
                            systems=[(26, 100), (40, 160), (52, 220), (68, 280), (80, 340), (104, 420)]),
+    # the large system: 2.4 TB of stored pairs, so the integrals are generated
+    # per batch on the device; a bounded sample of q (one GPU's q share) is
+    # timed and the iteration extrapolated (every q costs the same without
+    # symmetry)
+    "gw_nk216_nao104_nq600": dict(mesh=(6, 6, 6), nao=104, NQ=600, ns=2, beta=100.0, lam=1e4,
+                                  integrals="generated", q_share="0/216"),
 }
+DEFAULT_WORKLOAD = "gw_nk64_nao52_nq300"
 
 
 def log(*a):
@@ -100,12 +107,17 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="gw_nk64_nao52_nq300", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="fp64,fp32,bf16x3",
                     help="comma-separated modes; fp64 always runs (the error reference)")
     ap.add_argument("--solver", default="cholesky", choices=["cholesky", "lu"])
-    ap.add_argument("--integrals", default="host", choices=["host", "device"],
-                    help="host: pinned host memory, streamed per pair batch (overlapped); device: resident in HBM")
+    ap.add_argument("--integrals", default=None, choices=["host", "device", "generated"],
+                    help="host: pinned host memory, streamed per pair batch (overlapped); device: resident in "
+                         "HBM; generated: synthesised per batch on the device (default: the workload's, else host)")
+    ap.add_argument("--q-share", default=None,
+                    help="R/S: time only IBZ q = R, R+S, ... (one GPU's share of a q-sharded iteration) and "
+                         "extrapolate the iteration (default: the workload's, else 0/1)")
+    ap.add_argument("--no-hf", action="store_true", help="skip the HF build")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--cpu-pairs", type=int, default=2)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -121,19 +133,25 @@ def main():
     import green_gpu_amd.synth as synth
 
     w = WORKLOADS[a.workload]
+    a.integrals = a.integrals or w.get("integrals", "host")
     if "systems" in w:
         return run_batch(a, w, pkg, synth)
     sym = pkg.mesh_symmetry_none(w["mesh"])
     nk, nao, NQ, ns = sym.nk, w["nao"], w["NQ"], w["ns"]
     tau, T_wt, T_tw = synth.load_ir(w["beta"], w["lam"])
     nts, nw_b = tau.size, T_wt.shape[0]
-    log(f"workload {a.workload}: nk={nk} nao={nao} NQ={NQ} ns={ns} nts={nts} nw_b={nw_b}")
+    q_rank, q_size = (int(x) for x in (a.q_share or w.get("q_share", "0/1")).split("/"))
+    if not (0 <= q_rank < q_size):
+        sys.exit("--q-share R/S needs 0 <= R < S")
+    q_run = len(range(q_rank, sym.inq, q_size))
+    log(f"workload {a.workload}: nk={nk} nao={nao} NQ={NQ} ns={ns} nts={nts} nw_b={nw_b}; "
+        f"q share {q_rank}/{q_size}: {q_run} of {sym.inq} q; integrals {a.integrals}")
     t = time.perf_counter()
     G, _ = synth.make_green(nk, nao, ns, w["beta"], tau, 1000)
     log(f"G generated in {time.perf_counter() - t:.1f}s")
     Gd = pkg.DeviceArray.from_numpy(G)
     seed, sigma = 1017, 0.1
-    fl = gw_flops(nk, nao, NQ, ns, nts, nw_b)
+    fl = {k: v * q_run / sym.inq for k, v in gw_flops(nk, nao, NQ, ns, nts, nw_b).items()}
     modes = [m.strip() for m in a.precision.split(",")]
     if "fp64" not in modes:
         modes = ["fp64"] + modes
@@ -141,6 +159,7 @@ def main():
     for mode in modes:
         kern = pkg.GwKernel(nao, ns, NQ, T_wt, T_tw, sym, precision=mode, solver=a.solver, integrals=a.integrals)
         kern.synth_integrals(seed, sigma)
+        kern.set_q_share(q_rank, q_size)
         Sd = pkg.DeviceArray.empty_like(Gd)
         pkg.synchronize()
         log(f"[{mode}] inputs resident in HBM")
@@ -159,7 +178,7 @@ def main():
         dt = (time.perf_counter() - t0) / a.steps
         kern.set_profiling(False)
         runs[mode] = {"s": dt, "sites": kern.site_stats(), "phases": kern.timings(),
-                      "streamed": kern.streamed_bytes()}
+                      "streamed": kern.streamed_bytes(), "generated": kern.generated_bytes()}
         sigmas[mode] = Sd.numpy()
         Sd.free()
         # P0(tau) of the first IBZ q, for the per-mode P0 error
@@ -189,20 +208,23 @@ def main():
 
     # HF build (exchange K + direct J; FP64 in every mode) on the same system:
     # D = -G(beta), synthetic integrals with the same seed
-    hk = pkg.HfKernel(nao, ns, NQ, sym)
-    hk.synth_integrals(seed, sigma)
-    D = -np.ascontiguousarray(G[-1])                                   # (ns, ink, nao, nao)
-    hk.solve(D)                                                        # warm-up
-    t = time.perf_counter()
-    F = hk.solve(D)
-    hf_s = time.perf_counter() - t
-    hk.close()
-    hf_fl = float(nk) * nk * ns * 16.0 * NQ * nao ** 3
-    # E_HF-like scalar, accumulated in FP64 on the host (the reference defines none)
-    e_hf = float(0.5 * np.real(np.einsum("skab,skba->", D, F)) / nk)
-    hf = {"seconds": round(hf_s, 4), "exchange_tflops": round(hf_fl / hf_s / 1e12, 2), "flop": hf_fl,
-          "e_hf_like": e_hf, "precision": "fp64", "includes": "K + J, host-to-host"}
-    log(f"[hf] {hf_s:.3f}s  {hf_fl / hf_s / 1e12:.1f} TFLOP/s (exchange FLOPs)")
+    hf = None
+    if not a.no_hf:
+        hk = pkg.HfKernel(nao, ns, NQ, sym, integrals="generated" if a.integrals == "generated" else "device")
+        hk.synth_integrals(seed, sigma)
+        D = -np.ascontiguousarray(G[-1])                                   # (ns, ink, nao, nao)
+        if q_run == sym.inq:
+            hk.solve(D)                                                    # warm-up
+        t = time.perf_counter()
+        F = hk.solve(D)
+        hf_s = time.perf_counter() - t
+        hk.close()
+        hf_fl = float(nk) * nk * ns * 16.0 * NQ * nao ** 3
+        # E_HF-like scalar, accumulated in FP64 on the host (the reference defines none)
+        e_hf = float(0.5 * np.real(np.einsum("skab,skba->", D, F)) / nk)
+        hf = {"seconds": round(hf_s, 4), "exchange_tflops": round(hf_fl / hf_s / 1e12, 2), "flop": hf_fl,
+              "e_hf_like": e_hf, "precision": "fp64", "includes": "K + J, host-to-host"}
+        log(f"[hf] {hf_s:.3f}s  {hf_fl / hf_s / 1e12:.1f} TFLOP/s (exchange FLOPs)")
 
     # headline: the fastest mode whose Sigma stays within 1e-6 of FP64
     ok = [m for m in modes if runs[m]["sigma_max_abs_err"] <= 1e-6]
@@ -228,7 +250,7 @@ def main():
         "data": "synthetic (seeded random H(k) -> G(tau); Gaussian V, sigma 0.1)",
         "config": {"workload": a.workload, "nk": nk, "nao": nao, "NQ": NQ, "ns": ns, "nts": nts,
                    "nw_b": nw_b, "beta": w["beta"], "ir_lambda": w["lam"], "precision": head,
-                   "solver": a.solver, "integrals": a.integrals},
+                   "solver": a.solver, "integrals": a.integrals, "q_share": f"{q_rank}/{q_size}"},
         "gw_iteration_s": round(r["s"], 3),
         "gw_flop_per_iteration": fl["total"],
         "modes": {m: {"gw_iteration_s": round(runs[m]["s"], 3), "tflops": round(runs[m]["tflops"], 2),
@@ -240,14 +262,19 @@ def main():
         "sigma_abs_max": scale,
         "integrals": {"residence": a.integrals,
                       "h2d_gb_per_iteration": round(r["streamed"] / 1e9, 2),
-                      "h2d_gb_per_s_average": round(r["streamed"] / 1e9 / r["s"], 2)},
+                      "h2d_gb_per_s_average": round(r["streamed"] / 1e9 / r["s"], 2),
+                      "generated_gb_per_iteration": round(r["generated"] / 1e9, 2)},
         "phases_ms": {k: round(v, 1) for k, v in r["phases"].items()},
         "sites": {k: {"ms": round(v[0], 1), "tflops": round(v[1] / (v[0] * 1e-3) / 1e12, 2), "launches": v[2]}
                   for k, v in r["sites"].items()},
         "roofline": roofline,
         "hf": hf,
     }
-    if not a.no_cpu:
+    if q_run < sym.inq:
+        out["q_share"] = {"rank": q_rank, "size": q_size, "q_timed": q_run, "inq": sym.inq,
+                          "extrapolated_iteration_s": {m: round(runs[m]["s"] * sym.inq / q_run, 1) for m in modes},
+                          "note": "step = this share's q-points; all q cost the same without symmetry"}
+    if not a.no_cpu and a.workload == DEFAULT_WORKLOAD:
         pairs = {}
         for k in range(a.cpu_pairs):
             for k1 in (int(sym.k1_from_k2q[k, 0]), int(sym.k2_from_k1q[k, 0])):

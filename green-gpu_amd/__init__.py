@@ -24,7 +24,7 @@ LIB_PATH = os.path.join(HERE, "libgreen_gpu_amd.so")
 
 PRECISION = {"fp64": 0, "fp32": 1, "bf16x3": 2}
 SOLVER = {"cholesky": 0, "lu": 1}
-RESIDENCE = {"device": 0, "host": 1}
+RESIDENCE = {"device": 0, "host": 1, "generated": 2}
 
 
 class GgpuError(RuntimeError):
@@ -60,6 +60,8 @@ def lib():
                                           pd, pd, i32, i32, sz]
         L.ggpu_gw_set_integral_residence.argtypes = [vp, i32]
         L.ggpu_gw_streamed_bytes.argtypes = [vp, pd]
+        L.ggpu_gw_generated_bytes.argtypes = [vp, pd]
+        L.ggpu_gw_set_q_share.argtypes = [vp, i32, i32]
         L.ggpu_gw_load_integrals.argtypes = [vp, pd]
         L.ggpu_gw_synth_integrals.argtypes = [vp, u64, f64]
         L.ggpu_gw_iterate.argtypes = [vp, pd, pd]
@@ -80,6 +82,7 @@ EXPORTED = ["ggpu_last_error", "ggpu_device_count", "ggpu_device_memory",
             "ggpu_hf_synth_integrals",
             "ggpu_hf_exchange", "ggpu_hf_solve", "ggpu_hf_plan_destroy",
             "ggpu_gw_plan_create", "ggpu_gw_set_integral_residence", "ggpu_gw_streamed_bytes",
+            "ggpu_gw_generated_bytes", "ggpu_gw_set_q_share",
             "ggpu_gw_load_integrals", "ggpu_gw_synth_integrals",
             "ggpu_gw_iterate", "ggpu_gw_iterate_device", "ggpu_gw_iterate_many", "ggpu_gw_polarization",
             "ggpu_gw_last_timings", "ggpu_gw_set_profiling", "ggpu_gw_site_stats",
@@ -442,6 +445,17 @@ class GwKernel:
         b = C.c_double()
         _check(lib().ggpu_gw_streamed_bytes(self._h, C.byref(b)))
         return b.value
+
+    def generated_bytes(self):
+        """Integral bytes generated on the device by the last iteration."""
+        b = C.c_double()
+        _check(lib().ggpu_gw_generated_bytes(self._h, C.byref(b)))
+        return b.value
+
+    def set_q_share(self, rank, size):
+        """Run only IBZ q = rank, rank + size, ... (cugw_utils.cu:115); the
+        Sigma of all shares sums to the full Sigma."""
+        _check(lib().ggpu_gw_set_q_share(self._h, int(rank), int(size)))
 
     def timings(self):
         t = np.zeros(6)

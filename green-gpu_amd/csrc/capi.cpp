@@ -72,9 +72,8 @@ int ggpu_hf_plan_create(ggpu_hf_plan** plan, int nao, int nso, int ns, int NQ, c
 int ggpu_hf_set_integral_residence(ggpu_hf_plan* plan, int residence) {
   return guarded([&] {
     ggpu::require(plan && plan->impl, "NULL plan");
-    ggpu::require(residence == GGPU_INTEGRALS_DEVICE || residence == GGPU_INTEGRALS_HOST, "unknown integral residence");
     auto* p = plan->impl;
-    p->store.set_host_resident(residence == GGPU_INTEGRALS_HOST, p->kb, (int64_t)p->NQ * p->nao * p->nao);
+    p->store.set_residence(residence, p->kb, (int64_t)p->NQ * p->nao * p->nao);
   });
 }
 
@@ -133,9 +132,8 @@ int ggpu_gw_plan_create(ggpu_gw_plan** plan, int nao, int nso, int ns, int NQ, i
 int ggpu_gw_set_integral_residence(ggpu_gw_plan* plan, int residence) {
   return guarded([&] {
     ggpu::require(plan && plan->impl, "NULL plan");
-    ggpu::require(residence == GGPU_INTEGRALS_DEVICE || residence == GGPU_INTEGRALS_HOST, "unknown integral residence");
     auto* p = plan->impl;
-    p->store.set_host_resident(residence == GGPU_INTEGRALS_HOST, p->kb, (int64_t)p->NQ * p->nao * p->nao);
+    p->store.set_residence(residence, p->kb, (int64_t)p->NQ * p->nao * p->nao);
   });
 }
 
@@ -143,6 +141,22 @@ int ggpu_gw_streamed_bytes(ggpu_gw_plan* plan, double* bytes) {
   return guarded([&] {
     ggpu::require(plan && plan->impl && bytes, "NULL argument");
     *bytes = plan->impl->streamed_last;
+  });
+}
+
+int ggpu_gw_generated_bytes(ggpu_gw_plan* plan, double* bytes) {
+  return guarded([&] {
+    ggpu::require(plan && plan->impl && bytes, "NULL argument");
+    *bytes = plan->impl->generated_last;
+  });
+}
+
+int ggpu_gw_set_q_share(ggpu_gw_plan* plan, int rank, int size) {
+  return guarded([&] {
+    ggpu::require(plan && plan->impl, "NULL plan");
+    ggpu::require(size >= 1 && rank >= 0 && rank < size, "q share: need 0 <= rank < size");
+    plan->impl->share_rank = rank;
+    plan->impl->share_size = size;
   });
 }
 

@@ -148,10 +148,11 @@ IntegralStore::~IntegralStore() {
   if (host_) (void)hipHostFree(host_);
 }
 
-void IntegralStore::set_host_resident(bool on, int slots, int64_t pair_elems) {
+void IntegralStore::set_residence(int mode, int slots, int64_t pair_elems) {
   require(!loaded_, "integrals: choose the residence before loading integrals", kErrState);
-  host_mode_ = on;
-  if (!on) return;
+  require(mode == kResDevice || mode == kResHost || mode == kResGenerated, "integrals: unknown residence");
+  mode_ = mode;
+  if (mode == kResDevice) return;
   require(slots >= 1 && pair_elems > 0, "integrals: bad staging size");
   slots_ = slots;
   pair_elems_ = pair_elems;
@@ -178,7 +179,8 @@ void IntegralStore::alloc_host(int64_t n_elems) {
 
 void IntegralStore::load_host(const double* V_stored, int64_t n_elems, hipStream_t s) {
   require(V_stored != nullptr, "integrals: NULL pointer");
-  if (host_mode_) {
+  require(mode_ != kResGenerated, "integrals: a generated store takes no loaded integrals", kErrState);
+  if (mode_ == kResHost) {
     alloc_host(n_elems);
     std::memcpy(host_, V_stored, (size_t)n_elems * sizeof(z64));
   } else {
@@ -200,7 +202,15 @@ void IntegralStore::synth(const SymTables& t, int NQ, int nao, uint64_t seed, do
   DeviceBuffer<int> d(diag.size());
   GGPU_HIP(hipMemcpy(d.get(), diag.data(), diag.size() * sizeof(int), hipMemcpyHostToDevice));
   const int64_t per = (int64_t)NQ * nao * nao, n = t.num_stored * per;
-  if (host_mode_) {
+  if (mode_ == kResGenerated) {
+    // nothing is stored: issue() generates each batch of pairs on demand
+    require(per == pair_elems_, "integrals: pair size differs from the staging size");
+    diag_ = std::move(d);
+    seed_ = seed;
+    sigma_ = sigma;
+    NQ_ = NQ;
+    nao_ = nao;
+  } else if (mode_ == kResHost) {
     // generate on the device in chunks of pairs, copy each to pinned memory
     alloc_host(n);
     const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(t.num_stored, ((int64_t)1 << 30) / (per * 16)));
@@ -229,16 +239,19 @@ void IntegralStore::issue(const PairTable& tab, int i0, int n, int buf) {
   for (int i = 0; i < n; ++i) {
     const int64_t pos = tab.host[i0 + i].pos;
     require(pos >= 0 && (pos + 1) * pair_elems_ <= n_elems_, "integrals: pair outside store");
-    GGPU_HIP(hipMemcpyAsync(stage_.get() + ((int64_t)buf * slots_ + i) * pair_elems_, host_ + pos * pair_elems_,
-                            bytes, hipMemcpyHostToDevice, copy_));
+    z64* dst = stage_.get() + ((int64_t)buf * slots_ + i) * pair_elems_;
+    if (mode_ == kResGenerated)
+      synth_integrals(dst, pos, 1, diag_.get() + pos, NQ_, nao_, seed_, sigma_, copy_);
+    else
+      GGPU_HIP(hipMemcpyAsync(dst, host_ + pos * pair_elems_, bytes, hipMemcpyHostToDevice, copy_));
   }
   GGPU_HIP(hipEventRecord(ev_copy_[buf], copy_));
-  streamed_ += (double)bytes * n;
+  (mode_ == kResGenerated ? generated_ : streamed_) += (double)bytes * n;
 }
 
 const z64* IntegralStore::view(const PairTable& tab, int i0, int n, hipStream_t s, const int64_t** offsets) {
   require(i0 >= 0 && i0 + n <= (int)tab.host.size(), "integrals: pair range");
-  if (!host_) {
+  if (mode_ == kResDevice) {
     *offsets = tab.in_off.get() + i0;
     return V_.get();
   }
@@ -267,7 +280,7 @@ const z64* IntegralStore::view(const PairTable& tab, int i0, int n, hipStream_t 
 }
 
 void IntegralStore::done(hipStream_t s) {
-  if (!host_ || cur_ < 0) return;
+  if (mode_ == kResDevice || cur_ < 0) return;
   GGPU_HIP(hipEventRecord(ev_used_[cur_], s));
   cur_ = -1;
 }
@@ -927,13 +940,13 @@ void GwPlan::iterate_device(const z64* G_dev, z64* S_dev) {
     ~Guard() { set_gemm_profiler(nullptr); }
   } guard(profiling ? &prof : nullptr);
   if (profiling) prof.reset();
-  const double streamed0 = store.bytes_streamed();
+  const double streamed0 = store.bytes_streamed(), generated0 = store.bytes_generated();
   GGPU_HIP(hipEventRecord(ev[0], stream));
   stage_G(G_dev);
   GGPU_HIP(hipEventRecord(ev[1], stream));
   zero(Sig_acc.get(), Sig_acc.size(), stream);
   zero_int(info.get(), nw_b, stream);
-  for (int iq = 0; iq < sym.inq; ++iq) {
+  for (int iq = share_rank; iq < sym.inq; iq += share_size) {
     GGPU_HIP(hipEventRecord(qev[4 * iq + 0], stream));
     p0_for_q(iq);
     GGPU_HIP(hipEventRecord(qev[4 * iq + 1], stream));
@@ -951,7 +964,7 @@ void GwPlan::iterate_device(const z64* G_dev, z64* S_dev) {
   GGPU_HIP(hipEventRecord(ev[6], stream));
   GGPU_HIP(hipStreamSynchronize(stream));
   double t_p0 = 0, t_dy = 0, t_sig = 0;
-  for (int iq = 0; iq < sym.inq; ++iq) {
+  for (int iq = share_rank; iq < sym.inq; iq += share_size) {
     float a = 0, b = 0, c = 0;
     GGPU_HIP(hipEventElapsedTime(&a, qev[4 * iq + 0], qev[4 * iq + 1]));
     GGPU_HIP(hipEventElapsedTime(&b, qev[4 * iq + 1], qev[4 * iq + 2]));
@@ -964,6 +977,7 @@ void GwPlan::iterate_device(const z64* G_dev, z64* S_dev) {
   GGPU_HIP(hipEventElapsedTime(&sg, ev[5], ev[6]));
   last_ms[0] = tot; last_ms[1] = g; last_ms[2] = t_p0; last_ms[3] = t_dy; last_ms[4] = t_sig; last_ms[5] = sg;
   streamed_last = store.bytes_streamed() - streamed0;
+  generated_last = store.bytes_generated() - generated0;
   if (profiling) prof.collect();
   check_info();
 }

@@ -44,13 +44,19 @@ struct PairTable;
 class IntegralStore {
  public:
   ~IntegralStore();
-  void set_host_resident(bool on, int slots, int64_t pair_elems);   // before loading
+  // Where the stored pairs live (before loading): kResDevice (HBM),
+  // kResHost (pinned host memory, copied per pair batch into staging) or
+  // kResGenerated (synthetic pairs generated per batch into staging on the
+  // device; only synth() fills such a store).
+  enum { kResDevice = 0, kResHost = 1, kResGenerated = 2 };
+  void set_residence(int mode, int slots, int64_t pair_elems);
   void load_host(const double* V_stored, int64_t n_elems, hipStream_t s);
   void synth(const SymTables& t, int NQ, int nao, uint64_t seed, double sigma, hipStream_t s);
   bool loaded() const { return loaded_; }
-  bool host_resident() const { return host_ != nullptr || host_mode_; }
+  bool staged() const { return mode_ != kResDevice; }
   int64_t elems() const { return n_elems_; }
   double bytes_streamed() const { return streamed_; }
+  double bytes_generated() const { return generated_; }
   // Device view of pairs [i0, i0 + n) of `tab` and the element offset of each
   // (n entries, device): the store itself when resident in HBM, else the
   // staging buffer the batch was copied to.  `s` waits for the copy; call
@@ -63,7 +69,13 @@ class IntegralStore {
   void issue(const PairTable& tab, int i0, int n, int buf);
   DeviceBuffer<z64> V_;
   z64* host_ = nullptr;
-  bool host_mode_ = false, loaded_ = false;
+  int mode_ = kResDevice;
+  bool loaded_ = false;
+  // generated residence: the synth() parameters and per-position diagonal flags
+  uint64_t seed_ = 0;
+  double sigma_ = 0;
+  int NQ_ = 0, nao_ = 0;
+  DeviceBuffer<int> diag_;
   int64_t n_elems_ = 0, pair_elems_ = 0;
   int slots_ = 0;
   hipStream_t copy_ = nullptr;
@@ -72,7 +84,7 @@ class IntegralStore {
   DeviceBuffer<int64_t> stage_off_;        // [2][slots]: (buf * slots + b) * pair_elems
   struct Pending { const PairTable* tab = nullptr; int i0 = -1, n = 0, buf = -1; } pre_;
   int cur_ = -1, next_ = 0;
-  double streamed_ = 0;
+  double streamed_ = 0, generated_ = 0;
 };
 
 // Per-outer-batch tables for BlockCopy (device arrays uploaded once).
@@ -162,6 +174,10 @@ struct GwPlan {
   int kb = 1;                           // pairs per batch
   double last_ms[6] = {0, 0, 0, 0, 0, 0};
   double streamed_last = 0;             // integral bytes copied host -> device by the last iterate
+  double generated_last = 0;            // integral bytes generated on the device by the last iterate
+  // This device's share of the IBZ q-points: iq = share_rank, share_rank +
+  // share_size, ... (the partial Sigma of all shares sums to the full one).
+  int share_rank = 0, share_size = 1;
   bool profiling = false;
   SiteProfiler prof;
   std::vector<hipEvent_t> qev;          // per-q phase events

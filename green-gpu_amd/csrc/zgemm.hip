@@ -74,12 +74,15 @@ __device__ inline int64_t xcd_logical(int64_t id, int64_t total) {
 // BKC: B's k index is the contiguous one (else its n index).
 // GAUSS: 3 real products per complex product (t1 = ar br, t2 = ai bi,
 //   t3 = (ar + ai)(br + bi); re = t1 - t2, im = t3 - t1 - t2) instead of 4.
-template <class T, class TO, bool AKC, bool BKC, bool GAUSS, int SITE>
+// BK: k per LDS stage (16, or 8 for half the LDS and twice the workgroups
+// per CU).
+template <class T, class TO, bool AKC, bool BKC, bool GAUSS, int SITE, int BK = BKT>
 __global__ __launch_bounds__(NTHREADS) void gemm_mfma_kernel(const GemmDesc d) {
   using CT = typename Cplx<T>::type;
   using AV = typename Acc<T>::type;
-  __shared__ T sA[2][2][BKT][BM + PAD];   // [buffer][re/im][k][m]
-  __shared__ T sB[2][2][BKT][BN + PAD];   // [buffer][re/im][k][n]
+  constexpr int EPT = BM * BK / NTHREADS;   // complex elements per thread and operand
+  __shared__ T sA[2][2][BK][BM + PAD];   // [buffer][re/im][k][m]
+  __shared__ T sB[2][2][BK][BN + PAD];   // [buffer][re/im][k][n]
 
   const int tid = threadIdx.x;
   const int tiles_m = (d.M + BM - 1) / BM;
@@ -95,40 +98,42 @@ __global__ __launch_bounds__(NTHREADS) void gemm_mfma_kernel(const GemmDesc d) {
   const CT* B = static_cast<const CT*>(d.B.ptr) + b0 * d.B.bs0 + (d.B.off1 ? d.B.off1[b1] : b1 * d.B.bs1);
   TO* C = static_cast<TO*>(d.C.ptr) + b0 * d.C.bs0 + (d.C.off1 ? d.C.off1[b1] : b1 * d.C.bs1);
 
-  // Loader coordinates inside the tile: 4 complex elements per thread.
-  int a_m[4], a_k[4], b_k[4], b_n[4];
+  // Loader coordinates inside the tile: EPT consecutive complex elements per
+  // thread along the operand's contiguous index.
+  constexpr int TK = BK / EPT, TMN = BM / EPT;   // threads per k-row / per m-row
+  int a_m[EPT], a_k[EPT], b_k[EPT], b_n[EPT];
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    if (AKC) { a_m[j] = tid >> 2; a_k[j] = ((tid & 3) << 2) + j; }
-    else     { a_k[j] = tid >> 4; a_m[j] = ((tid & 15) << 2) + j; }
-    if (BKC) { b_n[j] = tid >> 2; b_k[j] = ((tid & 3) << 2) + j; }
-    else     { b_k[j] = tid >> 4; b_n[j] = ((tid & 15) << 2) + j; }
+  for (int j = 0; j < EPT; ++j) {
+    if (AKC) { a_m[j] = tid / TK; a_k[j] = (tid % TK) * EPT + j; }
+    else     { a_k[j] = tid / TMN; a_m[j] = (tid % TMN) * EPT + j; }
+    if (BKC) { b_n[j] = tid / TK; b_k[j] = (tid % TK) * EPT + j; }
+    else     { b_k[j] = tid / TMN; b_n[j] = (tid % TMN) * EPT + j; }
   }
 
-  const int ktiles = (d.K + BKT - 1) / BKT;
+  const int ktiles = (d.K + BK - 1) / BK;
   const int niter = d.KB * ktiles;
-  CT ra[4], rb[4];
+  CT ra[EPT], rb[EPT];
 
-  auto gload = [&](int it) {
+  auto gload = [&](int it, CT (&xa)[EPT], CT (&xb)[EPT]) {
     const int kb = it / ktiles;
-    const int k0 = (it - kb * ktiles) * BKT;
+    const int k0 = (it - kb * ktiles) * BK;
     const CT* Ab = A + kb * d.A.kbs;
     const CT* Bb = B + kb * d.B.kbs;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < EPT; ++j) {
       const int gm = m0 + a_m[j], gk = k0 + a_k[j];
-      ra[j] = (gm < d.M && gk < d.K) ? Ab[gm * d.A.rs + gk * d.A.cs] : CT{0, 0};
+      xa[j] = (gm < d.M && gk < d.K) ? Ab[gm * d.A.rs + gk * d.A.cs] : CT{0, 0};
       const int gk2 = k0 + b_k[j], gn = n0 + b_n[j];
-      rb[j] = (gk2 < d.K && gn < d.N) ? Bb[gk2 * d.B.rs + gn * d.B.cs] : CT{0, 0};
+      xb[j] = (gk2 < d.K && gn < d.N) ? Bb[gk2 * d.B.rs + gn * d.B.cs] : CT{0, 0};
     }
   };
-  auto sstore = [&](int buf) {
+  auto sstore = [&](int buf, const CT (&xa)[EPT], const CT (&xb)[EPT]) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      sA[buf][0][a_k[j]][a_m[j]] = ra[j].x;
-      sA[buf][1][a_k[j]][a_m[j]] = d.A.conj ? -ra[j].y : ra[j].y;
-      sB[buf][0][b_k[j]][b_n[j]] = rb[j].x;
-      sB[buf][1][b_k[j]][b_n[j]] = d.B.conj ? -rb[j].y : rb[j].y;
+    for (int j = 0; j < EPT; ++j) {
+      sA[buf][0][a_k[j]][a_m[j]] = xa[j].x;
+      sA[buf][1][a_k[j]][a_m[j]] = d.A.conj ? -xa[j].y : xa[j].y;
+      sB[buf][0][b_k[j]][b_n[j]] = xb[j].x;
+      sB[buf][1][b_k[j]][b_n[j]] = d.B.conj ? -xb[j].y : xb[j].y;
     }
   };
 
@@ -143,55 +148,62 @@ __global__ __launch_bounds__(NTHREADS) void gemm_mfma_kernel(const GemmDesc d) {
 #pragma unroll
       for (int j = 0; j < 2; ++j) acc[a][i][j] = AV{0, 0, 0, 0};
 
+  // One k-step of 4: fragments from LDS, then the block products.
+  auto kstep = [&](int buf, int kk) {
+    T ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      ar[i] = sA[buf][0][kk + lk][wm + 16 * i + lr];
+      ai[i] = sA[buf][1][kk + lk][wm + 16 * i + lr];
+      br[i] = sB[buf][0][kk + lk][wn + 16 * i + lr];
+      bi[i] = sB[buf][1][kk + lk][wn + 16 * i + lr];
+    }
+    if (GAUSS) {
+      T as[2], bs[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) { as[i] = ar[i] + ai[i]; bs[i] = br[i] + bi[i]; }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          acc[0][i][j] = mfma<T, AV>(ar[i], br[j], acc[0][i][j]);
+          acc[1][i][j] = mfma<T, AV>(ai[i], bi[j], acc[1][i][j]);
+          acc[GAUSS ? 2 : 0][i][j] = mfma<T, AV>(as[i], bs[j], acc[GAUSS ? 2 : 0][i][j]);
+        }
+    } else {
+      T nai[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) nai[i] = -ai[i];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          acc[0][i][j] = mfma<T, AV>(ar[i], br[j], acc[0][i][j]);
+          acc[1][i][j] = mfma<T, AV>(ar[i], bi[j], acc[1][i][j]);
+          acc[0][i][j] = mfma<T, AV>(nai[i], bi[j], acc[0][i][j]);
+          acc[1][i][j] = mfma<T, AV>(ai[i], br[j], acc[1][i][j]);
+        }
+    }
+  };
+
+  // All k-steps of k-tile it (LDS buffer it & 1).
+  auto ksteps = [&](int it) {
+    const int buf = it & 1;
+    const int k0 = (it % ktiles) * BK;
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4)
+      if (k0 + kk < d.K) kstep(buf, kk);   // uniform: skip steps past the end of K
+  };
+
   if (niter > 0) {
-    gload(0);
-    sstore(0);
+    gload(0, ra, rb);
+    sstore(0, ra, rb);
   }
   __syncthreads();
   for (int it = 0; it < niter; ++it) {
-    const int buf = it & 1;
-    if (it + 1 < niter) gload(it + 1);
-    const int k0 = (it % ktiles) * BKT;
-#pragma unroll
-    for (int kk = 0; kk < BKT; kk += 4) {
-      if (k0 + kk < d.K) {           // uniform: skip MFMA steps past the end of K
-        T ar[2], ai[2], br[2], bi[2];
-#pragma unroll
-        for (int i = 0; i < 2; ++i) {
-          ar[i] = sA[buf][0][kk + lk][wm + 16 * i + lr];
-          ai[i] = sA[buf][1][kk + lk][wm + 16 * i + lr];
-          br[i] = sB[buf][0][kk + lk][wn + 16 * i + lr];
-          bi[i] = sB[buf][1][kk + lk][wn + 16 * i + lr];
-        }
-        if (GAUSS) {
-          T as[2], bs[2];
-#pragma unroll
-          for (int i = 0; i < 2; ++i) { as[i] = ar[i] + ai[i]; bs[i] = br[i] + bi[i]; }
-#pragma unroll
-          for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int j = 0; j < 2; ++j) {
-              acc[0][i][j] = mfma<T, AV>(ar[i], br[j], acc[0][i][j]);
-              acc[1][i][j] = mfma<T, AV>(ai[i], bi[j], acc[1][i][j]);
-              acc[GAUSS ? 2 : 0][i][j] = mfma<T, AV>(as[i], bs[j], acc[GAUSS ? 2 : 0][i][j]);
-            }
-        } else {
-          T nai[2];
-#pragma unroll
-          for (int i = 0; i < 2; ++i) nai[i] = -ai[i];
-#pragma unroll
-          for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int j = 0; j < 2; ++j) {
-              acc[0][i][j] = mfma<T, AV>(ar[i], br[j], acc[0][i][j]);
-              acc[1][i][j] = mfma<T, AV>(ar[i], bi[j], acc[1][i][j]);
-              acc[0][i][j] = mfma<T, AV>(nai[i], bi[j], acc[0][i][j]);
-              acc[1][i][j] = mfma<T, AV>(ai[i], br[j], acc[1][i][j]);
-            }
-        }
-      }
-    }
-    if (it + 1 < niter) sstore(buf ^ 1);
+    if (it + 1 < niter) gload(it + 1, ra, rb);   // next k-tile, into registers
+    ksteps(it);
+    if (it + 1 < niter) sstore((it & 1) ^ 1, ra, rb);
     __syncthreads();
   }
 
@@ -649,19 +661,31 @@ void launch_bf16x3(const GemmDesc& d, hipStream_t stream) {
   }
 }
 
-template <class T, class TO, bool GAUSS, int SITE>
-void launch_site(const GemmDesc& d, hipStream_t stream) {
+template <class T, class TO, bool GAUSS, int SITE, int BK>
+void launch_site_bk(const GemmDesc& d, hipStream_t stream) {
   const int64_t tiles = (int64_t)((d.M + BM - 1) / BM) * ((d.N + BN - 1) / BN);
   const int64_t total = tiles * d.nb0 * d.nb1;
   require(total < (int64_t)1 << 31, "gemm: grid too large");
   dim3 grid((unsigned)total), block(NTHREADS);
   const bool akc = (d.A.cs == 1) || (d.A.rs != 1);
   const bool bkc = (d.B.rs == 1) || (d.B.cs != 1);
-  if (akc && bkc) hipLaunchKernelGGL((gemm_mfma_kernel<T, TO, true, true, GAUSS, SITE>), grid, block, 0, stream, d);
-  else if (akc) hipLaunchKernelGGL((gemm_mfma_kernel<T, TO, true, false, GAUSS, SITE>), grid, block, 0, stream, d);
-  else if (bkc) hipLaunchKernelGGL((gemm_mfma_kernel<T, TO, false, true, GAUSS, SITE>), grid, block, 0, stream, d);
-  else hipLaunchKernelGGL((gemm_mfma_kernel<T, TO, false, false, GAUSS, SITE>), grid, block, 0, stream, d);
+  if (akc && bkc) hipLaunchKernelGGL((gemm_mfma_kernel<T, TO, true, true, GAUSS, SITE, BK>), grid, block, 0, stream, d);
+  else if (akc) hipLaunchKernelGGL((gemm_mfma_kernel<T, TO, true, false, GAUSS, SITE, BK>), grid, block, 0, stream, d);
+  else if (bkc) hipLaunchKernelGGL((gemm_mfma_kernel<T, TO, false, true, GAUSS, SITE, BK>), grid, block, 0, stream, d);
+  else hipLaunchKernelGGL((gemm_mfma_kernel<T, TO, false, false, GAUSS, SITE, BK>), grid, block, 0, stream, d);
   GGPU_HIP(hipGetLastError());
+}
+
+int g_mfma_bk = 0;   // 0: per-shape choice; 8 or 16 forces the LDS k-depth
+
+template <class T, class TO, bool GAUSS, int SITE>
+void launch_site(const GemmDesc& d, hipStream_t stream) {
+  // LDS k-depth 8 measured fastest (or within noise) at every call site on
+  // MI355X (nao = 52, NQ = 300, nts = 110; interleaved medians).  All
+  // variants give the same sums bit for bit.
+  constexpr int pref = 8;
+  if ((g_mfma_bk ? g_mfma_bk : pref) == 8) launch_site_bk<T, TO, GAUSS, SITE, 8>(d, stream);
+  else launch_site_bk<T, TO, GAUSS, SITE, 16>(d, stream);
 }
 
 // FP64 uses the 3-product (Gauss) complex scheme; FP32 keeps 4 products
@@ -738,6 +762,7 @@ void gemm_check_bounds(const GemmDesc& d) {
 
 void set_gemm_profiler(GemmProfiler* p) { g_profiler = p; }
 void set_bf16_tiles(int mode) { g_bf16_tiles = mode; }
+void set_mfma_bk(int bk) { g_mfma_bk = bk; }
 
 void gemm(const GemmDesc& d, hipStream_t stream) {
   gemm_check_bounds(d);

@@ -76,6 +76,8 @@ void set_gemm_profiler(GemmProfiler* p);
 // default) or fixed to one of these for every launch (all give the same sums).
 enum { kBf16Tile64x64 = 0, kBf16Tile128x64 = 1, kBf16Tile64x128 = 4, kBf16Tile160x64 = 5 };
 void set_bf16_tiles(int mode);
+// LDS k-depth of the FP64 / FP32 kernel: 0 picks per shape, 8 or 16 forces it.
+void set_mfma_bk(int bk);
 
 struct GemmDesc {
   int site = kSiteOther;

@@ -43,8 +43,16 @@ def make_green(nk, nao, ns, beta, tau, seed):
     W, _ = np.linalg.qr(Z)
     eps = rng.uniform(-2.0, 2.0, size=(ns, nk, nao))
     f = fermi_weight(eps, tau, beta)                       # (ns, nk, nts, nao)
-    # G[t,s,k] = -W diag(f[s,k,t]) W^dagger
-    G = -np.einsum("skai,skti,skbi->tskab", W, f, np.conj(W), optimize=True)
+    # G[t,s,k] = -W diag(f[s,k,t]) W^dagger, as batched products per k-chunk
+    nts = f.shape[2]
+    G = np.empty((nts, ns, nk, nao, nao), dtype=np.complex128)
+    WH = np.conj(np.swapaxes(W, -1, -2))
+    kc = max(1, min(nk, (1 << 28) // (nts * nao * nao * 16)))
+    for s in range(ns):
+        for k0 in range(0, nk, kc):
+            k1 = min(nk, k0 + kc)
+            X = W[s, k0:k1, None, :, :] * f[s, k0:k1, :, None, :]      # (kc, nts, nao, nao)
+            G[:, s, k0:k1] = -np.swapaxes(X @ WH[s, k0:k1, None], 0, 1)
     D = -G[-1]
     return np.ascontiguousarray(G), np.ascontiguousarray(D)
 

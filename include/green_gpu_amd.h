@@ -52,8 +52,16 @@ typedef enum {
  * streamed pair batch by pair batch into a double-buffered device staging
  * area on a copy stream of the plan's own, each batch's copy overlapping the
  * previous batch's compute (the reference's low-GPU-memory mode uploads every
- * (k, q) pair, cugw_qkpt.cu:115-160).  DEVICE (default): loaded once into HBM. */
-typedef enum { GGPU_INTEGRALS_DEVICE = 0, GGPU_INTEGRALS_HOST = 1 } ggpu_integral_residence;
+ * (k, q) pair, cugw_qkpt.cu:115-160).  DEVICE (default): loaded once into HBM.
+ * GENERATED: nothing is stored; each batch of synthetic pairs (the values
+ * *_synth_integrals defines) is generated on the device into the staging area
+ * when it is needed, so systems whose integrals fit neither HBM nor host
+ * memory can be run; only *_synth_integrals fills such a plan. */
+typedef enum {
+  GGPU_INTEGRALS_DEVICE = 0,
+  GGPU_INTEGRALS_HOST = 1,
+  GGPU_INTEGRALS_GENERATED = 2
+} ggpu_integral_residence;
 
 /*
  * Symmetry and integral-storage tables: the plain data green-symmetry
@@ -165,6 +173,15 @@ int ggpu_gw_plan_create(ggpu_gw_plan** plan, int nao, int nso, int ns, int NQ, i
 int ggpu_gw_set_integral_residence(ggpu_gw_plan* plan, int residence);
 /* Bytes of integrals copied host -> device by the last iterate (0 when resident in HBM). */
 int ggpu_gw_streamed_bytes(ggpu_gw_plan* plan, double* bytes);
+/* Bytes of integrals generated on the device by the last iterate (GENERATED residence). */
+int ggpu_gw_generated_bytes(ggpu_gw_plan* plan, double* bytes);
+/* This device's share of the IBZ q-points: iterations then run
+ * iq = rank, rank + size, ... < inq only, and Sigma holds that share's
+ * contribution (still divided by nk), so the Sigma of shares 0 .. size-1
+ * sum to the full one.  The reference's per-device q loop,
+ * cugw_utils.cu:115 (q_ibz_idx = devices_rank; q_ibz_idx += devices_size).
+ * Default: rank 0 of 1. */
+int ggpu_gw_set_q_share(ggpu_gw_plan* plan, int rank, int size);
 int ggpu_gw_load_integrals(ggpu_gw_plan* plan, const double* V_stored);
 int ggpu_gw_synth_integrals(ggpu_gw_plan* plan, uint64_t seed, double sigma);
 

@@ -220,6 +220,47 @@ def test_integrals_streamed_from_host(ggpu, synth, oracle):
     assert np.array_equal(hk["device"].exchange(D), hk["host"].exchange(D))
 
 
+def test_generated_integrals_and_q_shares(ggpu, synth, oracle):
+    """GENERATED residence (synthetic pairs generated per batch on the device,
+    nothing stored) gives Sigma and K bit for bit equal to the same synthetic
+    integrals resident in HBM, and the Sigma of the q shares of the
+    reference's per-device loop (cugw_utils.cu:115: q = rank, rank + size, ...)
+    sums to the full Sigma."""
+    mesh, nao, NQ, ns, seed = (2, 2, 1), 10, 30, 2, 77
+    sym, tau, T_wt, T_tw, G, D, _ = gw_system(ggpu, synth, mesh, nao, NQ, ns, 50.0, 1e3, 51)
+    pair_bytes = NQ * nao * nao * 16
+    work = 3 * (2 + 2 * ns * tau.size) * pair_bytes          # ~3 pairs per batch
+    print("GW with generated integrals and q shares of 3", flush=True)
+    out, parts = {}, []
+    for where in ("device", "generated"):
+        k = ggpu.GwKernel(nao, ns, NQ, T_wt, T_tw, sym, solver="cholesky", work_bytes=work, integrals=where)
+        k.synth_integrals(seed, 0.1)
+        out[where] = k.solve(G)
+        out[where + "_gen"] = k.generated_bytes()
+        if where == "generated":
+            for r in range(3):
+                k.set_q_share(r, 3)
+                parts.append(k.solve(G))
+            with pytest.raises(ggpu.GgpuError):
+                k.set_q_share(3, 3)
+    assert np.array_equal(out["device"], out["generated"])
+    nk = sym.nk
+    assert out["device_gen"] == 0
+    assert out["generated_gen"] == pair_bytes * nk * (nk + nk)
+    full = out["generated"]
+    assert max_abs(sum(parts), full) <= 1e-12 * np.abs(full).max()
+    assert all(np.abs(p).max() > 0 for p in parts)
+    osym = oracle.symmetry_none(mesh, nao, NQ)
+    V = synth.make_integrals(nk, NQ, nao, seed, 0.1)
+    assert max_abs(out["device"], oracle.gw_solve(G, V, osym, T_wt, T_tw, solver="cholesky")) < TOL
+    hk = {w: ggpu.HfKernel(nao, ns, NQ, sym, integrals=w) for w in ("device", "generated")}
+    for w in hk:
+        hk[w].synth_integrals(seed, 0.1)
+    assert np.array_equal(hk["device"].exchange(D), hk["generated"].exchange(D))
+    with pytest.raises(ggpu.GgpuError):
+        hk["generated"].load_integrals(V)
+
+
 def test_gw_many_uneven_systems(ggpu, synth, oracle):
     """Independent systems of uneven nao / NQ / spin / precision in one
     ggpu_gw_iterate_many call (each on its own stream): every Sigma equals
