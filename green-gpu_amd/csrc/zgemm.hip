@@ -680,10 +680,12 @@ int g_mfma_bk = 0;   // 0: per-shape choice; 8 or 16 forces the LDS k-depth
 
 template <class T, class TO, bool GAUSS, int SITE>
 void launch_site(const GemmDesc& d, hipStream_t stream) {
-  // LDS k-depth 8 measured fastest (or within noise) at every call site on
-  // MI355X (nao = 52, NQ = 300, nts = 110; interleaved medians).  All
-  // variants give the same sums bit for bit.
-  constexpr int pref = 8;
+  // LDS k-depth measured on MI355X (nao = 52, NQ = 300, nts = 110;
+  // interleaved medians): 8 is faster or equal at every call site; for the
+  // FP64 Y2 = P^T Y1 product 8 and 16 are within noise and 16 is kept.
+  // Both depths give the same sums bit for bit.
+  constexpr bool f64 = std::is_same<T, double>::value;
+  constexpr int pref = (f64 && SITE == kSiteSigY2) ? 16 : 8;
   if ((g_mfma_bk ? g_mfma_bk : pref) == 8) launch_site_bk<T, TO, GAUSS, SITE, 8>(d, stream);
   else launch_site_bk<T, TO, GAUSS, SITE, 16>(d, stream);
 }
