@@ -50,6 +50,7 @@ def lib():
         L.ggpu_device_memory.argtypes = [C.POINTER(sz), C.POINTER(sz)]
         L.ggpu_hf_plan_create.argtypes = [C.POINTER(vp), i32, i32, i32, i32, C.POINTER(SymmetryC)]
         L.ggpu_hf_set_integral_residence.argtypes = [vp, i32]
+        L.ggpu_hf_set_k_share.argtypes = [vp, i32, i32]
         L.ggpu_hf_load_integrals.argtypes = [vp, pd]
         L.ggpu_hf_synth_integrals.argtypes = [vp, u64, f64]
         L.ggpu_hf_exchange.argtypes = [vp, pd, pd]
@@ -78,7 +79,8 @@ def lib():
 
 
 EXPORTED = ["ggpu_last_error", "ggpu_device_count", "ggpu_device_memory",
-            "ggpu_hf_plan_create", "ggpu_hf_set_integral_residence", "ggpu_hf_load_integrals",
+            "ggpu_hf_plan_create", "ggpu_hf_set_integral_residence", "ggpu_hf_set_k_share",
+            "ggpu_hf_load_integrals",
             "ggpu_hf_synth_integrals",
             "ggpu_hf_exchange", "ggpu_hf_solve", "ggpu_hf_plan_destroy",
             "ggpu_gw_plan_create", "ggpu_gw_set_integral_residence", "ggpu_gw_streamed_bytes",
@@ -365,6 +367,11 @@ class HfKernel:
 
     def synth_integrals(self, seed, sigma=0.1):
         _check(lib().ggpu_hf_synth_integrals(self._h, int(seed), float(sigma)))
+
+    def set_k_share(self, rank, size):
+        """Compute only the rows of IBZ k = rank, rank + size, ...
+        (cuhf_utils.cu:140); the other rows are zero."""
+        _check(lib().ggpu_hf_set_k_share(self._h, int(rank), int(size)))
 
     def exchange(self, dm):
         dm = _c128(dm, self.mat_shape)

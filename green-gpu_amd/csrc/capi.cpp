@@ -77,6 +77,15 @@ int ggpu_hf_set_integral_residence(ggpu_hf_plan* plan, int residence) {
   });
 }
 
+int ggpu_hf_set_k_share(ggpu_hf_plan* plan, int rank, int size) {
+  return guarded([&] {
+    ggpu::require(plan && plan->impl, "NULL plan");
+    ggpu::require(size >= 1 && rank >= 0 && rank < size, "k share: need 0 <= rank < size");
+    plan->impl->share_rank = rank;
+    plan->impl->share_size = size;
+  });
+}
+
 int ggpu_hf_load_integrals(ggpu_hf_plan* plan, const double* V_stored) {
   return guarded([&] {
     ggpu::require(plan && plan->impl, "NULL plan");

@@ -1117,7 +1117,8 @@ void HfPlan::exchange_device() {
   z64* Vdir = work.get();
   z64* X = Vdir + kb * stride;
   const int64_t xs = (int64_t)kb * stride;          // one spin slab of X
-  for (int ik = 0; ik < sym.ink; ++ik) {
+  if (share_size > 1) zero(Kout.get(), Kout.size(), stream);
+  for (int ik = share_rank; ik < sym.ink; ik += share_size) {
     zero(Kpart.get(), Kpart.size(), stream);
     for (int i0 = 0; i0 < sym.nk; i0 += kb) {
       const int n = std::min(kb, sym.nk - i0);
@@ -1277,6 +1278,10 @@ void HfPlan::solve(const double* dm, const double* S_k, double madelung, double*
       gemm_on(h, Ftmp, S_dev, Kout, stream, kSiteHF);
     }
   }
+  // rows of IBZ k outside this plan's share hold only J and Ewald: clear them
+  if (share_size > 1)
+    for (int ik = 0; ik < sym.ink; ++ik)
+      if (ik % share_size != share_rank) zero(Kout.get() + (int64_t)ik * nps * n2, (int64_t)nps * n2, stream);
   fetch(F);
 }
 

@@ -214,6 +214,10 @@ struct HfPlan {
   DeviceBuffer<int64_t> bidx_off;
   std::vector<PairTable> tables;   // per IBZ k: pairs (k, k2) for all k2
   int kb = 1, nsplit = 1;
+  // This device's share of the IBZ k-points: ik = share_rank, share_rank +
+  // share_size, ...; the other rows of K and F are zero, so the results of
+  // all shares sum to the full ones.
+  int share_rank = 0, share_size = 1;
 
   HfPlan(int nao, int nso, int ns, int NQ, const ggpu_symmetry& s);
   ~HfPlan();

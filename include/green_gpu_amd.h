@@ -123,6 +123,13 @@ int ggpu_hf_plan_create(ggpu_hf_plan** plan, int nao, int nso, int ns, int NQ, c
 /* Choose where the integrals live (ggpu_integral_residence); call before
  * loading or synthesising them.  Results do not depend on the choice. */
 int ggpu_hf_set_integral_residence(ggpu_hf_plan* plan, int residence);
+/* This device's share of the IBZ k-points: exchange and solve then compute
+ * the rows ik = rank, rank + size, ... < ink only and leave the other rows
+ * zero, so the results of shares 0 .. size-1 sum to the full K or Fock.  The
+ * reference's per-device k loop, cuhf_utils.cu:140 (k_reduced_id =
+ * devices_rank; k_reduced_id += devices_size), whose Fock rows are summed by
+ * the allreduce of hf_gpu_kernel.cpp:93.  Default: rank 0 of 1. */
+int ggpu_hf_set_k_share(ggpu_hf_plan* plan, int rank, int size);
 
 /* Copy the stored integral pairs (num_kpair_stored x NQ x nao x nao complex,
  * the buffer df_integral_t::read_entire fills, df_integral_t.h:96-129) to the

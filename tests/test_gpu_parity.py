@@ -225,7 +225,8 @@ def test_generated_integrals_and_q_shares(ggpu, synth, oracle):
     nothing stored) gives Sigma and K bit for bit equal to the same synthetic
     integrals resident in HBM, and the Sigma of the q shares of the
     reference's per-device loop (cugw_utils.cu:115: q = rank, rank + size, ...)
-    sums to the full Sigma."""
+    sums to the full Sigma; the HF k shares (cuhf_utils.cu:140) give the full
+    K and Fock rows of their IBZ k-points and zero elsewhere."""
     mesh, nao, NQ, ns, seed = (2, 2, 1), 10, 30, 2, 77
     sym, tau, T_wt, T_tw, G, D, _ = gw_system(ggpu, synth, mesh, nao, NQ, ns, 50.0, 1e3, 51)
     pair_bytes = NQ * nao * nao * 16
@@ -256,9 +257,23 @@ def test_generated_integrals_and_q_shares(ggpu, synth, oracle):
     hk = {w: ggpu.HfKernel(nao, ns, NQ, sym, integrals=w) for w in ("device", "generated")}
     for w in hk:
         hk[w].synth_integrals(seed, 0.1)
-    assert np.array_equal(hk["device"].exchange(D), hk["generated"].exchange(D))
+    K_full = hk["device"].exchange(D)
+    assert np.array_equal(K_full, hk["generated"].exchange(D))
     with pytest.raises(ggpu.GgpuError):
         hk["generated"].load_integrals(V)
+    # HF k shares (cuhf_utils.cu:140): K and the full Fock rows split by IBZ k
+    F_full = hk["device"].solve(D)
+    K_parts, F_parts = [], []
+    for r in range(3):
+        hk["device"].set_k_share(r, 3)
+        K_parts.append(hk["device"].exchange(D))
+        F_parts.append(hk["device"].solve(D))
+    for r in range(3):
+        other = [ik for ik in range(sym.ink) if ik % 3 != r]
+        assert np.abs(K_parts[r][:, other]).max() == 0 and np.abs(F_parts[r][:, other]).max() == 0
+        mine = [ik for ik in range(sym.ink) if ik % 3 == r]
+        assert np.array_equal(K_parts[r][:, mine], K_full[:, mine])
+        assert np.array_equal(F_parts[r][:, mine], F_full[:, mine])
 
 
 def test_gw_many_uneven_systems(ggpu, synth, oracle):
