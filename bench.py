@@ -258,7 +258,9 @@ def main():
                       "sigma_max_abs_err": runs[m]["sigma_max_abs_err"],
                       "sigma_rel_err": runs[m]["sigma_rel_err"],
                       "p0_max_abs_err": runs[m]["p0_max_abs_err"],
-                      "p0_rel_err": runs[m]["p0_rel_err"]} for m in modes},
+                      "p0_rel_err": runs[m]["p0_rel_err"],
+                      "phases_ms": {k: round(v, 1) for k, v in runs[m]["phases"].items()},
+                      "sites_ms": {k: round(v[0], 1) for k, v in runs[m]["sites"].items()}} for m in modes},
         "sigma_abs_max": scale,
         "integrals": {"residence": a.integrals,
                       "h2d_gb_per_iteration": round(r["streamed"] / 1e9, 2),

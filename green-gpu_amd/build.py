@@ -16,6 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libgreen_gpu_amd.so")
+SITE_BENCH = os.path.join(HERE, "gemm_site_bench")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
@@ -51,13 +52,20 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, LIB)
+    # the GEMM call-site benchmark, linked against the library just built
+    cmd = [HIPCC, "-O2", "-std=c++17", f"--offload-arch={ARCH}", os.path.join(CSRC, "gemm_site_bench.cpp"),
+           "-o", SITE_BENCH, "-L" + HERE, "-lgreen_gpu_amd", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("gemm_site_bench build failed:\n" + r.stdout + r.stderr)
     return LIB
 
 
 def clean():
     shutil.rmtree(BUILD, ignore_errors=True)
-    if os.path.exists(LIB):
-        os.remove(LIB)
+    for f in (LIB, SITE_BENCH):
+        if os.path.exists(f):
+            os.remove(f)
 
 
 if __name__ == "__main__":

@@ -76,8 +76,10 @@ __device__ inline int64_t xcd_logical(int64_t id, int64_t total) {
 //   t3 = (ar + ai)(br + bi); re = t1 - t2, im = t3 - t1 - t2) instead of 4.
 // BK: k per LDS stage (16, or 8 for half the LDS and twice the workgroups
 // per CU).
-template <class T, class TO, bool AKC, bool BKC, bool GAUSS, int SITE, int BK = BKT>
-__global__ __launch_bounds__(NTHREADS) void gemm_mfma_kernel(const GemmDesc d) {
+// WPE: waves per SIMD the register allocation must allow (1: no constraint).
+template <class T, class TO, bool AKC, bool BKC, bool GAUSS, int SITE, int BK = BKT, int WPE = 1>
+__global__ __launch_bounds__(NTHREADS) __attribute__((amdgpu_waves_per_eu(WPE))) void gemm_mfma_kernel(
+    const GemmDesc d) {
   using CT = typename Cplx<T>::type;
   using AV = typename Acc<T>::type;
   constexpr int EPT = BM * BK / NTHREADS;   // complex elements per thread and operand
@@ -424,8 +426,9 @@ __device__ inline void split2(float a, float b, unsigned& hi, unsigned& lo) {
   hi = h;
 }
 
-template <class TO, bool AKC, bool BKC, int WMB, int WNB, int SITE>
-__global__ __launch_bounds__(NTHREADS) void gemm_bf16x3_big_kernel(const GemmDesc d) {
+template <class TO, bool AKC, bool BKC, int WMB, int WNB, int SITE, int WPE = 1>
+__global__ __launch_bounds__(NTHREADS) __attribute__((amdgpu_waves_per_eu(WPE))) void gemm_bf16x3_big_kernel(
+    const GemmDesc d) {
   constexpr int TM = 32 * WMB, TN = 32 * WNB, LDK = XBK + XPAD;
   constexpr int GA = (TM * 4 + NTHREADS - 1) / NTHREADS;   // 8-element groups per thread
   constexpr int GB = (TN * 4 + NTHREADS - 1) / NTHREADS;
@@ -584,7 +587,7 @@ __global__ __launch_bounds__(NTHREADS) void gemm_bf16x3_big_kernel(const GemmDes
       }
 }
 
-template <class TO, int WMB, int WNB, int SITE>
+template <class TO, int WMB, int WNB, int SITE, int WPE = 1>
 void launch_bf16x3_big(const GemmDesc& d, hipStream_t stream) {
   constexpr int TM = 32 * WMB, TN = 32 * WNB;
   const int64_t tiles = (int64_t)((d.M + TM - 1) / TM) * ((d.N + TN - 1) / TN);
@@ -593,10 +596,14 @@ void launch_bf16x3_big(const GemmDesc& d, hipStream_t stream) {
   dim3 grid((unsigned)total), block(NTHREADS);
   const bool akc = (d.A.cs == 1) || (d.A.rs != 1);
   const bool bkc = (d.B.rs == 1) || (d.B.cs != 1);
-  if (akc && bkc) hipLaunchKernelGGL((gemm_bf16x3_big_kernel<TO, true, true, WMB, WNB, SITE>), grid, block, 0, stream, d);
-  else if (akc) hipLaunchKernelGGL((gemm_bf16x3_big_kernel<TO, true, false, WMB, WNB, SITE>), grid, block, 0, stream, d);
-  else if (bkc) hipLaunchKernelGGL((gemm_bf16x3_big_kernel<TO, false, true, WMB, WNB, SITE>), grid, block, 0, stream, d);
-  else hipLaunchKernelGGL((gemm_bf16x3_big_kernel<TO, false, false, WMB, WNB, SITE>), grid, block, 0, stream, d);
+  if (akc && bkc)
+    hipLaunchKernelGGL((gemm_bf16x3_big_kernel<TO, true, true, WMB, WNB, SITE, WPE>), grid, block, 0, stream, d);
+  else if (akc)
+    hipLaunchKernelGGL((gemm_bf16x3_big_kernel<TO, true, false, WMB, WNB, SITE, WPE>), grid, block, 0, stream, d);
+  else if (bkc)
+    hipLaunchKernelGGL((gemm_bf16x3_big_kernel<TO, false, true, WMB, WNB, SITE, WPE>), grid, block, 0, stream, d);
+  else
+    hipLaunchKernelGGL((gemm_bf16x3_big_kernel<TO, false, false, WMB, WNB, SITE, WPE>), grid, block, 0, stream, d);
   GGPU_HIP(hipGetLastError());
 }
 
@@ -607,23 +614,28 @@ template <class TO, int SITE>
 void launch_bf16x3_site(const GemmDesc& d, hipStream_t stream) {
   int mode = g_bf16_tiles;
   if (mode < 0) {
-    // Two waves per SIMD (8- and 10-block wave tiles) beat bigger tiles.
-    // Large shapes take the tile measured fastest for their call site on
-    // MI355X (nao = 52, NQ = 300, nts = 110) unless its tile waste is more
-    // than 10 % above the best fit; other shapes take the best fit.
+    // Two or three waves per SIMD (6- to 10-block wave tiles) beat bigger
+    // tiles.  Large shapes take the tile measured fastest for their call site
+    // inside a GW iteration on MI355X (nao = 52, NQ = 300, nts = 110; every
+    // tile forced in turn, per-launch times from the site profiler) unless
+    // its tile waste is more than 10 % above the best fit of the two-wave
+    // tiles; other shapes take that best fit.  (Repeated launches of one GEMM
+    // on the same operands rank the tiles differently: warm caches.)
     mode = kBf16Tile64x64;
     if (d.M >= 64 && d.N >= 48 && (d.M >= 128 || d.N >= 128)) {
-      const int cand[3] = {kBf16Tile128x64, kBf16Tile64x128, kBf16Tile160x64};
-      const double use[3] = {tile_use(d.M, 128) * tile_use(d.N, 64), tile_use(d.M, 64) * tile_use(d.N, 128),
-                             tile_use(d.M, 160) * tile_use(d.N, 64)};
+      const int cand[5] = {kBf16Tile128x64, kBf16Tile64x128, kBf16Tile160x64, kBf16Tile96x64W3, kBf16Tile64x64W3};
+      const double use[5] = {tile_use(d.M, 128) * tile_use(d.N, 64), tile_use(d.M, 64) * tile_use(d.N, 128),
+                             tile_use(d.M, 160) * tile_use(d.N, 64), tile_use(d.M, 96) * tile_use(d.N, 64),
+                             tile_use(d.M, 64) * tile_use(d.N, 64)};
       int best = 0;
       for (int c = 1; c < 3; ++c)
         if (use[c] > use[best] + 0.03) best = c;
       int pref = -1;
       switch (SITE) {
-        case kSiteP0X1: pref = 1; break;                   // 64 x 128
-        case kSiteP0X2: case kSiteP0: case kSiteSigY1: pref = 2; break;   // 160 x 64
-        case kSiteSigY2: case kSiteSigma: pref = 0; break;  // 128 x 64
+        case kSiteSigY1: case kSiteSigma: pref = 0; break;   // 128 x 64
+        case kSiteSigY2: pref = 1; break;                    // 64 x 128
+        case kSiteP0X2: case kSiteP0: pref = 2; break;       // 160 x 64
+        case kSiteP0X1: pref = 3; break;                     // 96 x 64, three waves
         default: break;
       }
       mode = cand[(pref >= 0 && use[pref] >= 0.9 * use[best]) ? pref : best];
@@ -633,6 +645,9 @@ void launch_bf16x3_site(const GemmDesc& d, hipStream_t stream) {
     case kBf16Tile128x64: return launch_bf16x3_big<TO, 4, 2, SITE>(d, stream);
     case kBf16Tile64x128: return launch_bf16x3_big<TO, 2, 4, SITE>(d, stream);
     case kBf16Tile160x64: return launch_bf16x3_big<TO, 5, 2, SITE>(d, stream);
+    case kBf16Tile96x64W3: return launch_bf16x3_big<TO, 3, 2, SITE, 3>(d, stream);
+    case kBf16Tile64x96W3: return launch_bf16x3_big<TO, 2, 3, SITE, 3>(d, stream);
+    case kBf16Tile64x64W3: return launch_bf16x3_big<TO, 2, 2, SITE, 3>(d, stream);
     default: break;
   }
   const int64_t tiles = (int64_t)((d.M + BM - 1) / BM) * ((d.N + BN - 1) / BN);
@@ -661,7 +676,7 @@ void launch_bf16x3(const GemmDesc& d, hipStream_t stream) {
   }
 }
 
-template <class T, class TO, bool GAUSS, int SITE, int BK>
+template <class T, class TO, bool GAUSS, int SITE, int BK, int WPE>
 void launch_site_bk(const GemmDesc& d, hipStream_t stream) {
   const int64_t tiles = (int64_t)((d.M + BM - 1) / BM) * ((d.N + BN - 1) / BN);
   const int64_t total = tiles * d.nb0 * d.nb1;
@@ -669,25 +684,33 @@ void launch_site_bk(const GemmDesc& d, hipStream_t stream) {
   dim3 grid((unsigned)total), block(NTHREADS);
   const bool akc = (d.A.cs == 1) || (d.A.rs != 1);
   const bool bkc = (d.B.rs == 1) || (d.B.cs != 1);
-  if (akc && bkc) hipLaunchKernelGGL((gemm_mfma_kernel<T, TO, true, true, GAUSS, SITE, BK>), grid, block, 0, stream, d);
-  else if (akc) hipLaunchKernelGGL((gemm_mfma_kernel<T, TO, true, false, GAUSS, SITE, BK>), grid, block, 0, stream, d);
-  else if (bkc) hipLaunchKernelGGL((gemm_mfma_kernel<T, TO, false, true, GAUSS, SITE, BK>), grid, block, 0, stream, d);
-  else hipLaunchKernelGGL((gemm_mfma_kernel<T, TO, false, false, GAUSS, SITE, BK>), grid, block, 0, stream, d);
+  if (akc && bkc) hipLaunchKernelGGL((gemm_mfma_kernel<T, TO, true, true, GAUSS, SITE, BK, WPE>), grid, block, 0, stream, d);
+  else if (akc) hipLaunchKernelGGL((gemm_mfma_kernel<T, TO, true, false, GAUSS, SITE, BK, WPE>), grid, block, 0, stream, d);
+  else if (bkc) hipLaunchKernelGGL((gemm_mfma_kernel<T, TO, false, true, GAUSS, SITE, BK, WPE>), grid, block, 0, stream, d);
+  else hipLaunchKernelGGL((gemm_mfma_kernel<T, TO, false, false, GAUSS, SITE, BK, WPE>), grid, block, 0, stream, d);
   GGPU_HIP(hipGetLastError());
 }
 
-int g_mfma_bk = 0;   // 0: per-shape choice; 8 or 16 forces the LDS k-depth
+// 0: per-site choice; 8 or 16 forces the LDS k-depth, 83 / 84 k-depth 8 with
+// registers allocated for 3 / 4 waves per SIMD
+int g_mfma_bk = 0;
 
 template <class T, class TO, bool GAUSS, int SITE>
 void launch_site(const GemmDesc& d, hipStream_t stream) {
-  // LDS k-depth measured on MI355X (nao = 52, NQ = 300, nts = 110;
-  // interleaved medians): 8 is faster or equal at every call site; for the
-  // FP64 Y2 = P^T Y1 product 8 and 16 are within noise and 16 is kept.
-  // Both depths give the same sums bit for bit.
+  // Variants measured on MI355X (nao = 52, NQ = 300, nts = 110; medians of
+  // interleaved rounds): staging 8 k per LDS step is faster than 16 or equal
+  // at every call site, and FP64 register allocation for three waves per
+  // SIMD (152-156 VGPRs, accumulators out of the AGPRs, no spills) instead
+  // of two gains 7-17 % at every site.  FP32 runs five waves unconstrained.
+  // All variants give the same sums bit for bit.
   constexpr bool f64 = std::is_same<T, double>::value;
-  constexpr int pref = (f64 && SITE == kSiteSigY2) ? 16 : 8;
-  if ((g_mfma_bk ? g_mfma_bk : pref) == 8) launch_site_bk<T, TO, GAUSS, SITE, 8>(d, stream);
-  else launch_site_bk<T, TO, GAUSS, SITE, 16>(d, stream);
+  constexpr int pref = f64 ? 83 : 8;
+  switch (g_mfma_bk ? g_mfma_bk : pref) {
+    case 8: return launch_site_bk<T, TO, GAUSS, SITE, 8, 1>(d, stream);
+    case 83: return launch_site_bk<T, TO, GAUSS, SITE, 8, 3>(d, stream);
+    case 84: return launch_site_bk<T, TO, GAUSS, SITE, 8, 4>(d, stream);
+    default: return launch_site_bk<T, TO, GAUSS, SITE, 16, 1>(d, stream);
+  }
 }
 
 // FP64 uses the 3-product (Gauss) complex scheme; FP32 keeps 4 products

@@ -74,10 +74,17 @@ struct GemmProfiler {
 void set_gemm_profiler(GemmProfiler* p);
 // Workgroup tile of the split-bf16 x3 GEMMs: chosen per shape (-1, the
 // default) or fixed to one of these for every launch (all give the same sums).
-enum { kBf16Tile64x64 = 0, kBf16Tile128x64 = 1, kBf16Tile64x128 = 4, kBf16Tile160x64 = 5 };
+// (the W3 tiles allocate registers for three waves per SIMD; their LDS
+// allows three workgroups per CU).
+enum {
+  kBf16Tile64x64 = 0, kBf16Tile128x64 = 1, kBf16Tile64x128 = 4, kBf16Tile160x64 = 5,
+  kBf16Tile96x64W3 = 6, kBf16Tile64x96W3 = 7, kBf16Tile64x64W3 = 8
+};
 void set_bf16_tiles(int mode);
-// LDS k-depth of the FP64 / FP32 kernel: 0 picks per shape, 8 or 16 forces it.
-void set_mfma_bk(int bk);
+// Variant of the FP64 / FP32 kernel: 0 picks per call site; 8 or 16 forces
+// the LDS k-depth; 83 or 84 forces k-depth 8 with registers allocated for 3 or
+// 4 waves per SIMD.  All variants give the same sums.
+void set_mfma_bk(int variant);
 
 struct GemmDesc {
   int site = kSiteOther;
