@@ -1059,7 +1059,10 @@ HfPlan::HfPlan(int nao_, int nso_, int ns_, int NQ_, const ggpu_symmetry& s)
   const size_t budget = std::min<size_t>((size_t)(0.4 * fr), (size_t)16 << 30);
   kb = (int)std::max<int64_t>(1, std::min<int64_t>(budget / per_pair, sym.nk));
   work.resize((size_t)kb * (per_pair / sizeof(z64)));
-  nsplit = 64;
+  // enough (b, Q) slices that a contraction launch (one nao x nao tile per
+  // density block and slice) has about 1024 workgroups on the 256 CUs
+  const int tiles = ((nao + 63) / 64) * ((nao + 63) / 64);
+  nsplit = std::max(64, std::min(512, 1024 / (tiles * nps)));
   Kpart.resize((size_t)nsplit * nps * n2);
   Kout.resize((size_t)sym.ink * nps * n2);
   jvec.resize(NQ);
@@ -1123,13 +1126,15 @@ void HfPlan::exchange_device() {
     for (int i0 = 0; i0 < sym.nk; i0 += kb) {
       const int n = std::min(kb, sym.nk - i0);
       unfold_pairs(store, tables[ik], i0, n, bidx_off.get(), Vdir, 0, NQ, nao, stream);
-      for (int b = 0; b < n; ++b) {
-        const int64_t k2 = i0 + b;
+      {
+        // X[s][b] = V(k, k2 = i0 + b) D_s(k2) for every density block s and
+        // pair b of the batch, in one launch
         GemmDesc g;
-        g.M = NQ * nao; g.N = nao; g.K = nao; g.nb0 = nps;
-        g.A.ptr = Vdir + b * stride; g.A.rs = nao; g.A.cs = 1;
-        g.B.ptr = dm_fbz.get() + k2 * nps * n2; g.B.rs = nao; g.B.cs = 1; g.B.bs0 = n2;
-        g.C.ptr = X + b * stride; g.C.rs = nao; g.C.cs = 1; g.C.bs0 = xs;
+        g.M = NQ * nao; g.N = nao; g.K = nao; g.nb0 = nps; g.nb1 = n;
+        g.A.ptr = Vdir; g.A.rs = nao; g.A.cs = 1; g.A.bs0 = 0; g.A.bs1 = stride;
+        g.B.ptr = dm_fbz.get() + (int64_t)i0 * nps * n2; g.B.rs = nao; g.B.cs = 1; g.B.bs0 = n2;
+        g.B.bs1 = (int64_t)nps * n2;
+        g.C.ptr = X; g.C.rs = nao; g.C.cs = 1; g.C.bs0 = xs; g.C.bs1 = stride;
         gemm_on(g, work, dm_fbz, work, stream, kSiteHF);
       }
       const int total = n * NQ;
