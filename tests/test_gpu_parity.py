@@ -304,6 +304,30 @@ def test_gw_many_uneven_systems(ggpu, synth, oracle):
             assert max_abs(S, oracle.gw_solve(G, V, osym, T_wt, T_tw, solver="cholesky")) < TOL
 
 
+def test_workload_dimensions_against_oracle_fixture(ggpu, synth):
+    """FP64 Sigma(tau) and HF K at the bench workload's per-pair dimensions
+    (nao = 52, NQ = 300, ns = 2, nts = 110) on a two-k mesh, against oracle
+    values committed by tests/golden/make_workload_fixture.py (inputs
+    regenerated from the same seeds): the production tile edges (M = 300,
+    N = 2704) and K-batches over 300 Q, at 1e-10 max-abs."""
+    f = np.load(os.path.join(ROOT, "tests", "golden", "fixtures", "gw_workload_nao52_nq300_nk2.npz"))
+    mesh = tuple(int(x) for x in f["mesh"])
+    nao, NQ, ns, beta = int(f["nao"]), int(f["NQ"]), int(f["ns"]), float(f["beta"])
+    sym = ggpu.mesh_symmetry_none(mesh)
+    tau, T_wt, T_tw = synth.load_ir(beta, float(f["lam"]))
+    G, D = synth.make_green(sym.nk, nao, ns, beta, tau, int(f["seed_g"]))
+    V = synth.make_integrals(sym.nk, NQ, nao, int(f["seed_v"]))
+    print(f"workload dimensions nk={sym.nk} nao={nao} NQ={NQ} ns={ns} nts={tau.size}", flush=True)
+    k = ggpu.GwKernel(nao, ns, NQ, T_wt, T_tw, sym, solver="cholesky")
+    k.load_integrals(V)
+    err_s = max_abs(k.solve(G)[f["tau_idx"]], f["sigma"])
+    h = ggpu.HfKernel(nao, ns, NQ, sym)
+    h.load_integrals(V)
+    err_k = max_abs(h.exchange(D), f["K"])
+    print(f"Sigma max-abs err {err_s:.2e} (max |Sigma| {float(f['sigma_abs_max']):.1f}), K err {err_k:.2e}")
+    assert err_s < TOL and err_k < TOL
+
+
 def test_gw_small_workload(ggpu, synth, oracle):
     """The reference's CPU-runnable workload: nk=2x2x2, nao=26, NQ=100, ns=1,
     beta=50, IR Lambda=1e3; plain FP64 GW iteration vs the oracle at 1e-10."""
