@@ -13,8 +13,7 @@ __device__ inline z64 cj(z64 a) { return z64{a.x, -a.y}; }
 __device__ inline z64 cmul(z64 a, z64 b) { return z64{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
 __device__ inline z64 csub(z64 a, z64 b) { return z64{a.x - b.x, a.y - b.y}; }
 
-__device__ inline c32 narrow(z64 v) { return c32{(float)v.x, (float)v.y}; }
-__device__ inline c32 narrow(c32 v) { return v; }
+template <class T> __device__ inline c32 narrow(T v) { return c32{(float)v.x, (float)v.y}; }
 template <class T> __device__ inline T cjt(T a) { return T{a.x, -a.y}; }
 template <class T> __device__ inline T scl(T a, double f) {
   using R = decltype(a.x);

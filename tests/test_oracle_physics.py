@@ -2,7 +2,6 @@
 symmetry-reduced vs unreduced calculation on the reference's symmetric
 fixture system, the Dyson residual, linearity and spin bookkeeping."""
 import numpy as np
-import pytest
 
 from conftest import max_abs
 
