@@ -69,6 +69,34 @@ __device__ inline int64_t xcd_logical(int64_t id, int64_t total) {
   return (xcd < rem) ? xcd * (per + 1) + slot : rem * (per + 1) + (xcd - rem) * per + slot;
 }
 
+// Logical tile -> (m-tile, n-tile, batch b0, batch b1).  Default order:
+// m-tile fastest, then n-tile, then batch.  With d.batch_major the batch
+// index runs fastest instead: chosen when every batch shares A (A's batch
+// strides are zero), so the workgroups reusing one A panel run back to back
+// and find it in L2 instead of re-reading A from memory once per batch.
+struct TileCoord { int mt, nt; int64_t b0, b1; };
+__device__ inline TileCoord tile_coord(const GemmDesc& d, int tiles_m, int tiles_n) {
+  int64_t lg = xcd_logical(blockIdx.x, gridDim.x);
+  TileCoord c;
+  if (d.batch_major) {
+    const int64_t nb = (int64_t)d.nb0 * d.nb1;
+    const int64_t b = lg % nb;
+    lg /= nb;
+    c.mt = (int)(lg % tiles_m);
+    c.nt = (int)(lg / tiles_m);
+    c.b1 = b % d.nb1;
+    c.b0 = b / d.nb1;
+  } else {
+    c.mt = (int)(lg % tiles_m);
+    lg /= tiles_m;
+    c.nt = (int)(lg % tiles_n);
+    lg /= tiles_n;
+    c.b1 = lg % d.nb1;
+    c.b0 = lg / d.nb1;
+  }
+  return c;
+}
+
 // T: arithmetic type of operands and MFMA; TO: complex element type of C.
 // AKC: A's k index is the contiguous one (else its m index).
 // BKC: B's k index is the contiguous one (else its n index).
@@ -89,12 +117,9 @@ __global__ __launch_bounds__(NTHREADS) __attribute__((amdgpu_waves_per_eu(WPE)))
   const int tid = threadIdx.x;
   const int tiles_m = (d.M + BM - 1) / BM;
   const int tiles_n = (d.N + BN - 1) / BN;
-  int64_t lg = xcd_logical(blockIdx.x, gridDim.x);
-  const int m0 = (int)(lg % tiles_m) * BM;
-  lg /= tiles_m;
-  const int n0 = (int)(lg % tiles_n) * BN;
-  lg /= tiles_n;
-  const int64_t b1 = lg % d.nb1, b0 = lg / d.nb1;
+  const TileCoord tc = tile_coord(d, tiles_m, tiles_n);
+  const int m0 = tc.mt * BM, n0 = tc.nt * BN;
+  const int64_t b1 = tc.b1, b0 = tc.b0;
 
   const CT* A = static_cast<const CT*>(d.A.ptr) + b0 * d.A.bs0 + (d.A.off1 ? d.A.off1[b1] : b1 * d.A.bs1);
   const CT* B = static_cast<const CT*>(d.B.ptr) + b0 * d.B.bs0 + (d.B.off1 ? d.B.off1[b1] : b1 * d.B.bs1);
@@ -272,12 +297,9 @@ __global__ __launch_bounds__(NTHREADS) void gemm_bf16x3_kernel(const GemmDesc d)
   const int tid = threadIdx.x;
   const int tiles_m = (d.M + BM - 1) / BM;
   const int tiles_n = (d.N + BN - 1) / BN;
-  int64_t lg = xcd_logical(blockIdx.x, gridDim.x);
-  const int m0 = (int)(lg % tiles_m) * BM;
-  lg /= tiles_m;
-  const int n0 = (int)(lg % tiles_n) * BN;
-  lg /= tiles_n;
-  const int64_t b1 = lg % d.nb1, b0 = lg / d.nb1;
+  const TileCoord tc = tile_coord(d, tiles_m, tiles_n);
+  const int m0 = tc.mt * BM, n0 = tc.nt * BN;
+  const int64_t b1 = tc.b1, b0 = tc.b0;
   const float2* A = static_cast<const float2*>(d.A.ptr) + b0 * d.A.bs0 + (d.A.off1 ? d.A.off1[b1] : b1 * d.A.bs1);
   const float2* B = static_cast<const float2*>(d.B.ptr) + b0 * d.B.bs0 + (d.B.off1 ? d.B.off1[b1] : b1 * d.B.bs1);
   TO* C = static_cast<TO*>(d.C.ptr) + b0 * d.C.bs0 + (d.C.off1 ? d.C.off1[b1] : b1 * d.C.bs1);
@@ -438,12 +460,9 @@ __global__ __launch_bounds__(NTHREADS) __attribute__((amdgpu_waves_per_eu(WPE)))
   const int tid = threadIdx.x;
   const int tiles_m = (d.M + TM - 1) / TM;
   const int tiles_n = (d.N + TN - 1) / TN;
-  int64_t lg = xcd_logical(blockIdx.x, gridDim.x);
-  const int m0 = (int)(lg % tiles_m) * TM;
-  lg /= tiles_m;
-  const int n0 = (int)(lg % tiles_n) * TN;
-  lg /= tiles_n;
-  const int64_t b1 = lg % d.nb1, b0 = lg / d.nb1;
+  const TileCoord tc = tile_coord(d, tiles_m, tiles_n);
+  const int m0 = tc.mt * TM, n0 = tc.nt * TN;
+  const int64_t b1 = tc.b1, b0 = tc.b0;
   const float2* A = static_cast<const float2*>(d.A.ptr) + b0 * d.A.bs0 + (d.A.off1 ? d.A.off1[b1] : b1 * d.A.bs1);
   const float2* B = static_cast<const float2*>(d.B.ptr) + b0 * d.B.bs0 + (d.B.off1 ? d.B.off1[b1] : b1 * d.B.bs1);
   TO* C = static_cast<TO*>(d.C.ptr) + b0 * d.C.bs0 + (d.C.off1 ? d.C.off1[b1] : b1 * d.C.bs1);
@@ -789,9 +808,11 @@ void set_gemm_profiler(GemmProfiler* p) { g_profiler = p; }
 void set_bf16_tiles(int mode) { g_bf16_tiles = mode; }
 void set_mfma_bk(int bk) { g_mfma_bk = bk; }
 
-void gemm(const GemmDesc& d, hipStream_t stream) {
-  gemm_check_bounds(d);
-  if (d.M == 0 || d.N == 0) return;
+void gemm(const GemmDesc& dd, hipStream_t stream) {
+  gemm_check_bounds(dd);
+  if (dd.M == 0 || dd.N == 0) return;
+  GemmDesc d = dd;
+  d.batch_major = (d.nb0 * (int64_t)d.nb1 > 1) && d.A.bs0 == 0 && d.A.bs1 == 0 && d.A.off1 == nullptr;
   if (g_profiler) g_profiler->before(d, stream);
   switch (d.compute) {
     case Compute::F64:

@@ -88,6 +88,7 @@ void set_mfma_bk(int variant);
 
 struct GemmDesc {
   int site = kSiteOther;
+  int batch_major = 0;   // set by gemm(): batch index fastest in the tile order
   int M = 0, N = 0, K = 0, KB = 1;
   int nb0 = 1, nb1 = 1;
   Operand A, B;
