@@ -120,6 +120,12 @@ def main():
     ap.add_argument("--no-hf", action="store_true", help="skip the HF build")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--cpu-pairs", type=int, default=2)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the built-in launch variants instead of autotuning each plan before the warmup")
+    ap.add_argument("--tune-rounds", type=int, default=2)
+    ap.add_argument("--graph", action="store_true",
+                    help="time the iteration as one captured HIP graph replay (needs --integrals device); "
+                         "per-site times then come from one profiled eager iteration before the timed steps")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one plain 1 GiB device-to-device copy kernel (scale for HBM counters)")
     a = ap.parse_args()
@@ -134,6 +140,8 @@ def main():
 
     w = WORKLOADS[a.workload]
     a.integrals = a.integrals or w.get("integrals", "host")
+    if a.graph and a.integrals != "device":
+        sys.exit("--graph replays a captured iteration and needs --integrals device")
     if "systems" in w:
         return run_batch(a, w, pkg, synth)
     sym = pkg.mesh_symmetry_none(w["mesh"])
@@ -163,12 +171,31 @@ def main():
         Sd = pkg.DeviceArray.empty_like(Gd)
         pkg.synchronize()
         log(f"[{mode}] inputs resident in HBM")
+        tune = None
+        if not a.no_autotune:
+            rep = kern.autotune_device(Gd.ptr, rounds=a.tune_rounds)
+            tune = {"wall_ms": round(rep["wall_ms"], 1),
+                    "chosen": {k: v["chosen"] for k, v in rep["sites"].items()},
+                    "candidate_ms": {k: {c: round(t, 3) for c, t in v["ms"].items()}
+                                     for k, v in rep["sites"].items()}}
+            log(f"[{mode}] autotuned in {rep['wall_ms'] / 1e3:.2f}s: {tune['chosen']}")
         for i in range(a.warmup):
             t = time.perf_counter()
             kern.solve_device(Gd.ptr, Sd.ptr)
             pkg.synchronize()
             log(f"[{mode}] warmup step {i}: {time.perf_counter() - t:.2f}s")
         kern.set_profiling(True)
+        if a.graph:
+            # per-site times from one profiled eager iteration, then the
+            # graph is captured (untimed) and replayed in the timed steps
+            kern.solve_device(Gd.ptr, Sd.ptr)
+            sites = kern.site_stats()
+            kern.set_profiling(False)
+            kern.set_graph(True)
+            t = time.perf_counter()
+            kern.solve_device(Gd.ptr, Sd.ptr)
+            pkg.synchronize()
+            log(f"[{mode}] graph captured and replayed once: {time.perf_counter() - t:.2f}s")
         pkg.synchronize()
         t0 = time.perf_counter()
         for i in range(a.steps):
@@ -177,8 +204,10 @@ def main():
             log(f"[{mode}] step {i} done")
         dt = (time.perf_counter() - t0) / a.steps
         kern.set_profiling(False)
-        runs[mode] = {"s": dt, "sites": kern.site_stats(), "phases": kern.timings(),
-                      "streamed": kern.streamed_bytes(), "generated": kern.generated_bytes()}
+        if not a.graph:
+            sites = kern.site_stats()
+        runs[mode] = {"s": dt, "sites": sites, "phases": kern.timings(),
+                      "streamed": kern.streamed_bytes(), "generated": kern.generated_bytes(), "autotune": tune}
         sigmas[mode] = Sd.numpy()
         Sd.free()
         # P0(tau) of the first IBZ q, for the per-mode P0 error
@@ -250,7 +279,8 @@ def main():
         "data": "synthetic (seeded random H(k) -> G(tau); Gaussian V, sigma 0.1)",
         "config": {"workload": a.workload, "nk": nk, "nao": nao, "NQ": NQ, "ns": ns, "nts": nts,
                    "nw_b": nw_b, "beta": w["beta"], "ir_lambda": w["lam"], "precision": head,
-                   "solver": a.solver, "integrals": a.integrals, "q_share": f"{q_rank}/{q_size}"},
+                   "solver": a.solver, "integrals": a.integrals, "q_share": f"{q_rank}/{q_size}",
+                   "graph": bool(a.graph)},
         "gw_iteration_s": round(r["s"], 3),
         "gw_flop_per_iteration": fl["total"],
         "modes": {m: {"gw_iteration_s": round(runs[m]["s"], 3), "tflops": round(runs[m]["tflops"], 2),
@@ -260,7 +290,8 @@ def main():
                       "p0_max_abs_err": runs[m]["p0_max_abs_err"],
                       "p0_rel_err": runs[m]["p0_rel_err"],
                       "phases_ms": {k: round(v, 1) for k, v in runs[m]["phases"].items()},
-                      "sites_ms": {k: round(v[0], 1) for k, v in runs[m]["sites"].items()}} for m in modes},
+                      "sites_ms": {k: round(v[0], 1) for k, v in runs[m]["sites"].items()},
+                      "autotune": runs[m]["autotune"]} for m in modes},
         "sigma_abs_max": scale,
         "integrals": {"residence": a.integrals,
                       "h2d_gb_per_iteration": round(r["streamed"] / 1e9, 2),
@@ -319,6 +350,11 @@ def run_batch(a, w, pkg, synth):
             kerns.append(k)
             Sd.append(pkg.DeviceArray.empty_like(Gd[i]))
         gp, sp = [g.ptr for g in Gd], [x.ptr for x in Sd]
+        tune_ms = 0.0
+        if not a.no_autotune:
+            for i, k in enumerate(kerns):
+                tune_ms += k.autotune_device(gp[i], rounds=a.tune_rounds)["wall_ms"]
+            log(f"[{mode}] {len(kerns)} plans autotuned in {tune_ms / 1e3:.2f}s")
         for _ in range(a.warmup):
             pkg.iterate_many(kerns, gp, sp)
         t0 = time.perf_counter()
@@ -331,7 +367,8 @@ def run_batch(a, w, pkg, synth):
             k.solve_device(gp[i], sp[i])
         sequential = time.perf_counter() - t0
         log(f"[{mode}] batched {batched:.2f}s, one after another {sequential:.2f}s")
-        res[mode] = {"batched_s": batched, "sequential_s": sequential, "tflops": fl / batched / 1e12}
+        res[mode] = {"batched_s": batched, "sequential_s": sequential, "tflops": fl / batched / 1e12,
+                     "autotune_ms": tune_ms}
         sig[mode] = [x.numpy() for x in Sd]
         for x in Sd:
             x.free()
@@ -359,7 +396,8 @@ def run_batch(a, w, pkg, synth):
                       "batching_speedup": round(v["sequential_s"] / v["batched_s"], 3),
                       "tflops": round(v["tflops"], 2),
                       "speedup_vs_fp64": round(res["fp64"]["batched_s"] / v["batched_s"], 3),
-                      "sigma_max_abs_err": v["sigma_max_abs_err"]} for m, v in res.items()},
+                      "sigma_max_abs_err": v["sigma_max_abs_err"],
+                      "autotune_ms": round(v["autotune_ms"], 1)} for m, v in res.items()},
     }
     print(json.dumps(out))
 

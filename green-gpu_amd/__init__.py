@@ -72,6 +72,13 @@ def lib():
         L.ggpu_gw_last_timings.argtypes = [vp, pd]
         L.ggpu_gw_set_profiling.argtypes = [vp, i32]
         L.ggpu_gw_site_stats.argtypes = [vp, i32, pd, pd, C.POINTER(C.c_int64)]
+        L.ggpu_gw_set_graph.argtypes = [vp, i32]
+        L.ggpu_gw_graph_captures.argtypes = [vp, C.POINTER(C.c_int)]
+        L.ggpu_gw_autotune.argtypes = [vp, pd, i32, i32]
+        L.ggpu_gw_autotune_device.argtypes = [vp, vp, i32, i32]
+        L.ggpu_gw_autotune_report.argtypes = [vp, i32, C.POINTER(C.c_int), C.POINTER(C.c_int), pd, pd]
+        L.ggpu_gw_get_variants.argtypes = [vp, C.POINTER(C.c_int)]
+        L.ggpu_gw_set_variants.argtypes = [vp, C.POINTER(C.c_int)]
         L.ggpu_gw_plan_destroy.argtypes = [vp]
         L.ggpu_gw_plan_destroy.restype = None
         _lib = L
@@ -88,7 +95,16 @@ EXPORTED = ["ggpu_last_error", "ggpu_device_count", "ggpu_device_memory",
             "ggpu_gw_load_integrals", "ggpu_gw_synth_integrals",
             "ggpu_gw_iterate", "ggpu_gw_iterate_device", "ggpu_gw_iterate_many", "ggpu_gw_polarization",
             "ggpu_gw_last_timings", "ggpu_gw_set_profiling", "ggpu_gw_site_stats",
+            "ggpu_gw_set_graph", "ggpu_gw_graph_captures",
+            "ggpu_gw_autotune", "ggpu_gw_autotune_device", "ggpu_gw_autotune_report",
+            "ggpu_gw_get_variants", "ggpu_gw_set_variants",
             "ggpu_gw_plan_destroy"]
+
+# Launch variants of the contraction GEMMs (ids as the library numbers them):
+# FP64 / FP32 LDS k-depth and waves-per-SIMD register allocation, split-bf16
+# workgroup tiles.
+VARIANTS_MFMA = {8: "bk8", 16: "bk16", 83: "bk8_w3", 84: "bk8_w4"}
+VARIANTS_BF16 = {0: "64x64", 1: "128x64", 4: "64x128", 5: "160x64", 6: "96x64_w3", 7: "64x96_w3", 8: "64x64_w3"}
 
 SITES = ["other", "p0_gv", "p0_gvh", "p0_contract", "sig_vg", "sig_py", "sig_contract",
          "ir", "dyson", "hf", "rotate"]
@@ -414,6 +430,7 @@ class GwKernel:
                                          C.byref(self._csym), _dptr(T_wt), _dptr(T_tw),
                                          PRECISION[precision], SOLVER[solver.lower()], int(work_bytes)))
         self._h = h
+        self.precision = precision
         _check(lib().ggpu_gw_set_integral_residence(self._h, RESIDENCE[integrals]))
 
     @property
@@ -481,6 +498,58 @@ class GwKernel:
             if n.value:
                 out[name] = (ms.value, fl.value, n.value)
         return out
+
+    def set_graph(self, on=True):
+        """Capture the iteration into a HIP graph on the next solve and replay
+        it afterwards (integrals resident in HBM only)."""
+        _check(lib().ggpu_gw_set_graph(self._h, 1 if on else 0))
+
+    def graph_captures(self):
+        n = C.c_int()
+        _check(lib().ggpu_gw_graph_captures(self._h, C.byref(n)))
+        return n.value
+
+    def variant_name(self, v):
+        """Readable name of a launch variant id in this plan's precision."""
+        names = VARIANTS_BF16 if self.precision == "bf16x3" else VARIANTS_MFMA
+        return "default" if v == -1 else names.get(v, str(v))
+
+    def autotune(self, G, rounds=2, batches=1):
+        """Time every launch variant of the six contraction sites on the first
+        q of this plan's share and keep the fastest per site (results are
+        unchanged).  Returns {site: {"chosen": name, "ms": {name: ms}}}."""
+        G = _c128(G, self.shape)
+        _check(lib().ggpu_gw_autotune(self._h, _dptr(G), int(rounds), int(batches)))
+        return self.autotune_report()
+
+    def autotune_device(self, G_ptr, rounds=2, batches=1):
+        _check(lib().ggpu_gw_autotune_device(self._h, C.c_void_p(G_ptr), int(rounds), int(batches)))
+        return self.autotune_report()
+
+    def autotune_report(self):
+        n = C.c_int()
+        cap = 16
+        ids = (C.c_int * cap)()
+        ms = np.full(cap * len(SITES), -1.0)
+        wall = C.c_double()
+        _check(lib().ggpu_gw_autotune_report(self._h, cap, C.byref(n), ids, _dptr(ms), C.byref(wall)))
+        chosen = self.get_variants()
+        out = {"wall_ms": wall.value, "sites": {}}
+        for si, name in enumerate(SITES):
+            times = {self.variant_name(ids[c]): float(ms[c * len(SITES) + si])
+                     for c in range(min(n.value, cap)) if ms[c * len(SITES) + si] >= 0}
+            if times:
+                out["sites"][name] = {"chosen": self.variant_name(chosen[si]), "ms": times}
+        return out
+
+    def get_variants(self):
+        v = (C.c_int * len(SITES))()
+        _check(lib().ggpu_gw_get_variants(self._h, v))
+        return list(v)
+
+    def set_variants(self, variants):
+        v = (C.c_int * len(SITES))(*[int(x) for x in variants])
+        _check(lib().ggpu_gw_set_variants(self._h, v))
 
     def close(self):
         if getattr(self, "_h", None):

@@ -3,6 +3,7 @@
 #include "../../include/green_gpu_amd.h"
 #include "engine.hpp"
 
+#include <algorithm>
 #include <exception>
 #include <new>
 #include <string>
@@ -142,6 +143,7 @@ int ggpu_gw_set_integral_residence(ggpu_gw_plan* plan, int residence) {
   return guarded([&] {
     ggpu::require(plan && plan->impl, "NULL plan");
     auto* p = plan->impl;
+    p->invalidate_graph();
     p->store.set_residence(residence, p->kb, (int64_t)p->NQ * p->nao * p->nao);
   });
 }
@@ -166,6 +168,7 @@ int ggpu_gw_set_q_share(ggpu_gw_plan* plan, int rank, int size) {
     ggpu::require(size >= 1 && rank >= 0 && rank < size, "q share: need 0 <= rank < size");
     plan->impl->share_rank = rank;
     plan->impl->share_size = size;
+    plan->impl->invalidate_graph();
   });
 }
 
@@ -173,6 +176,7 @@ int ggpu_gw_load_integrals(ggpu_gw_plan* plan, const double* V_stored) {
   return guarded([&] {
     ggpu::require(plan && plan->impl, "NULL plan");
     auto* p = plan->impl;
+    p->invalidate_graph();
     p->store.load_host(V_stored, p->sym.num_stored * (int64_t)p->NQ * p->nao * p->nao, p->stream);
   });
 }
@@ -181,6 +185,7 @@ int ggpu_gw_synth_integrals(ggpu_gw_plan* plan, uint64_t seed, double sigma) {
   return guarded([&] {
     ggpu::require(plan && plan->impl, "NULL plan");
     auto* p = plan->impl;
+    p->invalidate_graph();
     p->store.synth(p->sym, p->NQ, p->nao, seed, sigma, p->stream);
   });
 }
@@ -263,6 +268,66 @@ int ggpu_gw_site_stats(ggpu_gw_plan* plan, int site, double* ms, double* flops, 
     *ms = plan->impl->prof.ms[site];
     *flops = plan->impl->prof.flops[site];
     *launches = plan->impl->prof.launches[site];
+  });
+}
+
+int ggpu_gw_set_graph(ggpu_gw_plan* plan, int enable) {
+  return guarded([&] {
+    ggpu::require(plan && plan->impl, "NULL plan");
+    plan->impl->invalidate_graph();
+    plan->impl->use_graph = enable != 0;
+  });
+}
+
+int ggpu_gw_graph_captures(ggpu_gw_plan* plan, int* captures) {
+  return guarded([&] {
+    ggpu::require(plan && plan->impl && captures, "NULL argument");
+    *captures = plan->impl->graph_captures;
+  });
+}
+
+int ggpu_gw_autotune(ggpu_gw_plan* plan, const double* G, int rounds, int batches) {
+  return guarded([&] {
+    ggpu::require(plan && plan->impl && G, "NULL argument");
+    ggpu::GwPlan& p = *plan->impl;
+    GGPU_HIP(hipMemcpyAsync(p.G_host_dev.get(), G, p.G_host_dev.bytes(), hipMemcpyHostToDevice, p.stream));
+    p.autotune(p.G_host_dev.get(), rounds, batches);
+  });
+}
+
+int ggpu_gw_autotune_device(ggpu_gw_plan* plan, const void* G_device, int rounds, int batches) {
+  return guarded([&] {
+    ggpu::require(plan && plan->impl && G_device, "NULL argument");
+    plan->impl->autotune(static_cast<const ggpu::z64*>(G_device), rounds, batches);
+  });
+}
+
+int ggpu_gw_autotune_report(ggpu_gw_plan* plan, int cap, int* n, int* variants, double* ms, double* wall_ms) {
+  return guarded([&] {
+    ggpu::require(plan && plan->impl && n, "NULL argument");
+    const ggpu::GwPlan& p = *plan->impl;
+    *n = (int)p.tune_cand.size();
+    const int m = std::min(cap, *n);
+    for (int c = 0; c < m; ++c) {
+      if (variants) variants[c] = p.tune_cand[c];
+      if (ms)
+        for (int s = 0; s < ggpu::kNumSites; ++s) ms[(size_t)c * ggpu::kNumSites + s] = p.tune_ms[c][s];
+    }
+    if (wall_ms) *wall_ms = p.tune_wall_ms;
+  });
+}
+
+int ggpu_gw_get_variants(ggpu_gw_plan* plan, int* variants) {
+  return guarded([&] {
+    ggpu::require(plan && plan->impl && variants, "NULL argument");
+    std::copy(plan->impl->variants, plan->impl->variants + ggpu::kNumSites, variants);
+  });
+}
+
+int ggpu_gw_set_variants(ggpu_gw_plan* plan, const int* variants) {
+  return guarded([&] {
+    ggpu::require(plan && plan->impl, "NULL plan");
+    plan->impl->set_variants(variants);
   });
 }
 

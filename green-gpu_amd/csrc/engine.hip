@@ -504,6 +504,7 @@ GwPlan::GwPlan(int nao_, int nso_, int ns_, int NQ_, int nts_, int nw_b_, const 
   require(nao > 0 && NQ > 0, "gw: nao and NQ must be positive");
   require(nso == nao || nso == 2 * nao, "gw: nso must be nao or 2 nao");
   require(ns == 1 || ns == 2, "gw: ns must be 1 or 2");
+  std::fill(variants, variants + kNumSites, -1);
   x2c = nso == 2 * nao;
   require(!x2c || ns == 1, "gw: a two-component (nso = 2 nao) plan needs ns = 1");
   if (x2c) ns = 4;   // pseudo-spins aa, bb, ab, ba
@@ -656,6 +657,7 @@ GwPlan::GwPlan(int nao_, int nso_, int ns_, int NQ_, int nts_, int nw_b_, const 
 
 GwPlan::~GwPlan() {
   if (stream) (void)hipStreamSynchronize(stream);
+  invalidate_graph();
   for (auto& e : ev) (void)hipEventDestroy(e);
   for (auto& e : qev) (void)hipEventDestroy(e);
   if (stream) (void)hipStreamDestroy(stream);
@@ -705,9 +707,10 @@ void GwPlan::p0_for_q(int iq) {
   const void* Gr = lp ? (const void*)Grev32.get() : (const void*)Grev.get();
   const void* Gp = lp ? (const void*)Gf32.get() : (const void*)Gf.get();
   const Span gsp{Gp, Gf.size()}, grsp{Gr, Grev.size()};
-  for (int i0 = 0; i0 < npairs; i0 += kb) {
+  const int p0_end = batch_limit > 0 ? std::min(npairs, batch_limit * kb) : npairs;
+  for (int i0 = 0; i0 < p0_end; i0 += kb) {
     const int n = std::min(kb, npairs - i0);
-    unfold_pairs(store, tab, i0, n, lp ? bidx_off.get() : bidx_off.get(), Vdir, lp, NQ, nao, stream);
+    unfold_pairs(store, tab, i0, n, bidx_off.get(), Vdir, lp, NQ, nao, stream);
     BlockCopy tr;
     tr.in = Vdir; tr.out = VpmQ; tr.in_f32 = lp; tr.out_f32 = lp;
     tr.rows = (int)n2; tr.cols = NQ; tr.in_ld = n2; tr.out_ld = NQ;
@@ -878,7 +881,8 @@ void GwPlan::sigma_for_q(int iq) {
     }
     const Span psp{Pq, Pt.size()};
     const PairTable& tab = sig_tables[iq][j];
-    for (int i0 = 0; i0 < sym.ink; i0 += kb) {
+    const int sig_end = batch_limit > 0 ? std::min(sym.ink, batch_limit * kb) : sym.ink;
+    for (int i0 = 0; i0 < sig_end; i0 += kb) {
       const int n = std::min(kb, sym.ink - i0);
       unfold_pairs(store, tab, i0, n, bidx_off.get(), Vdir, lp, NQ, nao, stream);
       for (int b = 0; b < n; ++b) {
@@ -926,42 +930,173 @@ void GwPlan::check_info() {
                                                std::to_string(w) + ", column " + std::to_string(h[w]) + ")");
 }
 
-void GwPlan::iterate_device(const z64* G_dev, z64* S_dev) {
-  require(store.loaded(), "gw: integrals not loaded", kErrState);
-  const int64_t n2 = (int64_t)nao * nao;
-  while (qev.size() < (size_t)4 * sym.inq) {
-    hipEvent_t e;
-    GGPU_HIP(hipEventCreate(&e));
-    qev.push_back(e);
+// Install a GEMM profiler / per-site launch variants for the calling thread
+// while a plan runs.
+struct ProfilerGuard {
+  explicit ProfilerGuard(GemmProfiler* p) { set_gemm_profiler(p); }
+  ~ProfilerGuard() { set_gemm_profiler(nullptr); }
+};
+struct VariantGuard {
+  explicit VariantGuard(const int* v) { set_site_variants(v); }
+  ~VariantGuard() { set_site_variants(nullptr); }
+};
+
+// Sites whose launch variant the autotuner picks: the six contraction sites,
+// all in the plan's arithmetic (IR, Dyson and rotations stay FP64).
+static bool tunable_site(int s) { return s >= kSiteP0X1 && s <= kSiteSigma; }
+
+void GwPlan::set_variants(const int* v) {
+  require(v != nullptr, "gw: NULL variant table");
+  for (int s = 0; s < kNumSites; ++s) {
+    require(v[s] == -1 || (tunable_site(s) && gemm_variant_valid(compute, v[s])),
+            "gw: launch variant " + std::to_string(v[s]) + " not available for call site " + std::to_string(s) +
+                " in this plan's precision");
   }
-  struct Guard {
-    bool on;
-    explicit Guard(GemmProfiler* p) : on(p != nullptr) { set_gemm_profiler(p); }
-    ~Guard() { set_gemm_profiler(nullptr); }
-  } guard(profiling ? &prof : nullptr);
-  if (profiling) prof.reset();
-  const double streamed0 = store.bytes_streamed(), generated0 = store.bytes_generated();
-  GGPU_HIP(hipEventRecord(ev[0], stream));
+  std::copy(v, v + kNumSites, variants);
+  invalidate_graph();
+}
+
+void GwPlan::autotune(const z64* G_dev, int rounds, int batches) {
+  require(store.loaded(), "gw: integrals not loaded", kErrState);
+  require(G_dev != nullptr, "gw: NULL G");
+  require(rounds >= 1 && rounds <= 8, "gw: autotune rounds must be 1 .. 8");
+  require(batches >= 1, "gw: autotune needs at least one pair batch");
+  int cand[16];
+  const int nc = gemm_variant_candidates(compute, cand, 16);
+  tune_cand.assign(cand, cand + nc);
+  tune_ms.assign(nc, std::vector<double>(kNumSites, -1.0));
+  tune_wall_ms = 0;
+  invalidate_graph();
+  const int iq = share_rank;       // first q of this device's share
+  if (iq >= sym.inq) return;
+  hipEvent_t e0 = ev[0], e1 = ev[6];
+  GGPU_HIP(hipEventRecord(e0, stream));
+  SiteProfiler tp;
+  ProfilerGuard guard(&tp);
+  struct Limit {
+    int& r;
+    Limit(int& x, int v) : r(x) { r = v; }
+    ~Limit() { r = 0; }
+  } limit(batch_limit, batches);
   stage_G(G_dev);
-  GGPU_HIP(hipEventRecord(ev[1], stream));
+  int trial[kNumSites];
+  for (int r = 0; r < rounds; ++r)
+    for (int c = 0; c < nc; ++c) {
+      for (int s = 0; s < kNumSites; ++s) trial[s] = tunable_site(s) ? cand[c] : -1;
+      VariantGuard vg(trial);
+      tp.reset();
+      p0_for_q(iq);
+      sigma_for_q(iq);
+      GGPU_HIP(hipStreamSynchronize(stream));
+      tp.collect();
+      for (int s = 0; s < kNumSites; ++s) {
+        if (!tunable_site(s) || tp.launches[s] == 0) continue;
+        double& best = tune_ms[c][s];
+        best = best < 0 ? tp.ms[s] : std::min(best, tp.ms[s]);
+      }
+    }
+  GGPU_HIP(hipEventRecord(e1, stream));
+  GGPU_HIP(hipStreamSynchronize(stream));
+  float w = 0;
+  GGPU_HIP(hipEventElapsedTime(&w, e0, e1));
+  tune_wall_ms = w;
+  for (int s = 0; s < kNumSites; ++s) {
+    variants[s] = -1;
+    if (!tunable_site(s)) continue;
+    double best = -1;
+    for (int c = 0; c < nc; ++c)
+      if (tune_ms[c][s] >= 0 && (best < 0 || tune_ms[c][s] < best)) { best = tune_ms[c][s]; variants[s] = cand[c]; }
+  }
+}
+
+void GwPlan::enqueue_iteration(const z64* G_dev, z64* S_dev, bool timed) {
+  const int64_t n2 = (int64_t)nao * nao;
+  if (timed) GGPU_HIP(hipEventRecord(ev[0], stream));
+  stage_G(G_dev);
+  if (timed) GGPU_HIP(hipEventRecord(ev[1], stream));
   zero(Sig_acc.get(), Sig_acc.size(), stream);
   zero_int(info.get(), nw_b, stream);
   for (int iq = share_rank; iq < sym.inq; iq += share_size) {
-    GGPU_HIP(hipEventRecord(qev[4 * iq + 0], stream));
+    if (timed) GGPU_HIP(hipEventRecord(qev[4 * iq + 0], stream));
     p0_for_q(iq);
-    GGPU_HIP(hipEventRecord(qev[4 * iq + 1], stream));
+    if (timed) GGPU_HIP(hipEventRecord(qev[4 * iq + 1], stream));
     dyson_for_q();
-    GGPU_HIP(hipEventRecord(qev[4 * iq + 2], stream));
+    if (timed) GGPU_HIP(hipEventRecord(qev[4 * iq + 2], stream));
     sigma_for_q(iq);
-    GGPU_HIP(hipEventRecord(qev[4 * iq + 3], stream));
+    if (timed) GGPU_HIP(hipEventRecord(qev[4 * iq + 3], stream));
   }
-  GGPU_HIP(hipEventRecord(ev[5], stream));
+  if (timed) GGPU_HIP(hipEventRecord(ev[5], stream));
   if (x2c)
     spinor_copy(x2c_merge, Sig_acc.get(), Sig_acc.size(), nao, n2, S_dev, Sig_host_dev.size(), nso,
                 (int64_t)sym.ink * nso * nso, nao, nts, 1.0 / sym.nk, stream);
   else
     slab_gather(Sig_acc.get(), sig_src.get(), S_dev, sig_dst.get(), (int)sig_src.size(), n2, 1.0 / sym.nk, stream);
-  GGPU_HIP(hipEventRecord(ev[6], stream));
+  if (timed) GGPU_HIP(hipEventRecord(ev[6], stream));
+}
+
+void GwPlan::invalidate_graph() {
+  if (gexec) (void)hipGraphExecDestroy(gexec);
+  gexec = nullptr;
+  graph_G = graph_S = nullptr;
+}
+
+// Record the launches of one iteration into a graph (nothing runs) and
+// instantiate it.  Every buffer is zeroed by a kernel, not a memset node.
+void GwPlan::capture(const z64* G_dev, z64* S_dev) {
+  invalidate_graph();
+  GGPU_HIP(hipStreamSynchronize(stream));
+  GGPU_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+  hipGraph_t graph = nullptr;
+  try {
+    enqueue_iteration(G_dev, S_dev, false);
+  } catch (...) {
+    (void)hipStreamEndCapture(stream, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    throw;
+  }
+  GGPU_HIP(hipStreamEndCapture(stream, &graph));
+  const hipError_t e = hipGraphInstantiateWithFlags(&gexec, graph, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) {
+    gexec = nullptr;
+    GGPU_HIP(e);
+  }
+  graph_G = G_dev;
+  graph_S = S_dev;
+  ++graph_captures;
+}
+
+void GwPlan::iterate_device(const z64* G_dev, z64* S_dev) {
+  require(store.loaded(), "gw: integrals not loaded", kErrState);
+  while (qev.size() < (size_t)4 * sym.inq) {
+    hipEvent_t e;
+    GGPU_HIP(hipEventCreate(&e));
+    qev.push_back(e);
+  }
+  if (use_graph) {
+    require(!store.staged(), "gw: graph replay needs the integrals resident in HBM", kErrState);
+    require(!profiling, "gw: per-site profiling does not apply to graph replay", kErrState);
+    if (!gexec || graph_G != G_dev || graph_S != S_dev) {
+      VariantGuard vguard(variants);
+      capture(G_dev, S_dev);
+    }
+    GGPU_HIP(hipEventRecord(ev[0], stream));
+    GGPU_HIP(hipGraphLaunch(gexec, stream));
+    GGPU_HIP(hipEventRecord(ev[6], stream));
+    GGPU_HIP(hipStreamSynchronize(stream));
+    float tot = 0;
+    GGPU_HIP(hipEventElapsedTime(&tot, ev[0], ev[6]));
+    last_ms[0] = tot;
+    for (int i = 1; i < 6; ++i) last_ms[i] = 0;
+    streamed_last = generated_last = 0;
+    check_info();
+    return;
+  }
+  ProfilerGuard guard(profiling ? &prof : nullptr);
+  VariantGuard vguard(variants);
+  if (profiling) prof.reset();
+  const double streamed0 = store.bytes_streamed(), generated0 = store.bytes_generated();
+  enqueue_iteration(G_dev, S_dev, true);
   GGPU_HIP(hipStreamSynchronize(stream));
   double t_p0 = 0, t_dy = 0, t_sig = 0;
   for (int iq = share_rank; iq < sym.inq; iq += share_size) {
@@ -996,6 +1131,7 @@ void GwPlan::polarization(const double* G, int iq, int which, double* out) {
   require(which >= 0 && which <= 2, "gw: which must be 0, 1 or 2");
   require(G && out, "gw: NULL pointer");
   const int64_t Q2 = (int64_t)NQ * NQ;
+  VariantGuard vguard(variants);
   GGPU_HIP(hipMemcpyAsync(G_host_dev.get(), G, G_host_dev.bytes(), hipMemcpyHostToDevice, stream));
   stage_G(G_host_dev.get());
   zero_int(info.get(), nw_b, stream);

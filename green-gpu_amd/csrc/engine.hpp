@@ -184,6 +184,24 @@ struct GwPlan {
   bool profiling = false;
   SiteProfiler prof;
   std::vector<hipEvent_t> qev;          // per-q phase events
+  // Launch variant per GEMM call site (-1: the built-in choice), picked by
+  // autotune() or set by the caller; installed around the plan's launches.
+  int variants[kNumSites];
+  // Autotuner record: candidates and the per-site time (ms) of each, the best
+  // over the rounds (negative: the site did not run); wall time of the tuning.
+  std::vector<int> tune_cand;
+  std::vector<std::vector<double>> tune_ms;   // [candidate][site]
+  double tune_wall_ms = 0;
+  int batch_limit = 0;                  // pair batches per q (and star member); 0: all
+  // Graph replay: the iteration's launches captured once into a HIP graph
+  // (for the G / Sigma addresses, variants and q share of the capture) and
+  // replayed by every later iterate; needs the integrals resident in HBM.
+  bool use_graph = false;
+  hipGraphExec_t gexec = nullptr;
+  const void* graph_G = nullptr;
+  const void* graph_S = nullptr;
+  int graph_captures = 0;               // captures made (a new one after every change)
+  void invalidate_graph();
 
   SpinorTables x2c_split, x2c_merge;   // G spinor -> pseudo-spin blocks, Sigma back
 
@@ -195,9 +213,16 @@ struct GwPlan {
   void iterate_device(const z64* G_dev_hostlayout, z64* Sigma_dev_hostlayout);
   void iterate(const double* G, double* Sigma);
   void polarization(const double* G, int iq, int which, double* out);
+  // Time every launch variant of the contraction call sites (P0 and Sigma
+  // chains) on the first q of this plan's share, `batches` pair batches per
+  // chain, best of `rounds`, and keep the fastest per site.
+  void autotune(const z64* G_dev_hostlayout, int rounds, int batches);
+  void set_variants(const int* v);
 
  private:
   void stage_G(const z64* G_dev_hostlayout);
+  void enqueue_iteration(const z64* G_dev_hostlayout, z64* Sigma_dev_hostlayout, bool timed);
+  void capture(const z64* G_dev_hostlayout, z64* Sigma_dev_hostlayout);
   void p0_for_q(int iq);
   void dyson_for_q();
   void sigma_for_q(int iq);

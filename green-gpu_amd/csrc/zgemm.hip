@@ -630,8 +630,8 @@ void launch_bf16x3_big(const GemmDesc& d, hipStream_t stream) {
 inline double tile_use(int n, int t) { return (double)n / (double)(((n + t - 1) / t) * t); }
 
 template <class TO, int SITE>
-void launch_bf16x3_site(const GemmDesc& d, hipStream_t stream) {
-  int mode = g_bf16_tiles;
+void launch_bf16x3_site(const GemmDesc& d, hipStream_t stream, int variant) {
+  int mode = g_bf16_tiles >= 0 ? g_bf16_tiles : variant;
   if (mode < 0) {
     // Two or three waves per SIMD (6- to 10-block wave tiles) beat bigger
     // tiles.  Large shapes take the tile measured fastest for their call site
@@ -683,15 +683,15 @@ void launch_bf16x3_site(const GemmDesc& d, hipStream_t stream) {
 }
 
 template <class TO>
-void launch_bf16x3(const GemmDesc& d, hipStream_t stream) {
+void launch_bf16x3(const GemmDesc& d, hipStream_t stream, int v) {
   switch (d.site) {
-    case kSiteP0X1: launch_bf16x3_site<TO, kSiteP0X1>(d, stream); break;
-    case kSiteP0X2: launch_bf16x3_site<TO, kSiteP0X2>(d, stream); break;
-    case kSiteP0: launch_bf16x3_site<TO, kSiteP0>(d, stream); break;
-    case kSiteSigY1: launch_bf16x3_site<TO, kSiteSigY1>(d, stream); break;
-    case kSiteSigY2: launch_bf16x3_site<TO, kSiteSigY2>(d, stream); break;
-    case kSiteSigma: launch_bf16x3_site<TO, kSiteSigma>(d, stream); break;
-    default: launch_bf16x3_site<TO, kSiteOther>(d, stream); break;
+    case kSiteP0X1: launch_bf16x3_site<TO, kSiteP0X1>(d, stream, v); break;
+    case kSiteP0X2: launch_bf16x3_site<TO, kSiteP0X2>(d, stream, v); break;
+    case kSiteP0: launch_bf16x3_site<TO, kSiteP0>(d, stream, v); break;
+    case kSiteSigY1: launch_bf16x3_site<TO, kSiteSigY1>(d, stream, v); break;
+    case kSiteSigY2: launch_bf16x3_site<TO, kSiteSigY2>(d, stream, v); break;
+    case kSiteSigma: launch_bf16x3_site<TO, kSiteSigma>(d, stream, v); break;
+    default: launch_bf16x3_site<TO, kSiteOther>(d, stream, v); break;
   }
 }
 
@@ -715,7 +715,7 @@ void launch_site_bk(const GemmDesc& d, hipStream_t stream) {
 int g_mfma_bk = 0;
 
 template <class T, class TO, bool GAUSS, int SITE>
-void launch_site(const GemmDesc& d, hipStream_t stream) {
+void launch_site(const GemmDesc& d, hipStream_t stream, int variant) {
   // Variants measured on MI355X (nao = 52, NQ = 300, nts = 110; medians of
   // interleaved rounds): staging 8 k per LDS step is faster than 16 or equal
   // at every call site, and FP64 register allocation for three waves per
@@ -724,7 +724,7 @@ void launch_site(const GemmDesc& d, hipStream_t stream) {
   // All variants give the same sums bit for bit.
   constexpr bool f64 = std::is_same<T, double>::value;
   constexpr int pref = f64 ? 83 : 8;
-  switch (g_mfma_bk ? g_mfma_bk : pref) {
+  switch (g_mfma_bk ? g_mfma_bk : (variant > 0 ? variant : pref)) {
     case 8: return launch_site_bk<T, TO, GAUSS, SITE, 8, 1>(d, stream);
     case 83: return launch_site_bk<T, TO, GAUSS, SITE, 8, 3>(d, stream);
     case 84: return launch_site_bk<T, TO, GAUSS, SITE, 8, 4>(d, stream);
@@ -736,20 +736,21 @@ void launch_site(const GemmDesc& d, hipStream_t stream) {
 // (the Gauss recombination t3 - t1 - t2 loses relative accuracy when the
 // result is small against |A||B|, which FP32 cannot afford).
 template <class T, class TO>
-void launch_typed(const GemmDesc& d, hipStream_t stream) {
+void launch_typed(const GemmDesc& d, hipStream_t stream, int v) {
   constexpr bool G = std::is_same<T, double>::value;
   switch (d.site) {
-    case kSiteP0X1: launch_site<T, TO, G, kSiteP0X1>(d, stream); break;
-    case kSiteP0X2: launch_site<T, TO, G, kSiteP0X2>(d, stream); break;
-    case kSiteP0: launch_site<T, TO, G, kSiteP0>(d, stream); break;
-    case kSiteSigY1: launch_site<T, TO, G, kSiteSigY1>(d, stream); break;
-    case kSiteSigY2: launch_site<T, TO, G, kSiteSigY2>(d, stream); break;
-    case kSiteSigma: launch_site<T, TO, G, kSiteSigma>(d, stream); break;
-    default: launch_site<T, TO, G, kSiteOther>(d, stream); break;
+    case kSiteP0X1: launch_site<T, TO, G, kSiteP0X1>(d, stream, v); break;
+    case kSiteP0X2: launch_site<T, TO, G, kSiteP0X2>(d, stream, v); break;
+    case kSiteP0: launch_site<T, TO, G, kSiteP0>(d, stream, v); break;
+    case kSiteSigY1: launch_site<T, TO, G, kSiteSigY1>(d, stream, v); break;
+    case kSiteSigY2: launch_site<T, TO, G, kSiteSigY2>(d, stream, v); break;
+    case kSiteSigma: launch_site<T, TO, G, kSiteSigma>(d, stream, v); break;
+    default: launch_site<T, TO, G, kSiteOther>(d, stream, v); break;
   }
 }
 
 thread_local GemmProfiler* g_profiler = nullptr;
+thread_local const int* g_site_variants = nullptr;   // [kNumSites], -1: built-in choice
 
 // Range of element offsets (relative to ptr) touched by a 2-D view with the
 // given extents and strides, over all batch indices.
@@ -805,6 +806,25 @@ void gemm_check_bounds(const GemmDesc& d) {
 }
 
 void set_gemm_profiler(GemmProfiler* p) { g_profiler = p; }
+void set_site_variants(const int* table) { g_site_variants = table; }
+
+int gemm_variant_candidates(Compute c, int* out, int cap) {
+  static const int f64[] = {83, 8, 16};
+  static const int f32[] = {8, 16, 83};
+  static const int bf[] = {kBf16Tile128x64, kBf16Tile64x128, kBf16Tile160x64, kBf16Tile96x64W3,
+                           kBf16Tile64x96W3, kBf16Tile64x64W3, kBf16Tile64x64};
+  const int* v = c == Compute::F64 ? f64 : c == Compute::F32 ? f32 : bf;
+  const int n = c == Compute::F64 ? 3 : c == Compute::F32 ? 3 : 7;
+  for (int i = 0; i < n && i < cap; ++i) out[i] = v[i];
+  return n;
+}
+
+bool gemm_variant_valid(Compute c, int v) {
+  if (v == -1) return true;
+  int cand[8];
+  const int n = gemm_variant_candidates(c, cand, 8);
+  return std::find(cand, cand + n, v) != cand + n;
+}
 void set_bf16_tiles(int mode) { g_bf16_tiles = mode; }
 void set_mfma_bk(int bk) { g_mfma_bk = bk; }
 
@@ -813,19 +833,20 @@ void gemm(const GemmDesc& dd, hipStream_t stream) {
   if (dd.M == 0 || dd.N == 0) return;
   GemmDesc d = dd;
   d.batch_major = (d.nb0 * (int64_t)d.nb1 > 1) && d.A.bs0 == 0 && d.A.bs1 == 0 && d.A.off1 == nullptr;
+  const int v = (g_site_variants && d.site >= 0 && d.site < kNumSites) ? g_site_variants[d.site] : -1;
   if (g_profiler) g_profiler->before(d, stream);
   switch (d.compute) {
     case Compute::F64:
       require(!d.out_f32, "gemm: FP64 compute writes complex128");
-      launch_typed<double, double2>(d, stream);
+      launch_typed<double, double2>(d, stream, v);
       break;
     case Compute::F32:
-      if (d.out_f32) launch_typed<float, float2>(d, stream);
-      else launch_typed<float, double2>(d, stream);
+      if (d.out_f32) launch_typed<float, float2>(d, stream, v);
+      else launch_typed<float, double2>(d, stream, v);
       break;
     case Compute::BF16X3:
-      if (d.out_f32) launch_bf16x3<float2>(d, stream);
-      else launch_bf16x3<double2>(d, stream);
+      if (d.out_f32) launch_bf16x3<float2>(d, stream, v);
+      else launch_bf16x3<double2>(d, stream, v);
       break;
     default:
       throw Error(kErrArgument, "gemm: compute mode not available");

@@ -85,6 +85,15 @@ void set_bf16_tiles(int mode);
 // the LDS k-depth; 83 or 84 forces k-depth 8 with registers allocated for 3 or
 // 4 waves per SIMD.  All variants give the same sums.
 void set_mfma_bk(int variant);
+// Launch variant per call site for the launches of the calling thread
+// (table of kNumSites entries, -1: the built-in choice; nullptr: none):
+// an LDS k-depth / waves-per-SIMD value of set_mfma_bk for FP64 and FP32,
+// a kBf16Tile* value for split-bf16.  The global switches above win.
+void set_site_variants(const int* table);
+// Variants worth timing for a compute mode (the autotuner's candidates);
+// returns their count, writes at most cap of them.
+int gemm_variant_candidates(Compute c, int* out, int cap);
+bool gemm_variant_valid(Compute c, int variant);
 
 struct GemmDesc {
   int site = kSiteOther;

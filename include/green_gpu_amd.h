@@ -230,6 +230,49 @@ int ggpu_gw_set_profiling(ggpu_gw_plan* plan, int enable);
  * FLOPs (8 per complex multiply-add), launch count. */
 int ggpu_gw_site_stats(ggpu_gw_plan* plan, int site, double* ms, double* flops, int64_t* launches);
 
+/* Graph replay (enable != 0): the next iterate records the iteration's
+ * launches (G staging, every q's P0, IR, Dyson and Sigma chains, the Sigma
+ * layout change) into one HIP graph and replays it; later iterates replay it
+ * with one launch.  A new capture is made when G or Sigma change address
+ * (ggpu_gw_iterate always uses the plan's own device copies) or after
+ * ggpu_gw_set_q_share, *_set_variants, *_autotune, *_load_integrals,
+ * *_synth_integrals or *_set_integral_residence.  Needs the integrals
+ * resident in HBM (GGPU_INTEGRALS_DEVICE; GGPU_ERR_STATE otherwise) and no
+ * per-site profiling; the results are those of the launches it replays, and
+ * ggpu_gw_last_timings reports only the total.  No counterpart in the
+ * reference, which enqueues every launch from the host each iteration
+ * (cugw_utils.cu:103-202). */
+int ggpu_gw_set_graph(ggpu_gw_plan* plan, int enable);
+/* Number of graph captures the plan has made. */
+int ggpu_gw_graph_captures(ggpu_gw_plan* plan, int* captures);
+
+/* Autotuner: pick the launch variant of each contraction call site of this
+ * plan by timing every candidate on the plan's own shapes and data, inside the
+ * iteration's own launch sequence (operands cold in the caches as they are in
+ * an iteration): the P0 and Sigma chains of the first q of this plan's share,
+ * `batches` pair batches per chain (and per star member), best of `rounds`
+ * (1 .. 8).  Candidates: the LDS k-depth and waves-per-SIMD register
+ * allocation for FP64 / FP32, the workgroup tile for split-bf16.  The choice
+ * changes no result (every variant sums in the same order); it is kept by the
+ * plan and used by every later call.  Nothing the caller sees is written:
+ * the plan's own buffers hold the trial results, and the next iterate starts
+ * from zero.  G / G_device as for ggpu_gw_iterate / ggpu_gw_iterate_device.
+ * No counterpart in the reference, which leaves kernel choice to cuBLAS's
+ * internal heuristics (cublas_routines_prec.cu:53-228). */
+int ggpu_gw_autotune(ggpu_gw_plan* plan, const double* G, int rounds, int batches);
+int ggpu_gw_autotune_device(ggpu_gw_plan* plan, const void* G_device, int rounds, int batches);
+/* What the last autotune measured: *n candidates (their variant ids in
+ * variants[0 .. min(cap, n) - 1]); ms[c * 11 + site]: best time of candidate
+ * c at that call site (site numbers as for ggpu_gw_site_stats; negative:
+ * not run); *wall_ms: wall time of the tuning.  Output arrays may be NULL. */
+int ggpu_gw_autotune_report(ggpu_gw_plan* plan, int cap, int* n, int* variants, double* ms, double* wall_ms);
+/* The plan's launch variant per call site (11 entries, -1: built-in choice),
+ * to save a tuning and give it to another plan of the same shape and
+ * precision.  set rejects a variant the plan's precision does not have or a
+ * site that is not one of the six contraction sites (1 .. 6). */
+int ggpu_gw_get_variants(ggpu_gw_plan* plan, int* variants);
+int ggpu_gw_set_variants(ggpu_gw_plan* plan, const int* variants);
+
 void ggpu_gw_plan_destroy(ggpu_gw_plan* plan);
 
 #ifdef __cplusplus

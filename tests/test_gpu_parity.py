@@ -362,3 +362,108 @@ def test_gw_reduced_precision(ggpu, synth, oracle, precision, rtol):
     scale = np.abs(Sig_or).max()
     print(f"{precision}: max-abs {err:.3e}, |Sigma| max {scale:.3e}, rel {err / scale:.2e}", flush=True)
     assert err < rtol * scale
+
+
+# ---------------------------------------------------------------------------
+# Autotuner
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16x3"])
+def test_autotune_picks_fastest_and_keeps_results(ggpu, synth, oracle, precision):
+    """The autotuner times every launch variant of the six contraction call
+    sites on the plan's own shapes and keeps the fastest per site; the choice
+    changes no result (every variant sums in the same order), and a tuning
+    can be handed to another plan."""
+    mesh, nao, NQ, ns = (2, 2, 1), 13, 40, 2
+    sym, tau, T_wt, T_tw, G, D, V = gw_system(ggpu, synth, mesh, nao, NQ, ns, 50.0, 1e3, 31)
+    print(f"autotune {precision} mesh={mesh} nao={nao} NQ={NQ}", flush=True)
+    kern = ggpu.GwKernel(nao, ns, NQ, T_wt, T_tw, sym, precision=precision, solver="cholesky")
+    kern.load_integrals(V)
+    before = kern.solve(G)
+    assert kern.get_variants() == [-1] * len(ggpu.SITES)
+    rep = kern.autotune(G, rounds=2)
+    names = ["p0_gv", "p0_gvh", "p0_contract", "sig_vg", "sig_py", "sig_contract"]
+    assert sorted(rep["sites"]) == sorted(names)
+    ncand = {"fp64": 3, "fp32": 3, "bf16x3": 7}[precision]
+    chosen = kern.get_variants()
+    for si, name in enumerate(ggpu.SITES):
+        if name not in names:
+            assert chosen[si] == -1
+            continue
+        times = rep["sites"][name]["ms"]
+        assert len(times) == ncand and all(t > 0 for t in times.values())
+        assert rep["sites"][name]["chosen"] == min(times, key=times.get)
+        assert kern.variant_name(chosen[si]) == rep["sites"][name]["chosen"]
+    assert rep["wall_ms"] > 0
+    after = kern.solve(G)
+    assert np.array_equal(before, after)
+    if precision == "fp64":
+        osym = oracle.symmetry_none(mesh, nao, NQ)
+        assert max_abs(after, oracle.gw_solve(G, V, osym, T_wt, T_tw, solver="cholesky")) < TOL
+    other = ggpu.GwKernel(nao, ns, NQ, T_wt, T_tw, sym, precision=precision, solver="cholesky")
+    other.load_integrals(V)
+    other.set_variants(chosen)
+    assert other.get_variants() == chosen
+    assert np.array_equal(other.solve(G), before)
+    import green_gpu_amd as g
+    bad = [-1] * len(ggpu.SITES)
+    bad[ggpu.SITES.index("dyson")] = chosen[ggpu.SITES.index("sig_py")]
+    with pytest.raises(g.GgpuError) as e:
+        other.set_variants(bad)
+    assert e.value.code == 1 and "not available" in str(e.value)
+    bad = [-1] * len(ggpu.SITES)
+    bad[ggpu.SITES.index("sig_py")] = 12345
+    with pytest.raises(g.GgpuError):
+        other.set_variants(bad)
+
+
+# ---------------------------------------------------------------------------
+# Graph replay
+# ---------------------------------------------------------------------------
+def test_graph_replay_matches_eager(ggpu, synth, oracle):
+    """The iteration captured into one HIP graph and replayed: every replay
+    equals the eager iteration bit for bit, with eager work and
+    synchronisations between replays; a new G / Sigma address, a q share or a
+    new tuning makes a new capture; graph replay refuses streamed integrals."""
+    mesh, nao, NQ, ns = (2, 2, 1), 13, 40, 2
+    sym, tau, T_wt, T_tw, G, D, V = gw_system(ggpu, synth, mesh, nao, NQ, ns, 50.0, 1e3, 53)
+    print("graph replay mesh=(2,2,1) nao=13 NQ=40", flush=True)
+    eager = ggpu.GwKernel(nao, ns, NQ, T_wt, T_tw, sym, solver="cholesky")
+    eager.load_integrals(V)
+    ref = eager.solve(G)
+    osym = oracle.symmetry_none(mesh, nao, NQ)
+    assert max_abs(ref, oracle.gw_solve(G, V, osym, T_wt, T_tw, solver="cholesky")) < TOL
+    kern = ggpu.GwKernel(nao, ns, NQ, T_wt, T_tw, sym, solver="cholesky")
+    kern.load_integrals(V)
+    kern.set_graph(True)
+    G2 = synth.make_green(sym.nk, nao, ns, 50.0, tau, 54)[0]
+    ref2 = eager.solve(G2)
+    for rep in range(3):
+        assert np.array_equal(kern.solve(G), ref), f"replay {rep}"
+        assert np.array_equal(eager.solve(G2), ref2)         # eager work between replays
+        assert np.array_equal(kern.solve(G2), ref2), f"replay {rep}, second G"
+    assert kern.graph_captures() == 1          # host G is copied to the plan's own device buffer
+    Gd = ggpu.DeviceArray.from_numpy(G)
+    Sd = ggpu.DeviceArray.empty_like(Gd)
+    for _ in range(2):
+        kern.solve_device(Gd.ptr, Sd.ptr)
+        assert np.array_equal(Sd.numpy(), ref)
+    assert kern.graph_captures() == 2
+    parts = []
+    for r in range(2):
+        kern.set_q_share(r, 2)
+        parts.append(kern.solve(G))
+    assert kern.graph_captures() == 4
+    assert max_abs(parts[0] + parts[1], ref) < 1e-12
+    kern.set_q_share(0, 1)
+    kern.autotune(G, rounds=1)
+    assert np.array_equal(kern.solve(G), ref)
+    assert kern.graph_captures() == 5
+    Gd.free()
+    Sd.free()
+    import green_gpu_amd as g
+    host = ggpu.GwKernel(nao, ns, NQ, T_wt, T_tw, sym, solver="cholesky", integrals="host")
+    host.load_integrals(V)
+    host.set_graph(True)
+    with pytest.raises(g.GgpuError) as e:
+        host.solve(G)
+    assert e.value.code == 5 and "resident in HBM" in str(e.value)
