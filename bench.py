@@ -235,25 +235,41 @@ def main():
         src.free()
         dst.free()
 
-    # HF build (exchange K + direct J; FP64 in every mode) on the same system:
-    # D = -G(beta), synthetic integrals with the same seed
+    # HF build (exchange K + direct J) on the same system in every mode, the
+    # exchange products in the mode's arithmetic and K's partial sums, J and
+    # the Ewald-free Fock matrix in FP64: D = -G(beta), synthetic integrals
+    # with the same seed
     hf = None
     if not a.no_hf:
-        hk = pkg.HfKernel(nao, ns, NQ, sym, integrals="generated" if a.integrals == "generated" else "device")
-        hk.synth_integrals(seed, sigma)
         D = -np.ascontiguousarray(G[-1])                                   # (ns, ink, nao, nao)
-        if q_run == sym.inq:
-            hk.solve(D)                                                    # warm-up
-        t = time.perf_counter()
-        F = hk.solve(D)
-        hf_s = time.perf_counter() - t
-        hk.close()
         hf_fl = float(nk) * nk * ns * 16.0 * NQ * nao ** 3
-        # E_HF-like scalar, accumulated in FP64 on the host (the reference defines none)
-        e_hf = float(0.5 * np.real(np.einsum("skab,skba->", D, F)) / nk)
-        hf = {"seconds": round(hf_s, 4), "exchange_tflops": round(hf_fl / hf_s / 1e12, 2), "flop": hf_fl,
-              "e_hf_like": e_hf, "precision": "fp64", "includes": "K + J, host-to-host"}
-        log(f"[hf] {hf_s:.3f}s  {hf_fl / hf_s / 1e12:.1f} TFLOP/s (exchange FLOPs)")
+        hf_modes, F64 = {}, None
+        for mode in modes:
+            hk = pkg.HfKernel(nao, ns, NQ, sym, integrals="generated" if a.integrals == "generated" else "device",
+                              precision=mode)
+            hk.synth_integrals(seed, sigma)
+            if q_run == sym.inq:
+                hk.solve(D)                                                # warm-up
+            t = time.perf_counter()
+            F = hk.solve(D)
+            hf_s = time.perf_counter() - t
+            hk.close()
+            # E_HF-like scalar, accumulated in FP64 on the host (the reference defines none)
+            e_hf = float(0.5 * np.real(np.einsum("skab,skba->", D, F)) / nk)
+            if mode == "fp64":
+                F64, e64 = F, e_hf
+            kerr = float(np.abs(F - F64).max())
+            hf_modes[mode] = {"seconds": round(hf_s, 4), "exchange_tflops": round(hf_fl / hf_s / 1e12, 2),
+                              "e_hf_like": e_hf, "k_max_abs_err": kerr,
+                              # relative to max |F|: F - F_fp64 is K's error (J is FP64 in every mode)
+                              "k_rel_err": kerr / float(np.abs(F64).max()),
+                              "e_hf_abs_err": abs(e_hf - e64), "e_hf_rel_err": abs(e_hf - e64) / abs(e64)}
+            log(f"[hf {mode}] {hf_s:.3f}s  {hf_fl / hf_s / 1e12:.1f} TFLOP/s (exchange FLOPs)  "
+                f"K err {kerr:.2e}  E_HF err {abs(e_hf - e64):.2e}")
+        h64 = hf_modes["fp64"]
+        hf = {"seconds": h64["seconds"], "exchange_tflops": h64["exchange_tflops"], "flop": hf_fl,
+              "e_hf_like": h64["e_hf_like"], "precision": "fp64", "includes": "K + J, host-to-host",
+              "modes": hf_modes}
 
     # headline: the fastest mode whose Sigma stays within 1e-6 of FP64
     ok = [m for m in modes if runs[m]["sigma_max_abs_err"] <= 1e-6]
@@ -370,6 +386,21 @@ def run_batch(a, w, pkg, synth):
         res[mode] = {"batched_s": batched, "sequential_s": sequential, "tflops": fl / batched / 1e12,
                      "autotune_ms": tune_ms}
         sig[mode] = [x.numpy() for x in Sd]
+        if a.integrals == "device":
+            # all systems as one captured HIP graph (one launch per step)
+            b = pkg.GwBatch(kerns, gp, sp)
+            t = time.perf_counter()
+            b.run()
+            log(f"[{mode}] batch graph captured and replayed once: {time.perf_counter() - t:.2f}s")
+            t0 = time.perf_counter()
+            for _ in range(a.steps):
+                b.run()
+                log(f"[{mode}] graph step done")
+            res[mode]["graph_s"] = (time.perf_counter() - t0) / a.steps
+            res[mode]["graph_equals_eager"] = all(np.array_equal(x.numpy(), y) for x, y in zip(Sd, sig[mode]))
+            b.close()
+            log(f"[{mode}] one graph launch {res[mode]['graph_s']:.2f}s, "
+                f"equal to the eager result: {res[mode]['graph_equals_eager']}")
         for x in Sd:
             x.free()
         for k in kerns:
@@ -397,7 +428,9 @@ def run_batch(a, w, pkg, synth):
                       "tflops": round(v["tflops"], 2),
                       "speedup_vs_fp64": round(res["fp64"]["batched_s"] / v["batched_s"], 3),
                       "sigma_max_abs_err": v["sigma_max_abs_err"],
-                      "autotune_ms": round(v["autotune_ms"], 1)} for m, v in res.items()},
+                      "autotune_ms": round(v["autotune_ms"], 1),
+                      **({"graph_s": round(v["graph_s"], 3), "graph_equals_eager": v["graph_equals_eager"]}
+                         if "graph_s" in v else {})} for m, v in res.items()},
     }
     print(json.dumps(out))
 

@@ -51,6 +51,7 @@ def lib():
         L.ggpu_hf_plan_create.argtypes = [C.POINTER(vp), i32, i32, i32, i32, C.POINTER(SymmetryC)]
         L.ggpu_hf_set_integral_residence.argtypes = [vp, i32]
         L.ggpu_hf_set_k_share.argtypes = [vp, i32, i32]
+        L.ggpu_hf_set_precision.argtypes = [vp, i32]
         L.ggpu_hf_load_integrals.argtypes = [vp, pd]
         L.ggpu_hf_synth_integrals.argtypes = [vp, u64, f64]
         L.ggpu_hf_exchange.argtypes = [vp, pd, pd]
@@ -73,6 +74,11 @@ def lib():
         L.ggpu_gw_set_profiling.argtypes = [vp, i32]
         L.ggpu_gw_site_stats.argtypes = [vp, i32, pd, pd, C.POINTER(C.c_int64)]
         L.ggpu_gw_set_graph.argtypes = [vp, i32]
+        L.ggpu_gw_batch_create.argtypes = [C.POINTER(vp), C.POINTER(vp), i32, C.POINTER(vp), C.POINTER(vp)]
+        L.ggpu_gw_batch_run.argtypes = [vp, pd]
+        L.ggpu_gw_batch_captures.argtypes = [vp, C.POINTER(C.c_int)]
+        L.ggpu_gw_batch_destroy.argtypes = [vp]
+        L.ggpu_gw_batch_destroy.restype = None
         L.ggpu_gw_graph_captures.argtypes = [vp, C.POINTER(C.c_int)]
         L.ggpu_gw_autotune.argtypes = [vp, pd, i32, i32]
         L.ggpu_gw_autotune_device.argtypes = [vp, vp, i32, i32]
@@ -87,6 +93,7 @@ def lib():
 
 EXPORTED = ["ggpu_last_error", "ggpu_device_count", "ggpu_device_memory",
             "ggpu_hf_plan_create", "ggpu_hf_set_integral_residence", "ggpu_hf_set_k_share",
+            "ggpu_hf_set_precision",
             "ggpu_hf_load_integrals",
             "ggpu_hf_synth_integrals",
             "ggpu_hf_exchange", "ggpu_hf_solve", "ggpu_hf_plan_destroy",
@@ -96,6 +103,7 @@ EXPORTED = ["ggpu_last_error", "ggpu_device_count", "ggpu_device_memory",
             "ggpu_gw_iterate", "ggpu_gw_iterate_device", "ggpu_gw_iterate_many", "ggpu_gw_polarization",
             "ggpu_gw_last_timings", "ggpu_gw_set_profiling", "ggpu_gw_site_stats",
             "ggpu_gw_set_graph", "ggpu_gw_graph_captures",
+            "ggpu_gw_batch_create", "ggpu_gw_batch_run", "ggpu_gw_batch_captures", "ggpu_gw_batch_destroy",
             "ggpu_gw_autotune", "ggpu_gw_autotune_device", "ggpu_gw_autotune_report",
             "ggpu_gw_get_variants", "ggpu_gw_set_variants",
             "ggpu_gw_plan_destroy"]
@@ -252,6 +260,42 @@ def iterate_many(kernels, G_ptrs, Sigma_ptrs):
 _hip = None
 
 
+class GwBatch:
+    """Several GwKernel plans (integrals resident in HBM) replayed as one HIP
+    graph launch; G_ptrs / Sigma_ptrs are device pointers, fixed here."""
+
+    def __init__(self, kernels, G_ptrs, Sigma_ptrs):
+        n = len(kernels)
+        if not (n == len(G_ptrs) == len(Sigma_ptrs)):
+            raise ValueError("one G and one Sigma per kernel")
+        self._kernels = list(kernels)          # the plans must outlive the batch
+        plans = (C.c_void_p * n)(*[k._h.value for k in kernels])
+        gp = (C.c_void_p * n)(*G_ptrs)
+        sp = (C.c_void_p * n)(*Sigma_ptrs)
+        h = C.c_void_p()
+        _check(lib().ggpu_gw_batch_create(C.byref(h), plans, n, gp, sp))
+        self._h = h
+
+    def run(self):
+        """One replay of every system's iteration; returns its device time (ms)."""
+        ms = C.c_double()
+        _check(lib().ggpu_gw_batch_run(self._h, C.byref(ms)))
+        return ms.value
+
+    def captures(self):
+        n = C.c_int()
+        _check(lib().ggpu_gw_batch_captures(self._h, C.byref(n)))
+        return n.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ggpu_gw_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
 def _hip_runtime():
     """The HIP runtime libgreen_gpu_amd.so itself is linked against (already
     loaded by the library; dlopen resolves the same soname to it).  Device
@@ -361,7 +405,7 @@ class HfKernel:
     """HF kernel on the GPU (reference: scalar_hf_gpu_kernel, hf_gpu_kernel.h:67-84;
     x2c_hf_gpu_kernel, hf_gpu_kernel.h:87-100, when nso = 2 nao)."""
 
-    def __init__(self, nao, ns, NQ, sym, madelung=0.0, S_k=None, nso=None, integrals="device"):
+    def __init__(self, nao, ns, NQ, sym, madelung=0.0, S_k=None, nso=None, integrals="device", precision="fp64"):
         self.nao, self.ns, self.NQ = int(nao), int(ns), int(NQ)
         self.nso = self.nao if nso is None else int(nso)
         self.sym = sym if isinstance(sym, Symmetry) else Symmetry.like(sym)
@@ -372,6 +416,7 @@ class HfKernel:
         _check(lib().ggpu_hf_plan_create(C.byref(h), self.nao, self.nso, self.ns, self.NQ, C.byref(self._csym)))
         self._h = h
         _check(lib().ggpu_hf_set_integral_residence(self._h, RESIDENCE[integrals]))
+        _check(lib().ggpu_hf_set_precision(self._h, PRECISION[precision]))
 
     @property
     def mat_shape(self):

@@ -12,6 +12,7 @@
 
 struct ggpu_hf_plan { ggpu::HfPlan* impl; };
 struct ggpu_gw_plan { ggpu::GwPlan* impl; };
+struct ggpu_gw_batch { ggpu::GwBatch* impl = nullptr; };
 
 namespace {
 thread_local std::string g_last_error;
@@ -75,6 +76,18 @@ int ggpu_hf_set_integral_residence(ggpu_hf_plan* plan, int residence) {
     ggpu::require(plan && plan->impl, "NULL plan");
     auto* p = plan->impl;
     p->store.set_residence(residence, p->kb, (int64_t)p->NQ * p->nao * p->nao);
+  });
+}
+
+int ggpu_hf_set_precision(ggpu_hf_plan* plan, int precision) {
+  return guarded([&] {
+    ggpu::require(plan && plan->impl, "NULL plan");
+    ggpu::require(precision >= GGPU_PRECISION_FP64 && precision <= GGPU_PRECISION_BF16X3, "hf: unknown precision");
+    auto* p = plan->impl;
+    p->compute = precision == GGPU_PRECISION_FP64   ? ggpu::Compute::F64
+                 : precision == GGPU_PRECISION_FP32 ? ggpu::Compute::F32
+                                                    : ggpu::Compute::BF16X3;
+    p->dm32.resize(precision == GGPU_PRECISION_FP64 ? 0 : p->dm_fbz.size());
   });
 }
 
@@ -238,6 +251,48 @@ int ggpu_gw_iterate_many(ggpu_gw_plan* const* plans, int n, const void* const* G
       }
     }
   });
+}
+
+int ggpu_gw_batch_create(ggpu_gw_batch** batch, ggpu_gw_plan* const* plans, int n, const void* const* G_device,
+                         void* const* Sigma_device) {
+  return guarded([&] {
+    ggpu::require(batch != nullptr && plans != nullptr && n >= 1, "NULL argument");
+    *batch = nullptr;
+    std::vector<ggpu::GwPlan*> impl(n);
+    for (int i = 0; i < n; ++i) {
+      ggpu::require(plans[i] && plans[i]->impl, "gw batch: NULL plan");
+      impl[i] = plans[i]->impl;
+    }
+    auto* b = new ggpu_gw_batch;
+    try {
+      b->impl = new ggpu::GwBatch(impl.data(), n, G_device, Sigma_device);
+    } catch (...) {
+      delete b;
+      throw;
+    }
+    *batch = b;
+  });
+}
+
+int ggpu_gw_batch_run(ggpu_gw_batch* batch, double* ms) {
+  return guarded([&] {
+    ggpu::require(batch && batch->impl, "NULL batch");
+    batch->impl->run();
+    if (ms) *ms = batch->impl->last_ms;
+  });
+}
+
+int ggpu_gw_batch_captures(ggpu_gw_batch* batch, int* captures) {
+  return guarded([&] {
+    ggpu::require(batch && batch->impl && captures, "NULL argument");
+    *captures = batch->impl->captures;
+  });
+}
+
+void ggpu_gw_batch_destroy(ggpu_gw_batch* batch) {
+  if (!batch) return;
+  delete batch->impl;
+  delete batch;
 }
 
 int ggpu_gw_polarization(ggpu_gw_plan* plan, const double* G, int iq, int which, double* out) {

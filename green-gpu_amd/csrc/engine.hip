@@ -657,7 +657,7 @@ GwPlan::GwPlan(int nao_, int nso_, int ns_, int NQ_, int nts_, int nw_b_, const 
 
 GwPlan::~GwPlan() {
   if (stream) (void)hipStreamSynchronize(stream);
-  invalidate_graph();
+  drop_graph();
   for (auto& e : ev) (void)hipEventDestroy(e);
   for (auto& e : qev) (void)hipEventDestroy(e);
   if (stream) (void)hipStreamDestroy(stream);
@@ -1034,16 +1034,21 @@ void GwPlan::enqueue_iteration(const z64* G_dev, z64* S_dev, bool timed) {
   if (timed) GGPU_HIP(hipEventRecord(ev[6], stream));
 }
 
-void GwPlan::invalidate_graph() {
+void GwPlan::drop_graph() {
   if (gexec) (void)hipGraphExecDestroy(gexec);
   gexec = nullptr;
   graph_G = graph_S = nullptr;
 }
 
+void GwPlan::invalidate_graph() {
+  drop_graph();
+  ++config_gen;
+}
+
 // Record the launches of one iteration into a graph (nothing runs) and
 // instantiate it.  Every buffer is zeroed by a kernel, not a memset node.
 void GwPlan::capture(const z64* G_dev, z64* S_dev) {
-  invalidate_graph();
+  drop_graph();
   GGPU_HIP(hipStreamSynchronize(stream));
   GGPU_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
   hipGraph_t graph = nullptr;
@@ -1115,6 +1120,96 @@ void GwPlan::iterate_device(const z64* G_dev, z64* S_dev) {
   generated_last = store.bytes_generated() - generated0;
   if (profiling) prof.collect();
   check_info();
+}
+
+GwBatch::GwBatch(GwPlan* const* p, int n, const void* const* G_dev, void* const* S_dev) {
+  require(n >= 1 && p && G_dev && S_dev, "gw batch: need at least one plan and its G / Sigma");
+  for (int i = 0; i < n; ++i) {
+    require(p[i] != nullptr && G_dev[i] && S_dev[i], "gw batch: NULL plan, G or Sigma");
+    require(std::find(plans.begin(), plans.end(), p[i]) == plans.end(), "gw batch: a plan may appear only once");
+    require(p[i]->store.loaded(), "gw batch: system " + std::to_string(i) + ": integrals not loaded", kErrState);
+    require(!p[i]->store.staged(),
+            "gw batch: system " + std::to_string(i) + ": graph replay needs the integrals resident in HBM", kErrState);
+    plans.push_back(p[i]);
+    G.push_back(static_cast<const z64*>(G_dev[i]));
+    S.push_back(static_cast<z64*>(S_dev[i]));
+  }
+  GGPU_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  GGPU_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+  GGPU_HIP(hipEventCreate(&t0));
+  GGPU_HIP(hipEventCreate(&t1));
+  join.resize(n, nullptr);
+  for (auto& e : join) GGPU_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+}
+
+GwBatch::~GwBatch() {
+  if (stream) (void)hipStreamSynchronize(stream);
+  if (exec) (void)hipGraphExecDestroy(exec);
+  for (auto e : join)
+    if (e) (void)hipEventDestroy(e);
+  for (auto e : {fork, t0, t1})
+    if (e) (void)hipEventDestroy(e);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+// Fork every plan's stream from the batch stream inside one capture, record
+// each plan's iteration on its own stream and join them back.
+void GwBatch::capture() {
+  if (exec) (void)hipGraphExecDestroy(exec);
+  exec = nullptr;
+  for (auto* p : plans) GGPU_HIP(hipStreamSynchronize(p->stream));
+  GGPU_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+  hipGraph_t graph = nullptr;
+  try {
+    GGPU_HIP(hipEventRecord(fork, stream));
+    for (size_t i = 0; i < plans.size(); ++i) {
+      GwPlan& p = *plans[i];
+      GGPU_HIP(hipStreamWaitEvent(p.stream, fork, 0));
+      VariantGuard vg(p.variants);
+      p.enqueue_iteration(G[i], S[i], false);
+      GGPU_HIP(hipEventRecord(join[i], p.stream));
+      GGPU_HIP(hipStreamWaitEvent(stream, join[i], 0));
+    }
+  } catch (...) {
+    (void)hipStreamEndCapture(stream, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    throw;
+  }
+  GGPU_HIP(hipStreamEndCapture(stream, &graph));
+  const hipError_t e = hipGraphInstantiateWithFlags(&exec, graph, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) {
+    exec = nullptr;
+    GGPU_HIP(e);
+  }
+  gens.clear();
+  for (auto* p : plans) gens.push_back(p->config_gen);
+  ++captures;
+}
+
+void GwBatch::run() {
+  bool stale = exec == nullptr;
+  for (size_t i = 0; i < plans.size() && !stale; ++i) stale = gens[i] != plans[i]->config_gen;
+  for (size_t i = 0; i < plans.size(); ++i) {
+    require(!plans[i]->store.staged(),
+            "gw batch: system " + std::to_string(i) + ": graph replay needs the integrals resident in HBM", kErrState);
+    require(!plans[i]->profiling, "gw batch: per-site profiling does not apply to graph replay", kErrState);
+  }
+  if (stale) capture();
+  GGPU_HIP(hipEventRecord(t0, stream));
+  GGPU_HIP(hipGraphLaunch(exec, stream));
+  GGPU_HIP(hipEventRecord(t1, stream));
+  GGPU_HIP(hipStreamSynchronize(stream));
+  float ms = 0;
+  GGPU_HIP(hipEventElapsedTime(&ms, t0, t1));
+  last_ms = ms;
+  for (size_t i = 0; i < plans.size(); ++i) {
+    try {
+      plans[i]->check_info();
+    } catch (const Error& e) {
+      throw Error(e.code, "system " + std::to_string(i) + ": " + e.what());
+    }
+  }
 }
 
 void GwPlan::iterate(const double* G, double* Sigma) {
@@ -1253,25 +1348,35 @@ HfPlan::~HfPlan() {
 void HfPlan::exchange_device() {
   const int64_t n2 = (int64_t)nao * nao, stride = (int64_t)NQ * n2;
   const double px = (nps == 1) ? -0.5 : -1.0;   // -1 for ns = 2 and the X2C blocks
-  z64* Vdir = work.get();
-  z64* X = Vdir + kb * stride;
-  const int64_t xs = (int64_t)kb * stride;          // one spin slab of X
+  const bool lp = compute != Compute::F64;
+  const int64_t es = lp ? (int64_t)sizeof(c32) : (int64_t)sizeof(z64);
+  char* Vdir = reinterpret_cast<char*>(work.get());
+  char* X = Vdir + kb * stride * es;
+  const int64_t xs = (int64_t)kb * stride;          // one spin slab of X (elements)
+  const Span wk{work.get(), work.bytes() / (size_t)es};
+  if (lp) {
+    require(dm32.size() == dm_fbz.size(), "hf: reduced-precision density buffer missing");
+    to_c32(dm_fbz.get(), dm32.get(), (int64_t)dm_fbz.size(), stream);
+  }
+  const void* dm = lp ? (const void*)dm32.get() : (const void*)dm_fbz.get();
+  const Span dsp{dm, dm_fbz.size()};
   if (share_size > 1) zero(Kout.get(), Kout.size(), stream);
   for (int ik = share_rank; ik < sym.ink; ik += share_size) {
     zero(Kpart.get(), Kpart.size(), stream);
     for (int i0 = 0; i0 < sym.nk; i0 += kb) {
       const int n = std::min(kb, sym.nk - i0);
-      unfold_pairs(store, tables[ik], i0, n, bidx_off.get(), Vdir, 0, NQ, nao, stream);
+      unfold_pairs(store, tables[ik], i0, n, bidx_off.get(), Vdir, lp, NQ, nao, stream);
       {
         // X[s][b] = V(k, k2 = i0 + b) D_s(k2) for every density block s and
         // pair b of the batch, in one launch
         GemmDesc g;
+        g.compute = compute; g.out_f32 = lp;
         g.M = NQ * nao; g.N = nao; g.K = nao; g.nb0 = nps; g.nb1 = n;
         g.A.ptr = Vdir; g.A.rs = nao; g.A.cs = 1; g.A.bs0 = 0; g.A.bs1 = stride;
-        g.B.ptr = dm_fbz.get() + (int64_t)i0 * nps * n2; g.B.rs = nao; g.B.cs = 1; g.B.bs0 = n2;
+        g.B.ptr = (const char*)dm + (int64_t)i0 * nps * n2 * es; g.B.rs = nao; g.B.cs = 1; g.B.bs0 = n2;
         g.B.bs1 = (int64_t)nps * n2;
         g.C.ptr = X; g.C.rs = nao; g.C.cs = 1; g.C.bs0 = xs; g.C.bs1 = stride;
-        gemm_on(g, work, dm_fbz, work, stream, kSiteHF);
+        gemm_on(g, wk, dsp, wk, stream, kSiteHF);
       }
       const int total = n * NQ;
       const int ch = (total + nsplit - 1) / nsplit;
@@ -1282,14 +1387,17 @@ void HfPlan::exchange_device() {
         const int kbat = part == 0 ? ch : rem;
         const int64_t first = part == 0 ? 0 : (int64_t)full * ch;
         GemmDesc f;
+        f.compute = compute; f.out_f32 = 0;       // K partial sums in FP64
         f.M = nao; f.N = nao; f.K = nao; f.KB = kbat; f.nb0 = nps; f.nb1 = slices;
-        f.A.ptr = X + first * n2; f.A.rs = nao; f.A.cs = 1; f.A.kbs = n2; f.A.bs0 = xs; f.A.bs1 = (int64_t)ch * n2;
-        f.B.ptr = Vdir + first * n2; f.B.rs = 1; f.B.cs = nao; f.B.kbs = n2; f.B.bs0 = 0; f.B.bs1 = (int64_t)ch * n2;
+        f.A.ptr = X + first * n2 * es; f.A.rs = nao; f.A.cs = 1; f.A.kbs = n2; f.A.bs0 = xs;
+        f.A.bs1 = (int64_t)ch * n2;
+        f.B.ptr = Vdir + first * n2 * es; f.B.rs = 1; f.B.cs = nao; f.B.kbs = n2; f.B.bs0 = 0;
+        f.B.bs1 = (int64_t)ch * n2;
         f.B.conj = 1;
         f.C.ptr = Kpart.get() + (part == 0 ? 0 : (int64_t)full * nps * n2); f.C.rs = nao; f.C.cs = 1;
         f.C.bs0 = n2; f.C.bs1 = (int64_t)nps * n2;
         f.alpha_re = px / sym.nk; f.beta_re = 1.0;
-        gemm_on(f, work, work, Kpart, stream, kSiteHF);
+        gemm_on(f, wk, wk, Kpart, stream, kSiteHF);
       }
     }
     sum_partials(Kpart.get(), (int64_t)nps * n2, nsplit, Kout.get() + (int64_t)ik * nps * n2, (int64_t)nps * n2,

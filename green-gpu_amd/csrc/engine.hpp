@@ -201,7 +201,8 @@ struct GwPlan {
   const void* graph_G = nullptr;
   const void* graph_S = nullptr;
   int graph_captures = 0;               // captures made (a new one after every change)
-  void invalidate_graph();
+  int config_gen = 0;                   // bumped by every change that alters the launch sequence
+  void invalidate_graph();              // after such a change: drop the graph, bump config_gen
 
   SpinorTables x2c_split, x2c_merge;   // G spinor -> pseudo-spin blocks, Sigma back
 
@@ -218,15 +219,44 @@ struct GwPlan {
   // chain, best of `rounds`, and keep the fastest per site.
   void autotune(const z64* G_dev_hostlayout, int rounds, int batches);
   void set_variants(const int* v);
+  // The launches of one iteration on the plan's stream; with timed, the
+  // phase events around them.
+  void enqueue_iteration(const z64* G_dev_hostlayout, z64* Sigma_dev_hostlayout, bool timed);
+  void check_info();
 
  private:
   void stage_G(const z64* G_dev_hostlayout);
-  void enqueue_iteration(const z64* G_dev_hostlayout, z64* Sigma_dev_hostlayout, bool timed);
   void capture(const z64* G_dev_hostlayout, z64* Sigma_dev_hostlayout);
+  void drop_graph();
   void p0_for_q(int iq);
   void dyson_for_q();
   void sigma_for_q(int iq);
-  void check_info();
+};
+
+// Several independent GW plans replayed as one HIP graph: each plan's
+// iteration is a branch forked from and joined to the batch's own stream, so
+// one launch runs all systems concurrently.  Plans must keep their integrals
+// resident in HBM; a change to any plan (q share, variants, integrals) makes
+// the next run capture again.  Plans must outlive the batch and must not be
+// used from another thread while it runs.
+struct GwBatch {
+  std::vector<GwPlan*> plans;
+  std::vector<const z64*> G;
+  std::vector<z64*> S;
+  std::vector<int> gens;                // config_gen of each plan at the capture
+  hipStream_t stream = nullptr;
+  hipEvent_t fork = nullptr, t0 = nullptr, t1 = nullptr;
+  std::vector<hipEvent_t> join;
+  hipGraphExec_t exec = nullptr;
+  int captures = 0;
+  double last_ms = 0;
+
+  GwBatch(GwPlan* const* plans, int n, const void* const* G_dev, void* const* S_dev);
+  ~GwBatch();
+  void run();
+
+ private:
+  void capture();
 };
 
 struct HfPlan {
@@ -242,6 +272,10 @@ struct HfPlan {
   DeviceBuffer<int64_t> bidx_off;
   std::vector<PairTable> tables;   // per IBZ k: pairs (k, k2) for all k2
   int kb = 1, nsplit = 1;
+  // Arithmetic of the two exchange products (X = V D, K += X V^dagger); the
+  // partial sums of K, J and the Ewald term are always FP64.
+  Compute compute = Compute::F64;
+  DeviceBuffer<c32> dm32;               // complex64 copy of dm_fbz (reduced-precision modes)
   // This device's share of the IBZ k-points: ik = share_rank, share_rank +
   // share_size, ...; the other rows of K and F are zero, so the results of
   // all shares sum to the full ones.

@@ -97,6 +97,7 @@ typedef struct {
 
 typedef struct ggpu_hf_plan ggpu_hf_plan;
 typedef struct ggpu_gw_plan ggpu_gw_plan;
+typedef struct ggpu_gw_batch ggpu_gw_batch;
 
 /* Message of the last failure on this thread ("" if none). */
 const char* ggpu_last_error(void);
@@ -123,6 +124,12 @@ int ggpu_hf_plan_create(ggpu_hf_plan** plan, int nao, int nso, int ns, int NQ, c
 /* Choose where the integrals live (ggpu_integral_residence); call before
  * loading or synthesising them.  Results do not depend on the choice. */
 int ggpu_hf_set_integral_residence(ggpu_hf_plan* plan, int residence);
+/* Arithmetic of the two exchange products X = V D and K += X V^dagger
+ * (ggpu_precision; default FP64).  FP32 and split-bf16 x3 run them on
+ * complex64 copies of V and D with K's partial sums, the Hartree term J and
+ * the Ewald term kept in FP64.  The reference's exchange is FP64 or complex64
+ * as a whole (cuhf_utils<prec>, cuhf_utils.cu:32-159). */
+int ggpu_hf_set_precision(ggpu_hf_plan* plan, int precision);
 /* This device's share of the IBZ k-points: exchange and solve then compute
  * the rows ik = rank, rank + size, ... < ink only and leave the other rows
  * zero, so the results of shares 0 .. size-1 sum to the full K or Fock.  The
@@ -208,6 +215,24 @@ int ggpu_gw_iterate_device(ggpu_gw_plan* plan, const void* G_device, void* Sigma
  * which runs one system per process (gw_gpu_kernel::solve,
  * gw_gpu_kernel.cpp:163-205, called once per iteration). */
 int ggpu_gw_iterate_many(ggpu_gw_plan* const* plans, int n, const void* const* G_device, void* const* Sigma_device);
+
+/* Several independent systems as ONE HIP graph launch: the batch captures
+ * every plan's whole iteration (as ggpu_gw_set_graph does for one plan), each
+ * on its plan's stream, forked from and joined to the batch's own stream, so
+ * ggpu_gw_batch_run replays all systems concurrently with one launch.  G /
+ * Sigma device buffers as for ggpu_gw_iterate_device, fixed at creation.
+ * Every plan needs its integrals resident in HBM (GGPU_ERR_STATE otherwise);
+ * a later change to a plan (q share, variants, autotune, integrals) makes the
+ * next run capture again.  Plans must outlive the batch and must not be used
+ * elsewhere while it runs.  ms (may be NULL): device time of the replay.
+ * On a failed Dyson factorisation the message names the first failing
+ * system.  No counterpart in the reference (one system per process). */
+int ggpu_gw_batch_create(ggpu_gw_batch** batch, ggpu_gw_plan* const* plans, int n, const void* const* G_device,
+                         void* const* Sigma_device);
+int ggpu_gw_batch_run(ggpu_gw_batch* batch, double* ms);
+/* Number of graph captures the batch has made. */
+int ggpu_gw_batch_captures(ggpu_gw_batch* batch, int* captures);
+void ggpu_gw_batch_destroy(ggpu_gw_batch* batch);
 
 /* Intermediates of one IBZ q for error reports and tests:
  * which = 0: P0(tau)   (nts, NQ, NQ)   bare bubble, scaled by 1/nk and mirrored

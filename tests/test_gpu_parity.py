@@ -467,3 +467,72 @@ def test_graph_replay_matches_eager(ggpu, synth, oracle):
     with pytest.raises(g.GgpuError) as e:
         host.solve(G)
     assert e.value.code == 5 and "resident in HBM" in str(e.value)
+
+
+@pytest.mark.parametrize("precision,rtol", [("fp32", 2e-5), ("bf16x3", 2e-4)])
+def test_hf_reduced_precision(ggpu, synth, oracle, fx, precision, rtol):
+    """Exchange products in FP32 / split-bf16 x3 with FP64 partial sums, J and
+    Ewald: K and the full Fock matrix stay within the arithmetic's relative
+    error of the FP64 oracle, on a synthetic system and on the reference's HF
+    fixture (whose J and Ewald terms are then exact to FP64)."""
+    sym, tau, T_wt, T_tw, G, D, V = gw_system(ggpu, synth, (2, 2, 2), 26, 100, 2, 50.0, 1e3, 102)
+    print(f"HF {precision} nk=8 nao=26 NQ=100 ns=2", flush=True)
+    k = ggpu.HfKernel(26, 2, 100, sym, precision=precision)
+    k.load_integrals(V)
+    K = k.exchange(D)
+    K_or = oracle.hf_exchange(D, V, oracle.symmetry_none((2, 2, 2), 26, 100), 2)
+    err, scale = max_abs(K, K_or), np.abs(K_or).max()
+    print(f"{precision}: K max-abs {err:.3e}, |K| max {scale:.3e}, rel {err / scale:.2e}", flush=True)
+    assert 0 < err < rtol * scale
+    NQ, ns, nao = int(fx["params_NQ"]), int(fx["params_ns"]), int(fx["params_nao"])
+    fsym = oracle.symmetry_from_fixture(fx, NQ)
+    S_ibz = fx["S_k"][:, fsym.k_reduced_to_full]
+    dm = fx["hf_G_tau"][-1]
+    h = ggpu.HfKernel(nao, ns, NQ, fsym, madelung=float(fx["madelung"]), S_k=S_ibz, precision=precision)
+    h.load_integrals(fx["hf_VQ_0"])
+    F = h.solve(dm)
+    F_or = oracle.hf_solve(dm, fx["hf_VQ_0"], fsym, ns, S_ibz, float(fx["madelung"]))
+    Kf = oracle.hf_exchange(dm, fx["hf_VQ_0"], fsym, ns)
+    err = max_abs(F, F_or)
+    print(f"{precision}: HF fixture F max-abs {err:.3e}, |K| max {np.abs(Kf).max():.3e}", flush=True)
+    assert err < rtol * np.abs(Kf).max()
+
+
+def test_batch_graph_uneven_systems(ggpu, synth, oracle):
+    """Systems of uneven nao / NQ replayed as one HIP graph launch (every
+    plan's iteration a branch of the graph): each system's Sigma equals its
+    own eager iteration bit for bit on every replay, and a change to one plan
+    makes the next run capture again."""
+    mesh, ns = (2, 1, 1), 2
+    shapes = [(5, 12), (9, 30), (13, 41)]
+    tau, T_wt, T_tw = synth.load_ir(50.0, 1e3)
+    sym = ggpu.mesh_symmetry_none(mesh)
+    kerns, Gd, Sd, ref = [], [], [], []
+    for i, (nao, NQ) in enumerate(shapes):
+        G, _ = synth.make_green(sym.nk, nao, ns, 50.0, tau, 70 + i)
+        V = synth.make_integrals(sym.nk, NQ, nao, 80 + i)
+        k = ggpu.GwKernel(nao, ns, NQ, T_wt, T_tw, sym, solver="cholesky")
+        k.load_integrals(V)
+        ref.append(k.solve(G))
+        osym = oracle.symmetry_none(mesh, nao, NQ)
+        assert max_abs(ref[-1], oracle.gw_solve(G, V, osym, T_wt, T_tw, solver="cholesky")) < TOL
+        kerns.append(k)
+        Gd.append(ggpu.DeviceArray.from_numpy(G))
+        Sd.append(ggpu.DeviceArray.empty_like(Gd[-1]))
+    print(f"batch graph of {len(shapes)} systems {shapes}", flush=True)
+    b = ggpu.GwBatch(kerns, [g.ptr for g in Gd], [s.ptr for s in Sd])
+    for rep in range(3):
+        ms = b.run()
+        assert ms > 0
+        for i in range(len(shapes)):
+            assert np.array_equal(Sd[i].numpy(), ref[i]), f"replay {rep}, system {i}"
+        kerns[0].solve(synth.make_green(sym.nk, shapes[0][0], ns, 50.0, tau, 99)[0])   # eager work between
+    assert b.captures() == 1
+    kerns[1].set_q_share(0, 1)
+    b.run()
+    assert b.captures() == 2
+    for i in range(len(shapes)):
+        assert np.array_equal(Sd[i].numpy(), ref[i])
+    b.close()
+    for x in Gd + Sd:
+        x.free()
