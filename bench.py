@@ -123,6 +123,9 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the built-in launch variants instead of autotuning each plan before the warmup")
     ap.add_argument("--tune-rounds", type=int, default=2)
+    ap.add_argument("--graph-modes", default=None,
+                    help="batched workload: modes (comma-separated) also timed as one captured graph launch "
+                         "(default: every mode, when the integrals are resident in HBM)")
     ap.add_argument("--graph", action="store_true",
                     help="time the iteration as one captured HIP graph replay (needs --integrals device); "
                          "per-site times then come from one profiled eager iteration before the timed steps")
@@ -381,12 +384,13 @@ def run_batch(a, w, pkg, synth):
         t0 = time.perf_counter()
         for i, k in enumerate(kerns):
             k.solve_device(gp[i], sp[i])
+            log(f"[{mode}] system {i} alone done")
         sequential = time.perf_counter() - t0
         log(f"[{mode}] batched {batched:.2f}s, one after another {sequential:.2f}s")
         res[mode] = {"batched_s": batched, "sequential_s": sequential, "tflops": fl / batched / 1e12,
                      "autotune_ms": tune_ms}
         sig[mode] = [x.numpy() for x in Sd]
-        if a.integrals == "device":
+        if a.integrals == "device" and (a.graph_modes is None or mode in a.graph_modes.split(",")):
             # all systems as one captured HIP graph (one launch per step)
             b = pkg.GwBatch(kerns, gp, sp)
             t = time.perf_counter()
