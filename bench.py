@@ -210,7 +210,8 @@ def main():
         if not a.graph:
             sites = kern.site_stats()
         runs[mode] = {"s": dt, "sites": sites, "phases": kern.timings(),
-                      "streamed": kern.streamed_bytes(), "generated": kern.generated_bytes(), "autotune": tune}
+                      "streamed": kern.streamed_bytes(), "generated": kern.generated_bytes(), "autotune": tune,
+                      "staging_ms": kern.staging_ms()}
         sigmas[mode] = Sd.numpy()
         Sd.free()
         # P0(tau) of the first IBZ q, for the per-mode P0 error
@@ -315,6 +316,10 @@ def main():
         "integrals": {"residence": a.integrals,
                       "h2d_gb_per_iteration": round(r["streamed"] / 1e9, 2),
                       "h2d_gb_per_s_average": round(r["streamed"] / 1e9 / r["s"], 2),
+                      # rate of the copies (or generation) while the copy stream runs them
+                      "staging_s": round(r["staging_ms"] / 1e3, 3),
+                      "staging_gb_per_s": (round((r["streamed"] + r["generated"]) / 1e9 / (r["staging_ms"] / 1e3), 2)
+                                           if r["staging_ms"] > 0 else None),
                       "generated_gb_per_iteration": round(r["generated"] / 1e9, 2)},
         "phases_ms": {k: round(v, 1) for k, v in r["phases"].items()},
         "sites": {k: {"ms": round(v[0], 1), "tflops": round(v[1] / (v[0] * 1e-3) / 1e12, 2), "launches": v[2]}

@@ -63,6 +63,7 @@ def lib():
         L.ggpu_gw_set_integral_residence.argtypes = [vp, i32]
         L.ggpu_gw_streamed_bytes.argtypes = [vp, pd]
         L.ggpu_gw_generated_bytes.argtypes = [vp, pd]
+        L.ggpu_gw_staging_ms.argtypes = [vp, pd]
         L.ggpu_gw_set_q_share.argtypes = [vp, i32, i32]
         L.ggpu_gw_load_integrals.argtypes = [vp, pd]
         L.ggpu_gw_synth_integrals.argtypes = [vp, u64, f64]
@@ -98,7 +99,7 @@ EXPORTED = ["ggpu_last_error", "ggpu_device_count", "ggpu_device_memory",
             "ggpu_hf_synth_integrals",
             "ggpu_hf_exchange", "ggpu_hf_solve", "ggpu_hf_plan_destroy",
             "ggpu_gw_plan_create", "ggpu_gw_set_integral_residence", "ggpu_gw_streamed_bytes",
-            "ggpu_gw_generated_bytes", "ggpu_gw_set_q_share",
+            "ggpu_gw_generated_bytes", "ggpu_gw_staging_ms", "ggpu_gw_set_q_share",
             "ggpu_gw_load_integrals", "ggpu_gw_synth_integrals",
             "ggpu_gw_iterate", "ggpu_gw_iterate_device", "ggpu_gw_iterate_many", "ggpu_gw_polarization",
             "ggpu_gw_last_timings", "ggpu_gw_set_profiling", "ggpu_gw_site_stats",
@@ -519,6 +520,12 @@ class GwKernel:
         """Integral bytes generated on the device by the last iteration."""
         b = C.c_double()
         _check(lib().ggpu_gw_generated_bytes(self._h, C.byref(b)))
+        return b.value
+
+    def staging_ms(self):
+        """Copy-stream time (ms) of the last iteration's integral staging."""
+        b = C.c_double()
+        _check(lib().ggpu_gw_staging_ms(self._h, C.byref(b)))
         return b.value
 
     def set_q_share(self, rank, size):

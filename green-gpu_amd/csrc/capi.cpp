@@ -175,6 +175,13 @@ int ggpu_gw_generated_bytes(ggpu_gw_plan* plan, double* bytes) {
   });
 }
 
+int ggpu_gw_staging_ms(ggpu_gw_plan* plan, double* ms) {
+  return guarded([&] {
+    ggpu::require(plan && plan->impl && ms, "NULL argument");
+    *ms = plan->impl->staging_ms_last;
+  });
+}
+
 int ggpu_gw_set_q_share(ggpu_gw_plan* plan, int rank, int size) {
   return guarded([&] {
     ggpu::require(plan && plan->impl, "NULL plan");

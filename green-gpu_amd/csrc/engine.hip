@@ -140,6 +140,7 @@ PairView SymTables::view(int64_t k1, int64_t k2) const {
 // ---------------------------------------------------------------------------
 IntegralStore::~IntegralStore() {
   if (copy_) (void)hipStreamSynchronize(copy_);
+  for (auto e : tpool_) (void)hipEventDestroy(e);
   for (int b = 0; b < 2; ++b) {
     if (ev_copy_[b]) (void)hipEventDestroy(ev_copy_[b]);
     if (ev_used_[b]) (void)hipEventDestroy(ev_used_[b]);
@@ -235,6 +236,12 @@ void IntegralStore::synth(const SymTables& t, int NQ, int nao, uint64_t seed, do
 void IntegralStore::issue(const PairTable& tab, int i0, int n, int buf) {
   require(n <= slots_, "integrals: batch larger than the staging area");
   GGPU_HIP(hipStreamWaitEvent(copy_, ev_used_[buf], 0));
+  while (tpool_.size() < tused_ + 2) {
+    hipEvent_t e;
+    GGPU_HIP(hipEventCreate(&e));
+    tpool_.push_back(e);
+  }
+  GGPU_HIP(hipEventRecord(tpool_[tused_], copy_));
   const size_t bytes = (size_t)pair_elems_ * sizeof(z64);
   for (int i = 0; i < n; ++i) {
     const int64_t pos = tab.host[i0 + i].pos;
@@ -246,6 +253,8 @@ void IntegralStore::issue(const PairTable& tab, int i0, int n, int buf) {
       GGPU_HIP(hipMemcpyAsync(dst, host_ + pos * pair_elems_, bytes, hipMemcpyHostToDevice, copy_));
   }
   GGPU_HIP(hipEventRecord(ev_copy_[buf], copy_));
+  GGPU_HIP(hipEventRecord(tpool_[tused_ + 1], copy_));
+  tused_ += 2;
   (mode_ == kResGenerated ? generated_ : streamed_) += (double)bytes * n;
 }
 
@@ -277,6 +286,18 @@ const z64* IntegralStore::view(const PairTable& tab, int i0, int n, hipStream_t 
   next_ = buf ^ 1;
   *offsets = stage_off_.get() + (int64_t)buf * slots_;
   return stage_.get();
+}
+
+double IntegralStore::collect_staging_ms() {
+  double ms = 0;
+  if (copy_) GGPU_HIP(hipStreamSynchronize(copy_));
+  for (size_t i = 0; i + 1 < tused_; i += 2) {
+    float t = 0;
+    GGPU_HIP(hipEventElapsedTime(&t, tpool_[i], tpool_[i + 1]));
+    ms += t;
+  }
+  tused_ = 0;
+  return ms;
 }
 
 void IntegralStore::done(hipStream_t s) {
@@ -1000,6 +1021,7 @@ void GwPlan::autotune(const z64* G_dev, int rounds, int batches) {
   float w = 0;
   GGPU_HIP(hipEventElapsedTime(&w, e0, e1));
   tune_wall_ms = w;
+  (void)store.collect_staging_ms();
   for (int s = 0; s < kNumSites; ++s) {
     variants[s] = -1;
     if (!tunable_site(s)) continue;
@@ -1093,7 +1115,7 @@ void GwPlan::iterate_device(const z64* G_dev, z64* S_dev) {
     GGPU_HIP(hipEventElapsedTime(&tot, ev[0], ev[6]));
     last_ms[0] = tot;
     for (int i = 1; i < 6; ++i) last_ms[i] = 0;
-    streamed_last = generated_last = 0;
+    streamed_last = generated_last = staging_ms_last = 0;
     check_info();
     return;
   }
@@ -1101,6 +1123,7 @@ void GwPlan::iterate_device(const z64* G_dev, z64* S_dev) {
   VariantGuard vguard(variants);
   if (profiling) prof.reset();
   const double streamed0 = store.bytes_streamed(), generated0 = store.bytes_generated();
+  (void)store.collect_staging_ms();
   enqueue_iteration(G_dev, S_dev, true);
   GGPU_HIP(hipStreamSynchronize(stream));
   double t_p0 = 0, t_dy = 0, t_sig = 0;
@@ -1118,6 +1141,7 @@ void GwPlan::iterate_device(const z64* G_dev, z64* S_dev) {
   last_ms[0] = tot; last_ms[1] = g; last_ms[2] = t_p0; last_ms[3] = t_dy; last_ms[4] = t_sig; last_ms[5] = sg;
   streamed_last = store.bytes_streamed() - streamed0;
   generated_last = store.bytes_generated() - generated0;
+  staging_ms_last = store.collect_staging_ms();
   if (profiling) prof.collect();
   check_info();
 }
@@ -1248,6 +1272,7 @@ void GwPlan::polarization(const double* G, int iq, int which, double* out) {
         o[(size_t)t * Q2 + e] = z64{v.x * sc, v.y * sc};
         o[(size_t)(nts - 1 - t) * Q2 + e] = z64{v.x * sc, v.y * sc};
       }
+    (void)store.collect_staging_ms();
     return;
   }
   dyson_for_q();
@@ -1255,6 +1280,7 @@ void GwPlan::polarization(const double* G, int iq, int which, double* out) {
   const size_t cnt = (which == 1 ? (size_t)nw_b : (size_t)nts) * Q2;
   GGPU_HIP(hipMemcpyAsync(out, src, cnt * sizeof(z64), hipMemcpyDeviceToHost, stream));
   GGPU_HIP(hipStreamSynchronize(stream));
+  (void)store.collect_staging_ms();
   check_info();
 }
 
@@ -1424,6 +1450,7 @@ void HfPlan::stage_dm() {
 
 void HfPlan::fetch(double* out) {
   const int64_t n2 = (int64_t)nao * nao;
+  (void)store.collect_staging_ms();   // staging timers are not reported for HF
   if (x2c) {
     spinor_copy(x2c_merge, Kout.get(), Kout.size(), nao, 0, F2c.get(), F2c.size(), nso, 0, nao, 1, 1.0, stream);
     GGPU_HIP(hipMemcpyAsync(out, F2c.get(), F2c.bytes(), hipMemcpyDeviceToHost, stream));

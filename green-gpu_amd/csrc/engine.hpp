@@ -63,9 +63,15 @@ class IntegralStore {
   // done(s) after the last read of the view to release the buffer.
   const z64* view(const PairTable& tab, int i0, int n, hipStream_t s, const int64_t** offsets);
   void done(hipStream_t s);
+  // Device time (ms) the copy stream spent copying (or generating) staged
+  // batches since the last call, from an event pair around every batch;
+  // call after the streams have drained.
+  double collect_staging_ms();
 
  private:
   void alloc_host(int64_t n_elems);
+  std::vector<hipEvent_t> tpool_;          // (start, end) event pairs of staged batches
+  size_t tused_ = 0;
   void issue(const PairTable& tab, int i0, int n, int buf);
   DeviceBuffer<z64> V_;
   z64* host_ = nullptr;
@@ -178,6 +184,7 @@ struct GwPlan {
   double last_ms[6] = {0, 0, 0, 0, 0, 0};
   double streamed_last = 0;             // integral bytes copied host -> device by the last iterate
   double generated_last = 0;            // integral bytes generated on the device by the last iterate
+  double staging_ms_last = 0;           // copy-stream time of those copies / generation (ms)
   // This device's share of the IBZ q-points: iq = share_rank, share_rank +
   // share_size, ... (the partial Sigma of all shares sums to the full one).
   int share_rank = 0, share_size = 1;

@@ -189,6 +189,12 @@ int ggpu_gw_set_integral_residence(ggpu_gw_plan* plan, int residence);
 int ggpu_gw_streamed_bytes(ggpu_gw_plan* plan, double* bytes);
 /* Bytes of integrals generated on the device by the last iterate (GENERATED residence). */
 int ggpu_gw_generated_bytes(ggpu_gw_plan* plan, double* bytes);
+/* Device time (ms) the plan's copy stream spent staging those integrals in
+ * the last iterate: the host -> device copies (HOST residence) or the
+ * generation kernels (GENERATED), an event pair around every pair batch;
+ * bytes / time is the rate the staging runs at while it runs.  0 when the
+ * integrals are resident in HBM. */
+int ggpu_gw_staging_ms(ggpu_gw_plan* plan, double* ms);
 /* This device's share of the IBZ q-points: iterations then run
  * iq = rank, rank + size, ... < inq only, and Sigma holds that share's
  * contribution (still divided by nk), so the Sigma of shares 0 .. size-1

@@ -206,10 +206,12 @@ def test_integrals_streamed_from_host(ggpu, synth, oracle):
         k.load_integrals(V)
         out[where] = k.solve(G)
         out[where + "_bytes"] = k.streamed_bytes()
+        out[where + "_ms"] = k.staging_ms()
         out[where + "_2"] = k.solve(G)
     assert np.array_equal(out["device"], out["host"])
     assert np.array_equal(out["host"], out["host_2"])
     assert out["device_bytes"] == 0
+    assert out["device_ms"] == 0 and out["host_ms"] > 0
     nk = sym.nk
     assert out["host_bytes"] == pair_bytes * nk * (nk + nk)     # P0: nk pairs per q; Sigma: nk per q
     osym = oracle.symmetry_none(mesh, nao, NQ)
