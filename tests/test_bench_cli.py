@@ -24,3 +24,11 @@ def test_unknown_workload_is_rejected():
     r = _run("--workload", "no_such_workload")
     assert r.returncode != 0
     assert "invalid choice" in r.stderr
+
+
+def test_graph_replay_needs_resident_integrals():
+    """A captured iteration replays fixed device addresses, so --graph is
+    refused for integrals streamed from host memory (the workload's default)."""
+    r = _run("--graph")
+    assert r.returncode != 0
+    assert "--integrals device" in r.stderr
