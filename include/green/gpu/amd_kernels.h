@@ -54,6 +54,17 @@ namespace green::gpu {
       throw std::runtime_error(msg);
     }
 
+    // Arithmetic of the contractions: the "gpu_precision" parameter (fp64,
+    // fp32 or bf16x3) when set, else `fallback`.
+    inline int precision_param(const params::params& p, int fallback) {
+      const std::string s = p["gpu_precision"].as<std::string>();
+      if (s.empty()) return fallback;
+      if (s == "fp64") return GGPU_PRECISION_FP64;
+      if (s == "fp32") return GGPU_PRECISION_FP32;
+      if (s == "bf16x3") return GGPU_PRECISION_BF16X3;
+      throw std::runtime_error("gpu_precision must be fp64, fp32 or bf16x3, not \"" + s + "\"");
+    }
+
     // Owned copies of every table ggpu_symmetry points to
     // (the data make_cu_symmetry_data gathers, gw_gpu_kernel.cpp:34-91,
     // plus the k-pair lists df_integral_t uses, df_integral_t.h:68-86).
@@ -157,6 +168,8 @@ namespace green::gpu {
         check(ggpu_hf_plan_create(&h, static_cast<int>(nao), static_cast<int>(nso), static_cast<int>(ns),
                                   static_cast<int>(NQ), tables_.get()));
         plan_.reset(h);
+        // the reference's exchange is FP64 (hf_gpu_kernel.h:67-84)
+        check(ggpu_hf_set_precision(plan_.get(), precision_param(p, GGPU_PRECISION_FP64)));
         df_reader reader(p["dfintegral_hf_file"].as<std::string>(), nao, NQ);
         const auto V = reader.read_entire(tables_.num_stored());
         check(ggpu_hf_load_integrals(plan_.get(), reinterpret_cast<const double*>(V.data())));
@@ -191,7 +204,7 @@ namespace green::gpu {
         const int    nts  = static_cast<int>(ft.sd().repn_fermi().nts());
         const int    nw_b = static_cast<int>(ft.sd().repn_bose().nw());
         const bool   sp   = p["P_sp"].as<bool>() && p["Sigma_sp"].as<bool>();
-        const int    prec = sp ? GGPU_PRECISION_FP32 : GGPU_PRECISION_FP64;
+        const int    prec = precision_param(p, sp ? GGPU_PRECISION_FP32 : GGPU_PRECISION_FP64);
         const int    lin  = solver == LinearSolverType::Cholesky ? GGPU_SOLVER_CHOLESKY : GGPU_SOLVER_LU;
         ggpu_gw_plan* h = nullptr;
         check(ggpu_gw_plan_create(&h, static_cast<int>(nao), static_cast<int>(nso), static_cast<int>(ns),
@@ -202,20 +215,28 @@ namespace green::gpu {
         df_reader reader(p["dfintegral_file"].as<std::string>(), nao, NQ);
         const auto V = reader.read_entire(tables_.num_stored());
         check(ggpu_gw_load_integrals(plan_.get(), reinterpret_cast<const double*>(V.data())));
+        if (p["gpu_graph"].as<bool>()) check(ggpu_gw_set_graph(plan_.get(), 1));
+        autotune_ = p["gpu_autotune"].as<bool>();
       }
 
       template <typename G_type, typename S_type>
       void solve(G_type& g, S_type& sigma) {
         sigma.fence();
-        if (!utils::context().node_rank)
-          check(ggpu_gw_iterate(plan_.get(), reinterpret_cast<const double*>(g.object().data()),
-                                reinterpret_cast<double*>(sigma.object().data())));
+        if (!utils::context().node_rank) {
+          const double* G = reinterpret_cast<const double*>(g.object().data());
+          if (autotune_) {   // once, on the first iteration's own G
+            check(ggpu_gw_autotune(plan_.get(), G, 2, 1));
+            autotune_ = false;
+          }
+          check(ggpu_gw_iterate(plan_.get(), G, reinterpret_cast<double*>(sigma.object().data())));
+        }
         sigma.fence();
       }
 
     private:
       std::unique_ptr<ggpu_gw_plan, void (*)(ggpu_gw_plan*)> plan_{nullptr, ggpu_gw_plan_destroy};
       symmetry_tables tables_;
+      bool            autotune_ = false;
     };
 
   }  // namespace amd

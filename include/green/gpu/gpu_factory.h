@@ -35,7 +35,9 @@ namespace green::gpu {
   /**
    * GW kernel extension (reference: gpu_factory.h:78-89).  The linear solver
    * comes from the "cuda_linear_solver" parameter and the precision from
-   * P_sp && Sigma_sp (gw_gpu_kernel.h:64), as in the reference.
+   * P_sp && Sigma_sp (gw_gpu_kernel.h:64), as in the reference, unless
+   * "gpu_precision" names one; "gpu_autotune" and "gpu_graph" switch on the
+   * launch-variant tuning and graph replay.
    */
   inline std::tuple<std::shared_ptr<void>, std::function<void(G_type&, G_type&)>> custom_gw_kernel(
       bool X2C, const params::params& p, size_t nao, size_t nso, size_t ns, size_t NQ, const grids::transformer_t& ft,
@@ -60,6 +62,14 @@ namespace green::gpu {
     p.define<bool>("cuda_low_gpu_memory", "accepted for compatibility (device batches are sized automatically)");
     p.define<bool>("cuda_low_cpu_memory", "accepted for compatibility (integrals are read once per kernel)");
     p.define<size_t>("nt_batch", "accepted for compatibility (tau batches are sized automatically)", 0);
+    // no counterpart in the reference: arithmetic, launch-variant tuning and
+    // graph replay of this implementation
+    p.define<std::string>("gpu_precision",
+                          "contraction arithmetic: fp64, fp32 or bf16x3 (split-bf16 x3 on the matrix cores); "
+                          "empty: fp32 for GW when P_sp and Sigma_sp, else fp64",
+                          "");
+    p.define<bool>("gpu_autotune", "time every GEMM launch variant on the first GW iteration, keep the fastest", false);
+    p.define<bool>("gpu_graph", "replay every GW iteration as one captured HIP graph", false);
   }
 
 }  // namespace green::gpu

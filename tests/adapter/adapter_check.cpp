@@ -147,7 +147,8 @@ int main(int argc, char** argv) {
   // 3. parameters
   green::params::params p;
   green::gpu::custom_kernel_parameters(p);
-  for (const char* k : {"verbose", "cuda_linear_solver", "cuda_low_gpu_memory", "cuda_low_cpu_memory", "nt_batch"})
+  for (const char* k : {"verbose", "cuda_linear_solver", "cuda_low_gpu_memory", "cuda_low_cpu_memory", "nt_batch",
+                        "gpu_precision", "gpu_autotune", "gpu_graph"})
     EXPECT(p.defined(k), std::string("parameter ") + k);
   p.set("dfintegral_hf_file", df);
 
