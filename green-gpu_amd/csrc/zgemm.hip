@@ -606,6 +606,216 @@ __global__ __launch_bounds__(NTHREADS) __attribute__((amdgpu_waves_per_eu(WPE)))
       }
 }
 
+// A-stationary split-bf16 x3 kernel, for K <= 2 XBK, no K-batch and one A
+// shared by every batch (the G.V products of the P0 and Sigma chains multiply
+// one V by every tau).  A workgroup splits its A tile once, keeps its waves'
+// A fragments (four bf16 planes, both k-steps) in registers and walks `bpw`
+// consecutive batches, staging only B per batch (the next B is loaded while
+// the current one is multiplied).  Per output element the products are
+// accumulated in the order of gemm_bf16x3_big_kernel, so the sums are equal.
+template <class TO, bool AKC, bool BKC, int SITE>
+__global__ __launch_bounds__(NTHREADS) __attribute__((amdgpu_waves_per_eu(2))) void gemm_bf16x3_astat_kernel(
+    const GemmDesc d, int bpw) {
+  constexpr int WMB = 2, WNB = 2;
+  constexpr int TM = 32 * WMB, TN = 32 * WNB, KT = 2 * XBK, LDK2 = KT + XPAD;
+  constexpr int GA = (TM * (KT / 8) + NTHREADS - 1) / NTHREADS;   // 8-element k-groups per thread
+  constexpr int GB = (TN * (KT / 8) + NTHREADS - 1) / NTHREADS;
+  __shared__ __attribute__((aligned(16))) unsigned short sA[4][TM][LDK2];
+  __shared__ __attribute__((aligned(16))) unsigned short sB[4][TN][LDK2];
+
+  const int tid = threadIdx.x;
+  const int tiles_m = (d.M + TM - 1) / TM;
+  const int tiles_n = (d.N + TN - 1) / TN;
+  const int64_t nbatch = (int64_t)d.nb0 * d.nb1;
+  // m-tile fastest: the workgroups sharing a run of B tiles sit side by side
+  int64_t lg = xcd_logical(blockIdx.x, gridDim.x);
+  const int mt = (int)(lg % tiles_m);
+  lg /= tiles_m;
+  const int nt = (int)(lg % tiles_n);
+  const int64_t chunk = lg / tiles_n;
+  const int m0 = mt * TM, n0 = nt * TN;
+  const float2* A = static_cast<const float2*>(d.A.ptr);
+
+  constexpr int GPR = KT / 8;   // groups per row
+  int arow[GA], ak0[GA], brow[GB], bk0[GB];
+#pragma unroll
+  for (int u = 0; u < GA; ++u) {
+    const int g = tid + u * NTHREADS;
+    arow[u] = AKC ? (g / GPR) : (g % TM);
+    ak0[u] = AKC ? ((g % GPR) << 3) : ((g / TM) << 3);
+    if (g >= TM * GPR) arow[u] = -1;
+  }
+#pragma unroll
+  for (int u = 0; u < GB; ++u) {
+    const int g = tid + u * NTHREADS;
+    brow[u] = BKC ? (g / GPR) : (g % TN);
+    bk0[u] = BKC ? ((g % GPR) << 3) : ((g / TN) << 3);
+    if (g >= TN * GPR) brow[u] = -1;
+  }
+  auto split8 = [&](unsigned short (*S)[LDK2], int rows, int row, int k0, const float2* x, bool conj) {
+    unsigned hr[4], hi[4], lr_[4], li[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float sg = conj ? -1.f : 1.f;
+      split2(x[2 * q].x, x[2 * q + 1].x, hr[q], lr_[q]);
+      split2(sg * x[2 * q].y, sg * x[2 * q + 1].y, hi[q], li[q]);
+    }
+    *reinterpret_cast<v4u*>(&S[0 * rows + row][k0]) = v4u{hr[0], hr[1], hr[2], hr[3]};
+    *reinterpret_cast<v4u*>(&S[1 * rows + row][k0]) = v4u{hi[0], hi[1], hi[2], hi[3]};
+    *reinterpret_cast<v4u*>(&S[2 * rows + row][k0]) = v4u{lr_[0], lr_[1], lr_[2], lr_[3]};
+    *reinterpret_cast<v4u*>(&S[3 * rows + row][k0]) = v4u{li[0], li[1], li[2], li[3]};
+  };
+
+  // A tile: load, split, keep the fragments in registers
+  {
+    float2 ra[GA][8];
+#pragma unroll
+    for (int u = 0; u < GA; ++u) {
+      const int gm = m0 + arow[u];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int gk = ak0[u] + j;
+        ra[u][j] = (arow[u] >= 0 && gm < d.M && gk < d.K) ? A[gm * d.A.rs + gk * d.A.cs] : float2{0.f, 0.f};
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < GA; ++u)
+      if (arow[u] >= 0) split8(&sA[0][0], TM, arow[u], ak0[u], ra[u], d.A.conj != 0);
+  }
+  __syncthreads();
+  const int lane = tid & 63, w = tid >> 6;
+  const int wm = (w >> 1) * (16 * WMB), wn = (w & 1) * (16 * WNB);
+  const int lr = lane & 15, lk = lane >> 4;
+  const int ksteps = (d.K + XBK - 1) / XBK;   // 1 or 2
+  v4u a_rh[WMB][2], a_ih[WMB][2], a_rl[WMB][2], a_il[WMB][2];
+#pragma unroll
+  for (int i = 0; i < WMB; ++i)
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      const int ra_ = wm + 16 * i + lr, kc = ks * XBK + 8 * lk;
+      a_rh[i][ks] = *reinterpret_cast<const v4u*>(&sA[0][ra_][kc]);
+      a_ih[i][ks] = *reinterpret_cast<const v4u*>(&sA[1][ra_][kc]);
+      a_rl[i][ks] = *reinterpret_cast<const v4u*>(&sA[2][ra_][kc]);
+      a_il[i][ks] = *reinterpret_cast<const v4u*>(&sA[3][ra_][kc]);
+    }
+  const v4u sign = v4u{0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
+
+  const int64_t b_first = chunk * bpw;
+  const int nb = (int)(nbatch - b_first < bpw ? nbatch - b_first : bpw);
+  float2 rb[GB][8];
+  auto gload_b = [&](int j) {
+    const int64_t b = b_first + j, b0 = b / d.nb1, b1 = b % d.nb1;
+    const float2* B = static_cast<const float2*>(d.B.ptr) + b0 * d.B.bs0 + (d.B.off1 ? d.B.off1[b1] : b1 * d.B.bs1);
+#pragma unroll
+    for (int u = 0; u < GB; ++u) {
+      const int gn = n0 + brow[u];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int gk = bk0[u] + q;
+        rb[u][q] = (brow[u] >= 0 && gn < d.N && gk < d.K) ? B[gk * d.B.rs + gn * d.B.cs] : float2{0.f, 0.f};
+      }
+    }
+  };
+  const bool use_beta = (d.beta_re != 0.0 || d.beta_im != 0.0);
+  if (nb > 0) gload_b(0);
+  for (int j = 0; j < nb; ++j) {
+    __syncthreads();   // every wave is done with the previous B (and, first time, with sA)
+#pragma unroll
+    for (int u = 0; u < GB; ++u)
+      if (brow[u] >= 0) split8(&sB[0][0], TN, brow[u], bk0[u], rb[u], d.B.conj != 0);
+    __syncthreads();
+    if (j + 1 < nb) gload_b(j + 1);
+    v4f cr[WMB][WNB], ci[WMB][WNB];
+#pragma unroll
+    for (int i = 0; i < WMB; ++i)
+#pragma unroll
+      for (int q = 0; q < WNB; ++q) { cr[i][q] = v4f{0, 0, 0, 0}; ci[i][q] = v4f{0, 0, 0, 0}; }
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      if (ks >= ksteps) break;   // uniform
+      v8bf brh[WNB], bih[WNB], brl[WNB], bil[WNB];
+#pragma unroll
+      for (int q = 0; q < WNB; ++q) {
+        const int rb_ = wn + 16 * q + lr, kc = ks * XBK + 8 * lk;
+        brh[q] = __builtin_bit_cast(v8bf, *reinterpret_cast<const v4u*>(&sB[0][rb_][kc]));
+        bih[q] = __builtin_bit_cast(v8bf, *reinterpret_cast<const v4u*>(&sB[1][rb_][kc]));
+        brl[q] = __builtin_bit_cast(v8bf, *reinterpret_cast<const v4u*>(&sB[2][rb_][kc]));
+        bil[q] = __builtin_bit_cast(v8bf, *reinterpret_cast<const v4u*>(&sB[3][rb_][kc]));
+      }
+#pragma unroll
+      for (int i = 0; i < WMB; ++i) {
+        const v8bf arh = __builtin_bit_cast(v8bf, a_rh[i][ks]), arl = __builtin_bit_cast(v8bf, a_rl[i][ks]);
+        const v8bf aih = __builtin_bit_cast(v8bf, a_ih[i][ks]), ail = __builtin_bit_cast(v8bf, a_il[i][ks]);
+        const v8bf naih = __builtin_bit_cast(v8bf, a_ih[i][ks] ^ sign), nail = __builtin_bit_cast(v8bf, a_il[i][ks] ^ sign);
+#define GGPU_X3_STEP(acc, a, b)                                                       \
+  _Pragma("unroll") for (int q = 0; q < WNB; ++q) acc[i][q] =                         \
+      __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[q], acc[i][q], 0, 0, 0);
+        GGPU_X3_STEP(cr, arh, brh) GGPU_X3_STEP(ci, arh, bih)
+        GGPU_X3_STEP(cr, arh, brl) GGPU_X3_STEP(ci, arh, bil)
+        GGPU_X3_STEP(cr, arl, brh) GGPU_X3_STEP(ci, arl, bih)
+        GGPU_X3_STEP(cr, naih, bih) GGPU_X3_STEP(ci, aih, brh)
+        GGPU_X3_STEP(cr, naih, bil) GGPU_X3_STEP(ci, aih, brl)
+        GGPU_X3_STEP(cr, nail, bih) GGPU_X3_STEP(ci, ail, brh)
+#undef GGPU_X3_STEP
+      }
+    }
+    const int64_t b = b_first + j, b0 = b / d.nb1, b1 = b % d.nb1;
+    TO* C = static_cast<TO*>(d.C.ptr) + b0 * d.C.bs0 + (d.C.off1 ? d.C.off1[b1] : b1 * d.C.bs1);
+#pragma unroll
+    for (int i = 0; i < WMB; ++i)
+#pragma unroll
+      for (int q = 0; q < WNB; ++q)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = m0 + wm + 16 * i + 4 * lk + r;
+          const int col = n0 + wn + 16 * q + lr;
+          if (row < d.M && col < d.N) {
+            const double xr = cr[i][q][r], xi = ci[i][q][r];
+            double vr = d.alpha_re * xr - d.alpha_im * xi;
+            double vi = d.alpha_re * xi + d.alpha_im * xr;
+            TO* p = C + row * d.C.rs + col * d.C.cs;
+            if (use_beta) {
+              const TO o = *p;
+              vr += d.beta_re * o.x - d.beta_im * o.y;
+              vi += d.beta_re * o.y + d.beta_im * o.x;
+            }
+            using TS = decltype(TO{}.x);
+            *p = TO{static_cast<TS>(vr), static_cast<TS>(vi)};
+          }
+        }
+  }
+}
+
+// Whether the A-stationary kernel applies: one A for every batch, no
+// K-batch, K within its two k-steps, and several batches to walk.
+inline bool astat_fits(const GemmDesc& d) {
+  return d.A.bs0 == 0 && d.A.bs1 == 0 && d.A.off1 == nullptr && d.KB == 1 && d.K <= 2 * XBK &&
+         (int64_t)d.nb0 * d.nb1 >= 2;
+}
+
+template <class TO, int SITE>
+void launch_bf16x3_astat(const GemmDesc& d, hipStream_t stream) {
+  constexpr int TM = 64, TN = 64;
+  const int64_t tiles = (int64_t)((d.M + TM - 1) / TM) * ((d.N + TN - 1) / TN);
+  const int64_t nbatch = (int64_t)d.nb0 * d.nb1;
+  // batches per workgroup: as many as keep about 2048 workgroups in the grid
+  const int bpw = (int)std::max<int64_t>(1, std::min<int64_t>(16, tiles * nbatch / 2048));
+  const int64_t total = tiles * ((nbatch + bpw - 1) / bpw);
+  require(total < (int64_t)1 << 31, "gemm: grid too large");
+  dim3 grid((unsigned)total), block(NTHREADS);
+  const bool akc = (d.A.cs == 1) || (d.A.rs != 1);
+  const bool bkc = (d.B.rs == 1) || (d.B.cs != 1);
+  if (akc && bkc)
+    hipLaunchKernelGGL((gemm_bf16x3_astat_kernel<TO, true, true, SITE>), grid, block, 0, stream, d, bpw);
+  else if (akc)
+    hipLaunchKernelGGL((gemm_bf16x3_astat_kernel<TO, true, false, SITE>), grid, block, 0, stream, d, bpw);
+  else if (bkc)
+    hipLaunchKernelGGL((gemm_bf16x3_astat_kernel<TO, false, true, SITE>), grid, block, 0, stream, d, bpw);
+  else
+    hipLaunchKernelGGL((gemm_bf16x3_astat_kernel<TO, false, false, SITE>), grid, block, 0, stream, d, bpw);
+  GGPU_HIP(hipGetLastError());
+}
+
 template <class TO, int WMB, int WNB, int SITE, int WPE = 1>
 void launch_bf16x3_big(const GemmDesc& d, hipStream_t stream) {
   constexpr int TM = 32 * WMB, TN = 32 * WNB;
@@ -667,6 +877,9 @@ void launch_bf16x3_site(const GemmDesc& d, hipStream_t stream, int variant) {
     case kBf16Tile96x64W3: return launch_bf16x3_big<TO, 3, 2, SITE, 3>(d, stream);
     case kBf16Tile64x96W3: return launch_bf16x3_big<TO, 2, 3, SITE, 3>(d, stream);
     case kBf16Tile64x64W3: return launch_bf16x3_big<TO, 2, 2, SITE, 3>(d, stream);
+    case kBf16TileAstat64x64:
+      if (astat_fits(d)) return launch_bf16x3_astat<TO, SITE>(d, stream);
+      return launch_bf16x3_big<TO, 2, 2, SITE, 3>(d, stream);   // a 64 x 64 tile, A staged per batch
     default: break;
   }
   const int64_t tiles = (int64_t)((d.M + BM - 1) / BM) * ((d.N + BN - 1) / BN);
@@ -812,9 +1025,9 @@ int gemm_variant_candidates(Compute c, int* out, int cap) {
   static const int f64[] = {83, 8, 16};
   static const int f32[] = {8, 16, 83};
   static const int bf[] = {kBf16Tile128x64, kBf16Tile64x128, kBf16Tile160x64, kBf16Tile96x64W3,
-                           kBf16Tile64x96W3, kBf16Tile64x64W3, kBf16Tile64x64};
+                           kBf16Tile64x96W3, kBf16Tile64x64W3, kBf16Tile64x64, kBf16TileAstat64x64};
   const int* v = c == Compute::F64 ? f64 : c == Compute::F32 ? f32 : bf;
-  const int n = c == Compute::F64 ? 3 : c == Compute::F32 ? 3 : 7;
+  const int n = c == Compute::F64 ? 3 : c == Compute::F32 ? 3 : 8;
   for (int i = 0; i < n && i < cap; ++i) out[i] = v[i];
   return n;
 }

@@ -78,7 +78,11 @@ void set_gemm_profiler(GemmProfiler* p);
 // allows three workgroups per CU).
 enum {
   kBf16Tile64x64 = 0, kBf16Tile128x64 = 1, kBf16Tile64x128 = 4, kBf16Tile160x64 = 5,
-  kBf16Tile96x64W3 = 6, kBf16Tile64x96W3 = 7, kBf16Tile64x64W3 = 8
+  kBf16Tile96x64W3 = 6, kBf16Tile64x96W3 = 7, kBf16Tile64x64W3 = 8,
+  // 64 x 64, A split once and kept in registers while the workgroup walks
+  // several batches (one A for every batch, K <= 64, no K-batch; other
+  // launches take the 64 x 64 three-wave tile)
+  kBf16TileAstat64x64 = 9
 };
 void set_bf16_tiles(int mode);
 // Variant of the FP64 / FP32 kernel: 0 picks per call site; 8 or 16 forces
