@@ -27,6 +27,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "GW iteration wall-time (s) + TFLOP/s, mixed vs FP64 on MI355X at Σ err≤1e-6"
 PEAK = {"f64": 78.6, "f32": 157.3, "bf16x3": 2500.0}      # TFLOP/s, MI355X dense matrix
+# Matrix-core FLOPs executed per algorithmic FLOP (8 per complex multiply-add):
+# FP64 forms 3 real products per complex one (Gauss), FP32 4, split-bf16 x3
+# 4 real products of 3 bf16 products each.
+MFMA_PER_ALG = {"fp64": 0.75, "fp32": 1.0, "bf16x3": 3.0}
 HBM_PEAK = 8000.0                                           # GB/s
 
 WORKLOADS = {
@@ -58,6 +62,18 @@ def gw_flops(nk, nao, NQ, ns, nts, nw_b):
     ir = nk * 8.0 * NQ * NQ * (nw_b * (nts - 2) + nts * nw_b)
     dyson = nk * nw_b * (4.0 / 3.0 + 8.0) * NQ ** 3
     return {"p0": p0, "sigma": sig, "ir": ir, "dyson": dyson, "total": p0 + sig + ir + dyson}
+
+
+def mode_roofline(mode, sites):
+    """Dominant GEMM site of one precision mode against that mode's matrix
+    peak: algorithmic TFLOP/s, matrix-core TFLOP/s executed, and the
+    fraction of the peak the latter is."""
+    name, (ms, flops, n) = max(sites.items(), key=lambda kv: kv[1][0])
+    alg = flops / (ms * 1e-3) / 1e12
+    mfma = alg * MFMA_PER_ALG[mode]
+    peak = PEAK[{"fp64": "f64", "fp32": "f32", "bf16x3": "bf16x3"}[mode]]
+    return {"kernel": f"gemm site {name}", "achieved_algorithmic": round(alg, 2), "achieved_mfma": round(mfma, 2),
+            "peak": peak, "unit": "TFLOP/s", "frac_mfma": round(mfma / peak, 4)}
 
 
 def cpu_baseline(w, G, V_pairs, T_wt, T_tw, sym, npairs=2):
@@ -311,7 +327,8 @@ def main():
                       "p0_rel_err": runs[m]["p0_rel_err"],
                       "phases_ms": {k: round(v, 1) for k, v in runs[m]["phases"].items()},
                       "sites_ms": {k: round(v[0], 1) for k, v in runs[m]["sites"].items()},
-                      "autotune": runs[m]["autotune"]} for m in modes},
+                      "autotune": runs[m]["autotune"],
+                      "roofline": mode_roofline(m, runs[m]["sites"])} for m in modes},
         "sigma_abs_max": scale,
         "integrals": {"residence": a.integrals,
                       "h2d_gb_per_iteration": round(r["streamed"] / 1e9, 2),
