@@ -12,6 +12,13 @@
 // multiplied.  Global loads follow whichever operand dimension is contiguous
 // (template flags), so every operand layout of the HF/GW chains streams with
 // full 64-byte segments.
+//
+// Split-bf16 x3 has its own kernels further down: tiles from 64 x 64 to
+// 160 x 64 that split complex64 operands into bf16 hi / lo planes while
+// staging them, and an A-stationary 64 x 64 tile that splits one shared A
+// once and walks several batches.  Each call site is its own kernel symbol;
+// the launch variant per site comes from the built-in choice, a per-thread
+// table (the plan's autotuned choice) or a global override.
 #include "zgemm.hpp"
 
 #include <algorithm>
