@@ -1,18 +1,26 @@
 #!/usr/bin/env python3
 """GW-iteration benchmark on one MI355X.
 
-A step is one full GW iteration (P0 bubble for every q and k, tau -> Omega,
-Dyson solve, Omega -> tau, Sigma contraction, Sigma staging) on synthetic
-inputs already resident in HBM.  Default workload: nk = 4x4x4, nao = 52,
-NQ = 300, ns = 2, beta = 100, sparse-IR Lambda = 1e4 (the largest of the
-reference benchmark shapes whose integrals fit one GPU).
+Default workload (``gw_nk216_nao104_nq600``): nk = 6x6x6 = 216, nao = 104,
+NQ = 600, ns = 2, beta = 100, sparse-IR Lambda = 1e4 (nts = 110, nw_b = 99),
+synthetic inputs; the stored integral pairs (2.4 TB) fit neither HBM nor host
+memory, so each pair batch is generated on the device when it is needed.
 
-Every requested precision mode runs the same iteration on the same inputs;
-the FP64 result is the reference for each mode's Sigma error.  Prints one
-JSON line (rank 0).  `value` is the algorithmic TFLOP/s of the iteration
-(complex multiply-add = 8 FLOP, minimal contraction set) of the fastest mode
-whose Sigma stays within 1e-6 max-abs of FP64; every mode's time, rate,
-speedup and error are listed under "modes".
+A *step* is one q of that iteration: the P0 chain over all 216 k-pairs of the
+q, tau -> Omega, the Dyson solve of all 99 Omega, Omega -> tau and the Sigma
+chain over all 216 k.  Without symmetry every q costs the same, so one
+iteration is 216 steps; step i runs q = i and the iteration time is
+extrapolated.  A warm-up step is a partial q (the first pair batches of each
+chain).  Timed steps stop at --steps or when the mode's share of the wall-clock
+budget (--budget-s) is spent, with at least one step; "steps" reports the
+number actually timed.  The smaller workloads (``gw_nk64_nao52_nq300`` and
+``gw_nk8_nao26_nq100``) time whole iterations per step.
+
+Every requested precision mode runs on the same inputs; the FP64 run of q = 0
+is the reference for each mode's Sigma and P0 errors.  Prints one JSON line.
+``value`` is the algorithmic TFLOP/s (complex multiply-add = 8 FLOP, minimal
+contraction set) of the fastest mode whose Sigma stays within 1e-6 max-abs of
+FP64; every mode's time, rate, speedup and errors are listed under "modes".
 """
 import argparse
 import json
@@ -27,28 +35,32 @@ sys.path.insert(0, ROOT)
 
 METRIC = "GW iteration wall-time (s) + TFLOP/s, mixed vs FP64 on MI355X at Σ err≤1e-6"
 PEAK = {"f64": 78.6, "f32": 157.3, "bf16x3": 2500.0}      # TFLOP/s, MI355X dense matrix
+DTYPE = {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16x3"}
 # Matrix-core FLOPs executed per algorithmic FLOP (8 per complex multiply-add):
 # FP64 forms 3 real products per complex one (Gauss), FP32 4, split-bf16 x3
 # 4 real products of 3 bf16 products each.
 MFMA_PER_ALG = {"fp64": 0.75, "fp32": 1.0, "bf16x3": 3.0}
-HBM_PEAK = 8000.0                                           # GB/s
+SIGMA_BOUND = 1e-6                                          # Ha, max-abs against FP64
 
 WORKLOADS = {
     # the reference's small CPU-runnable system
-    "gw_nk8_nao26_nq100": dict(mesh=(2, 2, 2), nao=26, NQ=100, ns=1, beta=50.0, lam=1e3),
+    "gw_nk8_nao26_nq100": dict(mesh=(2, 2, 2), nao=26, NQ=100, ns=1, beta=50.0, lam=1e3, step="iteration"),
     # the plain-FP64 / mixed-precision single-GPU system
-    "gw_nk64_nao52_nq300": dict(mesh=(4, 4, 4), nao=52, NQ=300, ns=2, beta=100.0, lam=1e4),
+    "gw_nk64_nao52_nq300": dict(mesh=(4, 4, 4), nao=52, NQ=300, ns=2, beta=100.0, lam=1e4, step="iteration",
+                                integrals="host"),
     # six independent systems of uneven size in one batched call
     "gw_batch6_nk27": dict(mesh=(3, 3, 3), ns=2, beta=100.0, lam=1e4,
                            systems=[(26, 100), (40, 160), (52, 220), (68, 280), (80, 340), (104, 420)]),
-    # the large system: 2.4 TB of stored pairs, so the integrals are generated
-    # per batch on the device; a bounded sample of q (one GPU's q share) is
-    # timed and the iteration extrapolated (every q costs the same without
-    # symmetry)
-    "gw_nk216_nao104_nq600": dict(mesh=(6, 6, 6), nao=104, NQ=600, ns=2, beta=100.0, lam=1e4,
-                                  integrals="generated", q_share="0/216"),
+    # the large system: one q per step, integrals generated per pair batch
+    "gw_nk216_nao104_nq600": dict(mesh=(6, 6, 6), nao=104, NQ=600, ns=2, beta=100.0, lam=1e4, step="q",
+                                  integrals="generated", hf_k_share="0/216"),
 }
-DEFAULT_WORKLOAD = "gw_nk64_nao52_nq300"
+DEFAULT_WORKLOAD = "gw_nk216_nao104_nq600"
+# share of --budget-s each mode may spend on timed steps (the rest covers input
+# generation, plan set-up, autotuning, warm-up, error checks, HF and the CPU
+# sample)
+MODE_SHARE = {"fp64": 0.30, "bf16x3": 0.12, "fp32": 0.12}
+WARMUP_PAIR_BATCHES = 4
 
 
 def log(*a):
@@ -71,71 +83,148 @@ def mode_roofline(mode, sites):
     name, (ms, flops, n) = max(sites.items(), key=lambda kv: kv[1][0])
     alg = flops / (ms * 1e-3) / 1e12
     mfma = alg * MFMA_PER_ALG[mode]
-    peak = PEAK[{"fp64": "f64", "fp32": "f32", "bf16x3": "bf16x3"}[mode]]
+    peak = PEAK[DTYPE[mode]]
     return {"kernel": f"gemm site {name}", "achieved_algorithmic": round(alg, 2), "achieved_mfma": round(mfma, 2),
             "peak": peak, "unit": "TFLOP/s", "frac_mfma": round(mfma / peak, 4)}
 
 
-def cpu_baseline(w, G, V_pairs, T_wt, T_tw, sym, npairs=2):
-    """Oracle (numpy/OpenBLAS) on a bounded sample of the same iteration:
-    q = 0, P0 and Sigma contributions of `npairs` k-points, one Dyson solve.
-    Returns the sustained FLOP rate; the iteration time is extrapolated."""
+def host_info():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return model, os.cpu_count(), avail
+
+
+def cpu_baseline(w, G, T_wt, T_tw, sym, V_of, tchunk=8, sig_taus=16):
+    """Oracle (numpy/OpenBLAS, FP64) on a bounded sample of the same step:
+    q = 0, the P0 contribution of one (k, spin) pair for every t < nts/2, the
+    tau -> Omega transform, the Dyson solve of every Omega, Omega -> tau, and
+    the Sigma contribution of one (k, spin) pair at `sig_taus` tau points (tau
+    processed in chunks to bound memory).  Returns the sustained FLOP rate;
+    the step and iteration times are extrapolated from it."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import gw_oracle as O
     try:
         from threadpoolctl import threadpool_info
-        cores = max([i.get("num_threads", 1) for i in threadpool_info() if i.get("user_api") == "blas"] or [1])
+        threads = max([i.get("num_threads", 1) for i in threadpool_info() if i.get("user_api") == "blas"] or [1])
     except Exception:
-        cores = int(os.environ.get("OMP_NUM_THREADS", "1"))
+        threads = int(os.environ.get("OMP_NUM_THREADS", "1"))
     nao, NQ, ns = w["nao"], w["NQ"], w["ns"]
-    nts = T_tw.shape[0]
+    nts, nw_b = T_tw.shape[0], T_wt.shape[0]
     half = nts // 2
-    q = 0
+    per_tau = 16.0 * NQ * nao ** 3 + 8.0 * NQ * NQ * nao * nao
+    pref = -1.0 if ns == 2 else -2.0
+    q, k, s = 0, 0, 0
     flops = 0.0
     t0 = time.perf_counter()
+    k1 = int(sym.k1_from_k2q[k, q])
+    V = V_of(k, k1)
     P0 = np.zeros((nts, NQ, NQ), dtype=np.complex128)
-    for k in range(npairs):
-        k1 = int(sym.k1_from_k2q[k, q])
-        V = V_pairs[(k, k1)]
-        for s in range(ns):
-            Gm = G[nts - 1 - np.arange(half), s, k]
-            G1 = G[:half, s, k1]
-            P0[:half] += O.p0_pair(Gm, G1, V, -1.0 if ns == 2 else -2.0)
-            flops += half * (16.0 * NQ * nao ** 3 + 8.0 * NQ * NQ * nao * nao)
+    for c0 in range(0, half, tchunk):
+        t = np.arange(c0, min(half, c0 + tchunk))
+        P0[t] += O.p0_pair(G[nts - 1 - t, s, k], G[t, s, k1], V, pref)
+        flops += t.size * per_tau
+    P0[:half] /= sym.nk
     P0[nts - 1 - np.arange(half)] = P0[:half]
-    P0w = O.tau_to_omega(P0, T_wt)
-    Pw = O.gw_dyson(P0w, "cholesky")
-    Pt = O.omega_to_tau(Pw, T_tw)
-    nw_b = T_wt.shape[0]
+    Pt = O.omega_to_tau(O.gw_dyson(O.tau_to_omega(P0, T_wt), "cholesky"), T_tw)
     flops += 8.0 * NQ * NQ * (nw_b * (nts - 2) + nts * nw_b) + nw_b * (4.0 / 3.0 + 8.0) * NQ ** 3
-    for k in range(npairs):
-        k1 = int(sym.k2_from_k1q[k, q])
-        V = V_pairs[(k, k1)]
-        for s in range(ns):
-            O.sigma_pair(Pt, G[:, s, k1], V)
-            flops += nts * (16.0 * NQ * nao ** 3 + 8.0 * NQ * NQ * nao * nao)
+    k1 = int(sym.k2_from_k1q[k, q])
+    V = V_of(k, k1)
+    taus = np.linspace(0, nts - 1, sig_taus).round().astype(int)
+    for c0 in range(0, taus.size, tchunk):
+        t = taus[c0:c0 + tchunk]
+        O.sigma_pair(Pt[t], G[t, s, k1], V)
+        flops += t.size * per_tau
     dt = time.perf_counter() - t0
-    return flops / dt, dt, cores, flops
+    return flops / dt, dt, threads, flops
+
+
+def hf_share_flops(nk, ns, NQ, nao, n_ik):
+    return float(n_ik) * nk * ns * 16.0 * NQ * nao ** 3
+
+
+def run_hf(a, w, pkg, sym, G, modes, seed, sigma, deadline, t_start):
+    """HF build (K + J, Ewald-free) on the same system in every mode, the
+    exchange products in the mode's arithmetic and K's partial sums, J and
+    E_HF in FP64: D = -G(beta), synthetic integrals with the same seed.  The
+    large workload builds one IBZ k-point's Fock rows (the reference's per-
+    device k share, cuhf_utils.cu:140) and extrapolates."""
+    nk, nao, NQ, ns = sym.nk, w["nao"], w["NQ"], w["ns"]
+    D = -np.ascontiguousarray(G[-1])                                   # (ns, ink, nao, nao)
+    k_rank, k_size = (int(x) for x in w.get("hf_k_share", "0/1").split("/"))
+    n_ik = len(range(k_rank, sym.ink, k_size))
+    hf_fl = hf_share_flops(nk, ns, NQ, nao, n_ik)
+    res, F64, e64 = {}, None, None
+    for mode in modes:
+        if time.perf_counter() > deadline:
+            log(f"[hf {mode}] skipped: wall-clock budget spent ({time.perf_counter() - t_start:.0f}s)")
+            continue
+        hk = pkg.HfKernel(nao, ns, NQ, sym, integrals="generated" if a.integrals == "generated" else "device",
+                          precision=mode)
+        hk.synth_integrals(seed, sigma)
+        hk.set_k_share(k_rank, k_size)
+        hk.solve(D)                                                     # warm-up
+        t = time.perf_counter()
+        F = hk.solve(D)
+        hf_s = time.perf_counter() - t
+        e_hf = hk.energy()                    # FP64 traces on the device, star-weighted
+        hk.close()
+        if mode == "fp64":
+            F64, e64 = F, e_hf
+        if F64 is None:
+            continue
+        kerr = float(np.abs(F - F64).max())
+        res[mode] = {"seconds": round(hf_s, 4), "exchange_tflops": round(hf_fl / hf_s / 1e12, 2),
+                     "e_hf": e_hf, "k_max_abs_err": kerr,
+                     # relative to max |F|: F - F_fp64 is K's error (J is FP64 in every mode)
+                     "k_rel_err": kerr / float(np.abs(F64).max()),
+                     "e_hf_abs_err": abs(e_hf - e64), "e_hf_rel_err": abs(e_hf - e64) / abs(e64)}
+        log(f"[hf {mode}] {hf_s:.3f}s  {hf_fl / hf_s / 1e12:.1f} TFLOP/s (exchange FLOPs)  "
+            f"K err {kerr:.2e}  E_HF err {abs(e_hf - e64):.2e}")
+    if "fp64" not in res:
+        return None
+    h64 = res["fp64"]
+    out = {"seconds": h64["seconds"], "exchange_tflops": h64["exchange_tflops"], "flop": hf_fl,
+           "e_hf": h64["e_hf"], "precision": "fp64",
+           "includes": "K + J + E_HF, host-to-host",
+           "e_hf_definition": "(1/nk) sum_{s, k in IBZ} |star(k)| Re Tr[D_s(k) F_s(k)] / 2",
+           "k_share": f"{k_rank}/{k_size}", "ik_built": n_ik, "modes": res}
+    if n_ik < sym.ink:
+        out["extrapolated_build_s"] = round(h64["seconds"] * sym.ink / n_ik, 2)
+    return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS))
-    ap.add_argument("--precision", default="fp64,fp32,bf16x3",
-                    help="comma-separated modes; fp64 always runs (the error reference)")
+    ap.add_argument("--precision", default="fp64,bf16x3,fp32",
+                    help="comma-separated modes; fp64 always runs first (the error reference)")
     ap.add_argument("--solver", default="cholesky", choices=["cholesky", "lu"])
     ap.add_argument("--integrals", default=None, choices=["host", "device", "generated"],
                     help="host: pinned host memory, streamed per pair batch (overlapped); device: resident in "
-                         "HBM; generated: synthesised per batch on the device (default: the workload's, else host)")
-    ap.add_argument("--q-share", default=None,
-                    help="R/S: time only IBZ q = R, R+S, ... (one GPU's share of a q-sharded iteration) and "
-                         "extrapolate the iteration (default: the workload's, else 0/1)")
+                         "HBM; generated: synthesised per batch on the device (default: the workload's, else "
+                         "device)")
+    ap.add_argument("--budget-s", type=float, default=480.0,
+                    help="wall-clock budget of the whole run (s); each mode's timed steps get a share of it and "
+                         "later phases are skipped when it is spent")
+    ap.add_argument("--pair-batches", type=int, default=0,
+                    help="profiling runs: limit every timed step to its first N pair batches per chain "
+                         "(a partial step; the line is then marked partial)")
     ap.add_argument("--no-hf", action="store_true", help="skip the HF build")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
-    ap.add_argument("--cpu-pairs", type=int, default=2)
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the built-in launch variants instead of autotuning each plan before the warmup")
     ap.add_argument("--tune-rounds", type=int, default=2)
@@ -143,8 +232,8 @@ def main():
                     help="batched workload: modes (comma-separated) also timed as one captured graph launch "
                          "(default: every mode, when the integrals are resident in HBM)")
     ap.add_argument("--graph", action="store_true",
-                    help="time the iteration as one captured HIP graph replay (needs --integrals device); "
-                         "per-site times then come from one profiled eager iteration before the timed steps")
+                    help="time the steps as captured HIP graph replays (needs --integrals device); per-site "
+                         "times then come from one profiled eager step before the timed steps")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one plain 1 GiB device-to-device copy kernel (scale for HBM counters)")
     a = ap.parse_args()
@@ -152,44 +241,47 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
                           "n_gpus": a.gpus}))
         sys.exit(2)
+    w = WORKLOADS[a.workload]
+    a.integrals = a.integrals or w.get("integrals", "device")
+    if a.graph and a.integrals != "device":
+        sys.exit("--graph replays a captured iteration and needs --integrals device")
+    t_start = time.perf_counter()
 
     import __graft_entry__ as ge
     pkg = ge._load_package()
     import green_gpu_amd.synth as synth
 
-    w = WORKLOADS[a.workload]
-    a.integrals = a.integrals or w.get("integrals", "host")
-    if a.graph and a.integrals != "device":
-        sys.exit("--graph replays a captured iteration and needs --integrals device")
     if "systems" in w:
         return run_batch(a, w, pkg, synth)
+    deadline = t_start + a.budget_s
     sym = pkg.mesh_symmetry_none(w["mesh"])
     nk, nao, NQ, ns = sym.nk, w["nao"], w["NQ"], w["ns"]
     tau, T_wt, T_tw = synth.load_ir(w["beta"], w["lam"])
     nts, nw_b = tau.size, T_wt.shape[0]
-    q_rank, q_size = (int(x) for x in (a.q_share or w.get("q_share", "0/1")).split("/"))
-    if not (0 <= q_rank < q_size):
-        sys.exit("--q-share R/S needs 0 <= R < S")
-    q_run = len(range(q_rank, sym.inq, q_size))
+    per_q = w["step"] == "q"
     log(f"workload {a.workload}: nk={nk} nao={nao} NQ={NQ} ns={ns} nts={nts} nw_b={nw_b}; "
-        f"q share {q_rank}/{q_size}: {q_run} of {sym.inq} q; integrals {a.integrals}")
+        f"step = {'one q of ' + str(sym.inq) if per_q else 'one iteration'}; integrals {a.integrals}")
     t = time.perf_counter()
     G, _ = synth.make_green(nk, nao, ns, w["beta"], tau, 1000)
     log(f"G generated in {time.perf_counter() - t:.1f}s")
     Gd = pkg.DeviceArray.from_numpy(G)
     seed, sigma = 1017, 0.1
-    fl = {k: v * q_run / sym.inq for k, v in gw_flops(nk, nao, NQ, ns, nts, nw_b).items()}
-    modes = [m.strip() for m in a.precision.split(",")]
-    if "fp64" not in modes:
-        modes = ["fp64"] + modes
-    runs, sigmas, p0s = {}, {}, {}
+    fl_iter = gw_flops(nk, nao, NQ, ns, nts, nw_b)
+    fl_step = {k: v / sym.inq for k, v in fl_iter.items()} if per_q else fl_iter
+    modes = [m.strip() for m in a.precision.split(",") if m.strip()]
+    modes = ["fp64"] + [m for m in modes if m != "fp64"]
+    runs = {}
+    S64 = P064 = None
     for mode in modes:
+        if time.perf_counter() > deadline - 60:
+            log(f"[{mode}] skipped: wall-clock budget spent ({time.perf_counter() - t_start:.0f}s)")
+            continue
         kern = pkg.GwKernel(nao, ns, NQ, T_wt, T_tw, sym, precision=mode, solver=a.solver, integrals=a.integrals)
         kern.synth_integrals(seed, sigma)
-        kern.set_q_share(q_rank, q_size)
+        if per_q:
+            kern.set_q_share(0, sym.inq)
         Sd = pkg.DeviceArray.empty_like(Gd)
         pkg.synchronize()
-        log(f"[{mode}] inputs resident in HBM")
         tune = None
         if not a.no_autotune:
             rep = kern.autotune_device(Gd.ptr, rounds=a.tune_rounds)
@@ -198,55 +290,87 @@ def main():
                     "candidate_ms": {k: {c: round(t, 3) for c, t in v["ms"].items()}
                                      for k, v in rep["sites"].items()}}
             log(f"[{mode}] autotuned in {rep['wall_ms'] / 1e3:.2f}s: {tune['chosen']}")
+        kern.set_pair_batch_limit(WARMUP_PAIR_BATCHES)
         for i in range(a.warmup):
             t = time.perf_counter()
             kern.solve_device(Gd.ptr, Sd.ptr)
             pkg.synchronize()
-            log(f"[{mode}] warmup step {i}: {time.perf_counter() - t:.2f}s")
-        kern.set_profiling(True)
+            log(f"[{mode}] warm-up step {i} (first {WARMUP_PAIR_BATCHES} pair batches of q 0): "
+                f"{time.perf_counter() - t:.2f}s")
+        kern.set_pair_batch_limit(a.pair_batches)
+        sites, phases = {}, {}
         if a.graph:
-            # per-site times from one profiled eager iteration, then the
-            # graph is captured (untimed) and replayed in the timed steps
+            # per-site times from one profiled eager step; the graph is then
+            # captured (untimed) and replayed by the timed steps
+            kern.set_profiling(True)
             kern.solve_device(Gd.ptr, Sd.ptr)
             sites = kern.site_stats()
             kern.set_profiling(False)
             kern.set_graph(True)
+            kern.solve_device(Gd.ptr, Sd.ptr)
+            pkg.synchronize()
+        else:
+            kern.set_profiling(True)
+        mode_budget = MODE_SHARE.get(mode, 0.12) * a.budget_s
+        times = []
+        first = None
+        staged = {"streamed": 0.0, "generated": 0.0, "staging_ms": 0.0}
+        for i in range(max(1, a.steps)):
+            if per_q:
+                kern.set_q_share(i % sym.inq, sym.inq)
+            pkg.synchronize()
             t = time.perf_counter()
             kern.solve_device(Gd.ptr, Sd.ptr)
             pkg.synchronize()
-            log(f"[{mode}] graph captured and replayed once: {time.perf_counter() - t:.2f}s")
-        pkg.synchronize()
-        t0 = time.perf_counter()
-        for i in range(a.steps):
-            kern.solve_device(Gd.ptr, Sd.ptr)
-            pkg.synchronize()
-            log(f"[{mode}] step {i} done")
-        dt = (time.perf_counter() - t0) / a.steps
+            times.append(time.perf_counter() - t)
+            if not a.graph:
+                for k, (ms, f, n) in kern.site_stats().items():
+                    o = sites.get(k, (0.0, 0.0, 0))
+                    sites[k] = (o[0] + ms, o[1] + f, o[2] + n)
+                for k, v in kern.timings().items():
+                    phases[k] = phases.get(k, 0.0) + v
+            staged["streamed"] += kern.streamed_bytes()
+            staged["generated"] += kern.generated_bytes()
+            staged["staging_ms"] += kern.staging_ms()
+            log(f"[{mode}] step {i} ({'q ' + str(i % sym.inq) if per_q else 'iteration'}): {times[-1]:.2f}s")
+            if i == 0:
+                # Sigma and P0 of q = 0 (the whole iteration on the small
+                # workloads) for the error against FP64, read outside the timing
+                iq, P0 = kern.last_polarization(0)
+                first = {"sigma": Sd.numpy(), "p0": P0}
+            spent = sum(times)
+            if spent + times[-1] > mode_budget or time.perf_counter() + times[-1] > deadline - 45:
+                break
         kern.set_profiling(False)
-        if not a.graph:
-            sites = kern.site_stats()
-        runs[mode] = {"s": dt, "sites": sites, "phases": kern.timings(),
-                      "streamed": kern.streamed_bytes(), "generated": kern.generated_bytes(), "autotune": tune,
-                      "staging_ms": kern.staging_ms()}
-        sigmas[mode] = Sd.numpy()
+        n = len(times)
+        if a.graph:
+            for k in list(sites):
+                sites[k] = (sites[k][0] * n, sites[k][1] * n, sites[k][2] * n)
+        runs[mode] = {"s": sum(times) / n, "steps": n, "step_s": [round(x, 3) for x in times], "sites": sites,
+                      "phases": {k: v / n for k, v in phases.items()}, "autotune": tune,
+                      "streamed": staged["streamed"] / n, "generated": staged["generated"] / n,
+                      "staging_ms": staged["staging_ms"] / n}
+        if mode == "fp64":
+            S64, P064 = first["sigma"], first["p0"]
+            runs[mode].update(sigma_max_abs_err=0.0, sigma_rel_err=0.0, p0_max_abs_err=0.0, p0_rel_err=0.0)
+        else:
+            err = 0.0
+            flat_a, flat_b = first["sigma"].reshape(-1), S64.reshape(-1)
+            for c0 in range(0, flat_a.size, 1 << 26):
+                err = max(err, float(np.abs(flat_a[c0:c0 + (1 << 26)] - flat_b[c0:c0 + (1 << 26)]).max()))
+            perr = float(np.abs(first["p0"] - P064).max())
+            runs[mode].update(sigma_max_abs_err=err, sigma_rel_err=err / float(np.abs(S64).max()),
+                              p0_max_abs_err=perr, p0_rel_err=perr / float(np.abs(P064).max()))
+        del first
         Sd.free()
-        # P0(tau) of the first IBZ q, for the per-mode P0 error
-        p0s[mode] = kern.polarization(G, 0, 0)
         kern.close()
         del kern
-    S64 = sigmas["fp64"]
-    scale = float(np.abs(S64).max())
-    for mode in modes:
         r = runs[mode]
-        err = float(np.abs(sigmas[mode] - S64).max())
-        r["sigma_max_abs_err"] = err
-        r["sigma_rel_err"] = err / scale
-        r["tflops"] = fl["total"] / r["s"] / 1e12
-        p0_scale = float(np.abs(p0s["fp64"]).max())
-        r["p0_max_abs_err"] = float(np.abs(p0s[mode] - p0s["fp64"]).max())
-        r["p0_rel_err"] = r["p0_max_abs_err"] / p0_scale
+        log(f"[{mode}] {n} step(s), {r['s']:.2f}s per step, Sigma max-abs err {r['sigma_max_abs_err']:.2e}")
+    scale = float(np.abs(S64).max())
+    for mode, r in runs.items():
+        r["tflops"] = fl_step["total"] / r["s"] / 1e12
         r["speedup_vs_fp64"] = runs["fp64"]["s"] / r["s"]
-        log(f"[{mode}] {r['s']:.2f}s  {r['tflops']:.1f} TFLOP/s  Sigma max-abs err {err:.2e} (|Sigma| max {scale:.2e})")
     if a.calibrate_copy:
         src = pkg.DeviceArray.from_numpy(np.ones(1 << 27))            # 1 GiB
         dst = pkg.DeviceArray.empty_like(src)
@@ -254,115 +378,92 @@ def main():
         log("calibration copy: 1073741824 bytes read, 1073741824 bytes written")
         src.free()
         dst.free()
+    Gd.free()
 
-    # HF build (exchange K + direct J) on the same system in every mode, the
-    # exchange products in the mode's arithmetic and K's partial sums, J and
-    # the Ewald-free Fock matrix in FP64: D = -G(beta), synthetic integrals
-    # with the same seed
     hf = None
     if not a.no_hf:
-        D = -np.ascontiguousarray(G[-1])                                   # (ns, ink, nao, nao)
-        hf_fl = float(nk) * nk * ns * 16.0 * NQ * nao ** 3
-        hf_modes, F64 = {}, None
-        for mode in modes:
-            hk = pkg.HfKernel(nao, ns, NQ, sym, integrals="generated" if a.integrals == "generated" else "device",
-                              precision=mode)
-            hk.synth_integrals(seed, sigma)
-            if q_run == sym.inq:
-                hk.solve(D)                                                # warm-up
-            t = time.perf_counter()
-            F = hk.solve(D)
-            hf_s = time.perf_counter() - t
-            hk.close()
-            # E_HF-like scalar, accumulated in FP64 on the host (the reference defines none)
-            e_hf = float(0.5 * np.real(np.einsum("skab,skba->", D, F)) / nk)
-            if mode == "fp64":
-                F64, e64 = F, e_hf
-            kerr = float(np.abs(F - F64).max())
-            hf_modes[mode] = {"seconds": round(hf_s, 4), "exchange_tflops": round(hf_fl / hf_s / 1e12, 2),
-                              "e_hf_like": e_hf, "k_max_abs_err": kerr,
-                              # relative to max |F|: F - F_fp64 is K's error (J is FP64 in every mode)
-                              "k_rel_err": kerr / float(np.abs(F64).max()),
-                              "e_hf_abs_err": abs(e_hf - e64), "e_hf_rel_err": abs(e_hf - e64) / abs(e64)}
-            log(f"[hf {mode}] {hf_s:.3f}s  {hf_fl / hf_s / 1e12:.1f} TFLOP/s (exchange FLOPs)  "
-                f"K err {kerr:.2e}  E_HF err {abs(e_hf - e64):.2e}")
-        h64 = hf_modes["fp64"]
-        hf = {"seconds": h64["seconds"], "exchange_tflops": h64["exchange_tflops"], "flop": hf_fl,
-              "e_hf_like": h64["e_hf_like"], "precision": "fp64", "includes": "K + J, host-to-host",
-              "modes": hf_modes}
+        hf = run_hf(a, w, pkg, sym, G, list(runs), seed, sigma, deadline - 40, t_start)
 
-    # headline: the fastest mode whose Sigma stays within 1e-6 of FP64
-    ok = [m for m in modes if runs[m]["sigma_max_abs_err"] <= 1e-6]
+    # headline: the fastest mode whose Sigma stays within the bound of FP64
+    ok = [m for m in runs if runs[m]["sigma_max_abs_err"] <= SIGMA_BOUND]
     head = min(ok, key=lambda m: runs[m]["s"])
     r = runs[head]
-    dtype = {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16x3"}[head]
+    dtype = DTYPE[head]
     dom_name, (dom_ms, dom_fl, dom_n) = max(r["sites"].items(), key=lambda kv: kv[1][0])
     achieved = dom_fl / (dom_ms * 1e-3) / 1e12
-    # HBM bytes per launch of that kernel from the committed rocprofv3 FETCH_SIZE /
-    # WRITE_SIZE passes of this workload and precision (null when not profiled)
+    # HBM bytes per launch of that kernel from the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE passes of this workload and precision (null when
+    # not profiled)
     traffic = None
     tf = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     if os.path.exists(tf):
         traffic = json.load(open(tf)).get(a.workload, {}).get(head, {}).get(dom_name)
     roofline = {"bound": "mfma", "kernel": f"gemm site {dom_name}", "achieved": round(achieved, 3),
                 "peak": PEAK[dtype], "unit": "TFLOP/s", "frac": round(achieved / PEAK[dtype], 4),
-                "traffic": None if traffic is None else round(traffic), "launches": dom_n,
-                "avg_launch_ms": round(dom_ms / dom_n, 4)}
+                "traffic": None if traffic is None else round(traffic["bytes"]),
+                "traffic_source": None if traffic is None else traffic["source"],
+                "launches": dom_n, "avg_launch_ms": round(dom_ms / dom_n, 4)}
+    step_name = f"one q of {sym.inq}" if per_q else "one GW iteration"
+    if a.pair_batches:
+        step_name += f", first {a.pair_batches} pair batches per chain only (partial, profiling)"
     out = {
         "metric": METRIC, "value": round(r["tflops"], 3), "unit": "TFLOP/s", "n_gpus": 1,
-        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(r["s"] * 1e3, 1),
+        "steps": r["steps"], "warmup": a.warmup, "ms_per_step": round(r["s"] * 1e3, 1),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
         "data": "synthetic (seeded random H(k) -> G(tau); Gaussian V, sigma 0.1)",
-        "config": {"workload": a.workload, "nk": nk, "nao": nao, "NQ": NQ, "ns": ns, "nts": nts,
-                   "nw_b": nw_b, "beta": w["beta"], "ir_lambda": w["lam"], "precision": head,
-                   "solver": a.solver, "integrals": a.integrals, "q_share": f"{q_rank}/{q_size}",
-                   "graph": bool(a.graph)},
-        "gw_iteration_s": round(r["s"], 3),
-        "gw_flop_per_iteration": fl["total"],
-        "modes": {m: {"gw_iteration_s": round(runs[m]["s"], 3), "tflops": round(runs[m]["tflops"], 2),
-                      "speedup_vs_fp64": round(runs[m]["speedup_vs_fp64"], 3),
-                      "sigma_max_abs_err": runs[m]["sigma_max_abs_err"],
-                      "sigma_rel_err": runs[m]["sigma_rel_err"],
-                      "p0_max_abs_err": runs[m]["p0_max_abs_err"],
-                      "p0_rel_err": runs[m]["p0_rel_err"],
-                      "phases_ms": {k: round(v, 1) for k, v in runs[m]["phases"].items()},
-                      "sites_ms": {k: round(v[0], 1) for k, v in runs[m]["sites"].items()},
-                      "autotune": runs[m]["autotune"],
-                      "roofline": mode_roofline(m, runs[m]["sites"])} for m in modes},
-        "sigma_abs_max": scale,
+        "config": {"workload": a.workload, "step": step_name, "nk": nk, "nao": nao, "NQ": NQ, "ns": ns,
+                   "nts": nts, "nw_b": nw_b, "beta": w["beta"], "ir_lambda": w["lam"], "precision": head,
+                   "solver": a.solver, "integrals": a.integrals, "graph": bool(a.graph),
+                   "partial": bool(a.pair_batches)},
+        "gw_iteration_s": round(r["s"] * (sym.inq if per_q else 1), 2),
+        "gw_iteration_extrapolated": per_q,
+        "gw_flop_per_step": fl_step["total"],
+        "gw_flop_per_iteration": fl_iter["total"],
+        "modes": {m: {"step_s": round(v["s"], 3), "steps": v["steps"], "step_times_s": v["step_s"],
+                      "gw_iteration_s": round(v["s"] * (sym.inq if per_q else 1), 2),
+                      "tflops": round(v["tflops"], 2),
+                      "speedup_vs_fp64": round(v["speedup_vs_fp64"], 3),
+                      "sigma_max_abs_err": v["sigma_max_abs_err"], "sigma_rel_err": v["sigma_rel_err"],
+                      "p0_max_abs_err": v["p0_max_abs_err"], "p0_rel_err": v["p0_rel_err"],
+                      "meets_sigma_bound": v["sigma_max_abs_err"] <= SIGMA_BOUND,
+                      "phases_ms": {k: round(x, 1) for k, x in v["phases"].items()},
+                      "sites_ms": {k: round(x[0] / v["steps"], 1) for k, x in v["sites"].items()},
+                      "autotune": v["autotune"],
+                      "roofline": mode_roofline(m, v["sites"])} for m, v in runs.items()},
+        "sigma_abs_max_q0": scale,
+        "error_reference": "FP64 run of the same q (q = 0) on the same inputs" if per_q else
+                           "FP64 iteration on the same inputs",
         "integrals": {"residence": a.integrals,
-                      "h2d_gb_per_iteration": round(r["streamed"] / 1e9, 2),
+                      "h2d_gb_per_step": round(r["streamed"] / 1e9, 2),
                       "h2d_gb_per_s_average": round(r["streamed"] / 1e9 / r["s"], 2),
                       # rate of the copies (or generation) while the copy stream runs them
-                      "staging_s": round(r["staging_ms"] / 1e3, 3),
+                      "staging_s_per_step": round(r["staging_ms"] / 1e3, 3),
                       "staging_gb_per_s": (round((r["streamed"] + r["generated"]) / 1e9 / (r["staging_ms"] / 1e3), 2)
                                            if r["staging_ms"] > 0 else None),
-                      "generated_gb_per_iteration": round(r["generated"] / 1e9, 2)},
+                      "generated_gb_per_step": round(r["generated"] / 1e9, 2)},
         "phases_ms": {k: round(v, 1) for k, v in r["phases"].items()},
-        "sites": {k: {"ms": round(v[0], 1), "tflops": round(v[1] / (v[0] * 1e-3) / 1e12, 2), "launches": v[2]}
-                  for k, v in r["sites"].items()},
+        "sites": {k: {"ms_per_step": round(v[0] / r["steps"], 1), "tflops": round(v[1] / (v[0] * 1e-3) / 1e12, 2),
+                      "launches": v[2]} for k, v in r["sites"].items()},
         "roofline": roofline,
         "hf": hf,
     }
-    if q_run < sym.inq:
-        out["q_share"] = {"rank": q_rank, "size": q_size, "q_timed": q_run, "inq": sym.inq,
-                          "extrapolated_iteration_s": {m: round(runs[m]["s"] * sym.inq / q_run, 1) for m in modes},
-                          "note": "step = this share's q-points; all q cost the same without symmetry"}
-    if not a.no_cpu and a.workload == DEFAULT_WORKLOAD:
-        pairs = {}
-        for k in range(a.cpu_pairs):
-            for k1 in (int(sym.k1_from_k2q[k, 0]), int(sym.k2_from_k1q[k, 0])):
-                lo, hi = min(k, k1), max(k, k1)
-                p = hi * (hi + 1) // 2 + lo
-                S = synth.make_integrals(nk, NQ, nao, seed, sigma, pairs=[p])[0]
-                pairs[(k, k1)] = S if k >= k1 else np.conj(np.swapaxes(S, -1, -2))
-        rate, cdt, cores, cfl = cpu_baseline(w, G, pairs, T_wt, T_tw, sym, a.cpu_pairs)
+    if not a.no_cpu and time.perf_counter() < deadline:
+        def V_of(k, k1):
+            lo, hi = min(k, k1), max(k, k1)
+            S = synth.make_integrals(nk, NQ, nao, seed, sigma, pairs=[hi * (hi + 1) // 2 + lo])[0]
+            return S if k >= k1 else np.conj(np.swapaxes(S, -1, -2))
+        rate, cdt, threads, cfl = cpu_baseline(w, G, T_wt, T_tw, sym, V_of)
+        model, ncpu, navail = host_info()
         out["cpu_baseline"] = {
-            "value": round(rate / 1e12, 5), "unit": "TFLOP/s", "cores": cores, "kind": "port",
-            "sample": f"oracle (numpy/OpenBLAS, FP64) on q=0: P0 and Sigma of {a.cpu_pairs} k-points + one "
-                      f"Dyson solve ({cfl:.3g} FLOP in {cdt:.1f}s); extrapolated iteration "
-                      f"{fl['total'] / rate:.0f}s"}
-    print(json.dumps(out))
+            "value": round(rate / 1e12, 5), "unit": "TFLOP/s", "cores": threads, "kind": "port",
+            "nproc": navail, "cpu_count": ncpu, "cpu_model": model,
+            "sample": (f"oracle (numpy/OpenBLAS, FP64, {threads} BLAS threads) on q = 0: P0 of one (k, spin) pair "
+                       f"for all {nts // 2} tau < beta/2, tau->Omega, Dyson of all {nw_b} Omega, Omega->tau, "
+                       f"Sigma of one (k, spin) pair at 16 tau: {cfl:.3g} FLOP in {cdt:.1f}s; extrapolated "
+                       f"{'step' if per_q else 'iteration'} {fl_step['total'] / rate:.0f}s, iteration "
+                       f"{fl_iter['total'] / rate:.0f}s")}
+    out["wall_s"] = round(time.perf_counter() - t_start, 1)
+    print(json.dumps(out), flush=True)
 
 
 def run_batch(a, w, pkg, synth):
@@ -372,9 +473,8 @@ def run_batch(a, w, pkg, synth):
     nk, ns = sym.nk, w["ns"]
     tau, T_wt, T_tw = synth.load_ir(w["beta"], w["lam"])
     nts, nw_b = tau.size, T_wt.shape[0]
-    modes = [m.strip() for m in a.precision.split(",")]
-    if "fp64" not in modes:
-        modes = ["fp64"] + modes
+    modes = [m.strip() for m in a.precision.split(",") if m.strip()]
+    modes = ["fp64"] + [m for m in modes if m != "fp64"]
     Gd = []
     for i, (nao, NQ) in enumerate(w["systems"]):
         G, _ = synth.make_green(nk, nao, ns, w["beta"], tau, 2000 + i)
@@ -436,14 +536,14 @@ def run_batch(a, w, pkg, synth):
         for i in range(len(Gd)):
             err = max(err, float(np.abs(sig[mode][i] - sig["fp64"][i]).max()))
         res[mode]["sigma_max_abs_err"] = err
-    ok = [m for m in modes if res[m]["sigma_max_abs_err"] <= 1e-6]
+    ok = [m for m in modes if res[m]["sigma_max_abs_err"] <= SIGMA_BOUND]
     head = min(ok, key=lambda m: res[m]["batched_s"])
     r = res[head]
     out = {
         "metric": METRIC, "value": round(r["tflops"], 3), "unit": "TFLOP/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(r["batched_s"] * 1e3, 1),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16x3"}[head],
+        "dtype": DTYPE[head],
         "data": "synthetic (seeded random H(k) -> G(tau); Gaussian V, sigma 0.1)",
         "config": {"workload": a.workload, "nk": nk, "ns": ns, "systems": w["systems"], "nts": nts, "nw_b": nw_b,
                    "beta": w["beta"], "ir_lambda": w["lam"], "precision": head, "solver": a.solver,

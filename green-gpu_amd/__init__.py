@@ -56,6 +56,7 @@ def lib():
         L.ggpu_hf_synth_integrals.argtypes = [vp, u64, f64]
         L.ggpu_hf_exchange.argtypes = [vp, pd, pd]
         L.ggpu_hf_solve.argtypes = [vp, pd, pd, f64, pd]
+        L.ggpu_hf_energy.argtypes = [vp, pd]
         L.ggpu_hf_plan_destroy.argtypes = [vp]
         L.ggpu_hf_plan_destroy.restype = None
         L.ggpu_gw_plan_create.argtypes = [C.POINTER(vp), i32, i32, i32, i32, i32, i32, C.POINTER(SymmetryC),
@@ -71,6 +72,8 @@ def lib():
         L.ggpu_gw_iterate_device.argtypes = [vp, vp, vp]
         L.ggpu_gw_iterate_many.argtypes = [C.POINTER(vp), i32, C.POINTER(vp), C.POINTER(vp)]
         L.ggpu_gw_polarization.argtypes = [vp, pd, i32, i32, pd]
+        L.ggpu_gw_last_polarization.argtypes = [vp, i32, C.POINTER(C.c_int), pd]
+        L.ggpu_gw_set_pair_batch_limit.argtypes = [vp, i32]
         L.ggpu_gw_last_timings.argtypes = [vp, pd]
         L.ggpu_gw_set_profiling.argtypes = [vp, i32]
         L.ggpu_gw_site_stats.argtypes = [vp, i32, pd, pd, C.POINTER(C.c_int64)]
@@ -97,11 +100,12 @@ EXPORTED = ["ggpu_last_error", "ggpu_device_count", "ggpu_device_memory",
             "ggpu_hf_set_precision",
             "ggpu_hf_load_integrals",
             "ggpu_hf_synth_integrals",
-            "ggpu_hf_exchange", "ggpu_hf_solve", "ggpu_hf_plan_destroy",
+            "ggpu_hf_exchange", "ggpu_hf_solve", "ggpu_hf_energy", "ggpu_hf_plan_destroy",
             "ggpu_gw_plan_create", "ggpu_gw_set_integral_residence", "ggpu_gw_streamed_bytes",
             "ggpu_gw_generated_bytes", "ggpu_gw_staging_ms", "ggpu_gw_set_q_share",
             "ggpu_gw_load_integrals", "ggpu_gw_synth_integrals",
             "ggpu_gw_iterate", "ggpu_gw_iterate_device", "ggpu_gw_iterate_many", "ggpu_gw_polarization",
+            "ggpu_gw_last_polarization", "ggpu_gw_set_pair_batch_limit",
             "ggpu_gw_last_timings", "ggpu_gw_set_profiling", "ggpu_gw_site_stats",
             "ggpu_gw_set_graph", "ggpu_gw_graph_captures",
             "ggpu_gw_batch_create", "ggpu_gw_batch_run", "ggpu_gw_batch_captures", "ggpu_gw_batch_destroy",
@@ -450,6 +454,13 @@ class HfKernel:
         _check(lib().ggpu_hf_solve(self._h, _dptr(dm), _dptr(S), self.madelung, _dptr(F)))
         return F
 
+    def energy(self):
+        """Two-electron HF energy of the last solve: (1/nk) sum_{s, k in IBZ}
+        w_k Re Tr[D_s(k) F_s(k)] / 2 with star weights w_k (FP64 on the device)."""
+        e = C.c_double()
+        _check(lib().ggpu_hf_energy(self._h, C.byref(e)))
+        return e.value
+
     def close(self):
         if getattr(self, "_h", None):
             lib().ggpu_hf_plan_destroy(self._h)
@@ -510,6 +521,20 @@ class GwKernel:
         out = np.empty((n, self.NQ, self.NQ), dtype=np.complex128)
         _check(lib().ggpu_gw_polarization(self._h, _dptr(G), int(iq), int(which), _dptr(out)))
         return out
+
+    def last_polarization(self, which):
+        """(iq, array): P0(tau) (which 0), P(Omega) (1) or P(tau) (2) of the last
+        q the last iteration processed, read back without recomputation."""
+        n = {0: self.nts, 1: self.nw_b, 2: self.nts}[which]
+        out = np.empty((n, self.NQ, self.NQ), dtype=np.complex128)
+        iq = C.c_int()
+        _check(lib().ggpu_gw_last_polarization(self._h, int(which), C.byref(iq), _dptr(out)))
+        return iq.value, out
+
+    def set_pair_batch_limit(self, batches):
+        """Run only the first `batches` pair batches of every q's P0 and Sigma
+        chains (0: all) - a partial iteration for warm-up and profiling."""
+        _check(lib().ggpu_gw_set_pair_batch_limit(self._h, int(batches)))
 
     def streamed_bytes(self):
         """Integral bytes copied host -> device by the last iteration."""

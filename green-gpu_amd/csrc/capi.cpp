@@ -130,6 +130,14 @@ int ggpu_hf_solve(ggpu_hf_plan* plan, const double* dm, const double* S_k, doubl
   });
 }
 
+int ggpu_hf_energy(ggpu_hf_plan* plan, double* energy) {
+  return guarded([&] {
+    ggpu::require(plan && plan->impl && energy, "NULL argument");
+    ggpu::require(plan->impl->energy_valid, "hf: no solve has run yet", ggpu::kErrState);
+    *energy = plan->impl->energy;
+  });
+}
+
 void ggpu_hf_plan_destroy(ggpu_hf_plan* plan) {
   if (!plan) return;
   delete plan->impl;
@@ -306,6 +314,22 @@ int ggpu_gw_polarization(ggpu_gw_plan* plan, const double* G, int iq, int which,
   return guarded([&] {
     ggpu::require(plan && plan->impl, "NULL plan");
     plan->impl->polarization(G, iq, which, out);
+  });
+}
+
+int ggpu_gw_last_polarization(ggpu_gw_plan* plan, int which, int* iq, double* out) {
+  return guarded([&] {
+    ggpu::require(plan && plan->impl, "NULL plan");
+    plan->impl->last_polarization(which, iq, out);
+  });
+}
+
+int ggpu_gw_set_pair_batch_limit(ggpu_gw_plan* plan, int batches) {
+  return guarded([&] {
+    ggpu::require(plan && plan->impl, "NULL plan");
+    ggpu::require(batches >= 0, "gw: pair batch limit must be >= 0");
+    plan->impl->invalidate_graph();
+    plan->impl->batch_limit = batches;
   });
 }
 

@@ -25,6 +25,7 @@ enum Status : int {
   kErrMemory = 3,
   kErrNotPositiveDefinite = 4,
   kErrState = 5,
+  kErrSingular = 6,
 };
 
 #define GGPU_HIP(call)                                                                     \

@@ -601,6 +601,7 @@ GwPlan::GwPlan(int nao_, int nso_, int ns_, int NQ_, int nts_, int nw_b_, const 
     GGPU_HIP(hipMemcpy(Uq_dev.get(), sym.U_q.data(), Uq_dev.bytes(), hipMemcpyHostToDevice));
   }
   info.resize(nw_b);
+  if (solver == GGPU_SOLVER_LU) ipiv.resize((size_t)nw_b * NQ);
 
   // pair tables
   const int64_t stride = (int64_t)NQ * n2;
@@ -795,7 +796,29 @@ void GwPlan::dyson_for_q() {
     gemm_on(g, Tf_wt, P0, P0w, stream);
   }
   herm_one_minus(P0w.get(), Amat.get(), n, nw_b, stream);
-  // blocked Cholesky of (1 - P0), right-looking
+  if (solver == GGPU_SOLVER_LU) {
+    lu_solve_dyson();
+  } else {
+    cholesky_solve_dyson();
+  }
+  {
+    GemmDesc g;
+    g.M = nts; g.N = (int)Q2; g.K = nw_b;
+    g.A.ptr = T_tw.get(); g.A.rs = nw_b; g.A.cs = 1;
+    g.B.ptr = P0w.get(); g.B.rs = Q2; g.B.cs = 1;
+    g.C.ptr = Pt.get(); g.C.rs = Q2; g.C.cs = 1;
+    g.site = kSiteIR;
+    gemm_on(g, T_tw, P0w, Pt, stream);
+  }
+  if (lowp()) to_c32(Pt.get(), Pt32.get(), (int64_t)Pt.size(), stream);
+}
+
+// (1 - P0) X = P0 by a batched blocked Cholesky factorisation (potrf /
+// potrs semantics, cugw_qpt.cu:223-264): right-looking, 32-column blocks
+// factored in LDS, trailing updates on the matrix cores; X overwrites P0w.
+void GwPlan::cholesky_solve_dyson() {
+  const int64_t Q2 = (int64_t)NQ * NQ;
+  const int n = NQ;
   const int nb = 32;
   for (int j0 = 0; j0 < n; j0 += nb) {
     const int jb = std::min(nb, n - j0);
@@ -844,17 +867,69 @@ void GwPlan::dyson_for_q() {
     }
     trsm_upper_h_block(Amat.get(), P0w.get(), n, n, nw_b, i0, ib, stream);
   }
-  if (solver == GGPU_SOLVER_LU) herm_one_minus(P0w.get(), nullptr, n, nw_b, stream);
-  {
-    GemmDesc g;
-    g.M = nts; g.N = (int)Q2; g.K = nw_b;
-    g.A.ptr = T_tw.get(); g.A.rs = nw_b; g.A.cs = 1;
-    g.B.ptr = P0w.get(); g.B.rs = Q2; g.B.cs = 1;
-    g.C.ptr = Pt.get(); g.C.rs = Q2; g.C.cs = 1;
-    g.site = kSiteIR;
-    gemm_on(g, T_tw, P0w, Pt, stream);
+}
+
+// (1 - P0) X = P0 by a batched blocked LU factorisation with partial
+// pivoting (getrf / getrs semantics, cugw_qpt.cu:266-322): right-looking,
+// 32-column panels factored with row interchanges, the interchanges applied
+// to the columns left and right of the panel, U12 = L11^-1 A12, and the
+// trailing update on the matrix cores; then the pivots are applied to the
+// right-hand side, L (unit lower) and U solved blockwise, and X is
+// Hermitised again (cu_compute_Pq.cu:44-53).  X overwrites P0w.
+void GwPlan::lu_solve_dyson() {
+  const int64_t Q2 = (int64_t)NQ * NQ;
+  const int n = NQ;
+  const int nb = 32;
+  for (int j0 = 0; j0 < n; j0 += nb) {
+    const int jb = std::min(nb, n - j0);
+    lu_panel(Amat.get(), n, nw_b, j0, jb, ipiv.get(), info.get(), stream);
+    lu_row_swaps(Amat.get(), n, n, nw_b, ipiv.get(), n, j0, j0 + jb, j0, j0 + jb, stream);
+    const int rem = n - j0 - jb;
+    if (rem > 0) {
+      trsm_unit_lower_cols(Amat.get(), Amat.get(), n, nw_b, j0, jb, j0 + jb, rem, stream);
+      GemmDesc h;   // A22 -= L21 U12
+      h.M = rem; h.N = rem; h.K = jb; h.nb0 = nw_b;
+      h.A.ptr = Amat.get() + (int64_t)(j0 + jb) * n + j0; h.A.rs = n; h.A.cs = 1; h.A.bs0 = Q2;
+      h.B.ptr = Amat.get() + (int64_t)j0 * n + (j0 + jb); h.B.rs = n; h.B.cs = 1; h.B.bs0 = Q2;
+      h.C.ptr = Amat.get() + (int64_t)(j0 + jb) * n + (j0 + jb); h.C.rs = n; h.C.cs = 1; h.C.bs0 = Q2;
+      h.alpha_re = -1.0; h.beta_re = 1.0;
+      h.site = kSiteDyson;
+      gemm_on(h, Amat, Amat, Amat, stream);
+    }
   }
-  if (lowp()) to_c32(Pt.get(), Pt32.get(), (int64_t)Pt.size(), stream);
+  // P X = L U X = B: interchange the rows of B, then L Y = P B, U X = Y
+  lu_row_swaps(P0w.get(), n, n, nw_b, ipiv.get(), n, 0, n, 0, 0, stream);
+  for (int i0 = 0; i0 < n; i0 += nb) {
+    const int ib = std::min(nb, n - i0);
+    if (i0 > 0) {
+      GemmDesc g;
+      g.M = ib; g.N = n; g.K = i0; g.nb0 = nw_b;
+      g.A.ptr = Amat.get() + (int64_t)i0 * n; g.A.rs = n; g.A.cs = 1; g.A.bs0 = Q2;
+      g.B.ptr = P0w.get(); g.B.rs = n; g.B.cs = 1; g.B.bs0 = Q2;
+      g.C.ptr = P0w.get() + (int64_t)i0 * n; g.C.rs = n; g.C.cs = 1; g.C.bs0 = Q2;
+      g.alpha_re = -1.0; g.beta_re = 1.0;
+      g.site = kSiteDyson;
+      gemm_on(g, Amat, P0w, P0w, stream);
+    }
+    trsm_unit_lower_cols(Amat.get(), P0w.get(), n, nw_b, i0, ib, 0, n, stream);
+  }
+  const int nblk = (n + nb - 1) / nb;
+  for (int bi = nblk - 1; bi >= 0; --bi) {
+    const int i0 = bi * nb, ib = std::min(nb, n - i0);
+    const int rest = n - i0 - ib;
+    if (rest > 0) {
+      GemmDesc g;
+      g.M = ib; g.N = n; g.K = rest; g.nb0 = nw_b;
+      g.A.ptr = Amat.get() + (int64_t)i0 * n + (i0 + ib); g.A.rs = n; g.A.cs = 1; g.A.bs0 = Q2;
+      g.B.ptr = P0w.get() + (int64_t)(i0 + ib) * n; g.B.rs = n; g.B.cs = 1; g.B.bs0 = Q2;
+      g.C.ptr = P0w.get() + (int64_t)i0 * n; g.C.rs = n; g.C.cs = 1; g.C.bs0 = Q2;
+      g.alpha_re = -1.0; g.beta_re = 1.0;
+      g.site = kSiteDyson;
+      gemm_on(g, Amat, P0w, P0w, stream);
+    }
+    trsm_upper_cols(Amat.get(), P0w.get(), n, nw_b, i0, ib, 0, n, stream);
+  }
+  herm_one_minus(P0w.get(), nullptr, n, nw_b, stream);
 }
 
 // Sigma(k, tau) -= sum_{Q,Q'} P~(q, tau)[Q'][Q] V^{Q'}(k,k-q) G(k-q, tau) V^{Q dagger}(k,k-q)
@@ -946,9 +1021,13 @@ void GwPlan::check_info() {
   std::vector<int> h(nw_b);
   GGPU_HIP(hipMemcpy(h.data(), info.get(), nw_b * sizeof(int), hipMemcpyDeviceToHost));
   for (int w = 0; w < nw_b; ++w)
-    if (h[w] != 0)
+    if (h[w] != 0) {
+      if (solver == GGPU_SOLVER_LU)
+        throw Error(kErrSingular, "gw: 1 - P0(Omega) is singular (Omega index " + std::to_string(w) +
+                                      ", zero pivot in column " + std::to_string(h[w]) + ")");
       throw Error(kErrNotPositiveDefinite, "gw: 1 - P0(Omega) is not positive definite (Omega index " +
                                                std::to_string(w) + ", column " + std::to_string(h[w]) + ")");
+    }
 }
 
 // Install a GEMM profiler / per-site launch variants for the calling thread
@@ -996,8 +1075,9 @@ void GwPlan::autotune(const z64* G_dev, int rounds, int batches) {
   ProfilerGuard guard(&tp);
   struct Limit {
     int& r;
-    Limit(int& x, int v) : r(x) { r = v; }
-    ~Limit() { r = 0; }
+    int saved;
+    Limit(int& x, int v) : r(x), saved(x) { r = v; }
+    ~Limit() { r = saved; }
   } limit(batch_limit, batches);
   stage_G(G_dev);
   int trial[kNumSites];
@@ -1040,6 +1120,7 @@ void GwPlan::enqueue_iteration(const z64* G_dev, z64* S_dev, bool timed) {
   zero_int(info.get(), nw_b, stream);
   for (int iq = share_rank; iq < sym.inq; iq += share_size) {
     if (timed) GGPU_HIP(hipEventRecord(qev[4 * iq + 0], stream));
+    last_q = iq;
     p0_for_q(iq);
     if (timed) GGPU_HIP(hipEventRecord(qev[4 * iq + 1], stream));
     dyson_for_q();
@@ -1254,11 +1335,27 @@ void GwPlan::polarization(const double* G, int iq, int which, double* out) {
   GGPU_HIP(hipMemcpyAsync(G_host_dev.get(), G, G_host_dev.bytes(), hipMemcpyHostToDevice, stream));
   stage_G(G_host_dev.get());
   zero_int(info.get(), nw_b, stream);
+  last_q = iq;
   p0_for_q(iq);
+  if (which != 0) dyson_for_q();
+  GGPU_HIP(hipStreamSynchronize(stream));
+  (void)store.collect_staging_ms();
+  if (which != 0) check_info();
+  int q_done = -1;
+  last_polarization(which, &q_done, out);
+}
+
+void GwPlan::last_polarization(int which, int* iq, double* out) {
+  require(which >= 0 && which <= 2, "gw: which must be 0, 1 or 2");
+  require(out != nullptr && iq != nullptr, "gw: NULL pointer");
+  require(last_q >= 0, "gw: no q has been computed yet", kErrState);
+  const int64_t Q2 = (int64_t)NQ * NQ;
+  GGPU_HIP(hipStreamSynchronize(stream));
+  *iq = last_q;
   if (which == 0) {
+    // sum the split-K partial slices, scale by 1/nk and mirror tau
     std::vector<z64> h((size_t)p0_split * half * Q2);
-    GGPU_HIP(hipMemcpyAsync(h.data(), P0.get(), h.size() * sizeof(z64), hipMemcpyDeviceToHost, stream));
-    GGPU_HIP(hipStreamSynchronize(stream));
+    GGPU_HIP(hipMemcpy(h.data(), P0.get(), h.size() * sizeof(z64), hipMemcpyDeviceToHost));
     z64* o = reinterpret_cast<z64*>(out);
     const double sc = 1.0 / sym.nk;
     for (int t = 0; t < half; ++t)
@@ -1272,16 +1369,11 @@ void GwPlan::polarization(const double* G, int iq, int which, double* out) {
         o[(size_t)t * Q2 + e] = z64{v.x * sc, v.y * sc};
         o[(size_t)(nts - 1 - t) * Q2 + e] = z64{v.x * sc, v.y * sc};
       }
-    (void)store.collect_staging_ms();
     return;
   }
-  dyson_for_q();
   const z64* src = (which == 1) ? P0w.get() : Pt.get();
   const size_t cnt = (which == 1 ? (size_t)nw_b : (size_t)nts) * Q2;
-  GGPU_HIP(hipMemcpyAsync(out, src, cnt * sizeof(z64), hipMemcpyDeviceToHost, stream));
-  GGPU_HIP(hipStreamSynchronize(stream));
-  (void)store.collect_staging_ms();
-  check_info();
+  GGPU_HIP(hipMemcpy(out, src, cnt * sizeof(z64), hipMemcpyDeviceToHost));
 }
 
 // ---------------------------------------------------------------------------
@@ -1371,7 +1463,7 @@ HfPlan::~HfPlan() {
 //   K[s][i][j]     += (px/nk) sum_{b,Q,c} X[s][b][Q][i][c] conj(V^Q(k,k2_b)[j][c])
 // The (b, Q) reduction is split into `nsplit` slices written to partial
 // buffers, so each launch has nsplit * ns workgroups instead of ns.
-void HfPlan::exchange_device() {
+void HfPlan::exchange_device(z64* kpart, int nsp) {
   const int64_t n2 = (int64_t)nao * nao, stride = (int64_t)NQ * n2;
   const double px = (nps == 1) ? -0.5 : -1.0;   // -1 for ns = 2 and the X2C blocks
   const bool lp = compute != Compute::F64;
@@ -1387,8 +1479,9 @@ void HfPlan::exchange_device() {
   const void* dm = lp ? (const void*)dm32.get() : (const void*)dm_fbz.get();
   const Span dsp{dm, dm_fbz.size()};
   if (share_size > 1) zero(Kout.get(), Kout.size(), stream);
+  const int64_t part_elems = (int64_t)nsp * nps * n2;
   for (int ik = share_rank; ik < sym.ink; ik += share_size) {
-    zero(Kpart.get(), Kpart.size(), stream);
+    zero(kpart, part_elems, stream);
     for (int i0 = 0; i0 < sym.nk; i0 += kb) {
       const int n = std::min(kb, sym.nk - i0);
       unfold_pairs(store, tables[ik], i0, n, bidx_off.get(), Vdir, lp, NQ, nao, stream);
@@ -1405,7 +1498,7 @@ void HfPlan::exchange_device() {
         gemm_on(g, wk, dsp, wk, stream, kSiteHF);
       }
       const int total = n * NQ;
-      const int ch = (total + nsplit - 1) / nsplit;
+      const int ch = (total + nsp - 1) / nsp;
       const int full = total / ch, rem = total - full * ch;
       for (int part = 0; part < 2; ++part) {
         const int slices = part == 0 ? full : (rem > 0 ? 1 : 0);
@@ -1420,14 +1513,13 @@ void HfPlan::exchange_device() {
         f.B.ptr = Vdir + first * n2 * es; f.B.rs = 1; f.B.cs = nao; f.B.kbs = n2; f.B.bs0 = 0;
         f.B.bs1 = (int64_t)ch * n2;
         f.B.conj = 1;
-        f.C.ptr = Kpart.get() + (part == 0 ? 0 : (int64_t)full * nps * n2); f.C.rs = nao; f.C.cs = 1;
+        f.C.ptr = kpart + (part == 0 ? 0 : (int64_t)full * nps * n2); f.C.rs = nao; f.C.cs = 1;
         f.C.bs0 = n2; f.C.bs1 = (int64_t)nps * n2;
         f.alpha_re = px / sym.nk; f.beta_re = 1.0;
-        gemm_on(f, wk, wk, Kpart, stream, kSiteHF);
+        gemm_on(f, wk, wk, Span{kpart, (size_t)part_elems}, stream, kSiteHF);
       }
     }
-    sum_partials(Kpart.get(), (int64_t)nps * n2, nsplit, Kout.get() + (int64_t)ik * nps * n2, (int64_t)nps * n2,
-                 stream);
+    sum_partials(kpart, (int64_t)nps * n2, nsp, Kout.get() + (int64_t)ik * nps * n2, (int64_t)nps * n2, stream);
   }
 }
 
@@ -1438,13 +1530,13 @@ static void ibz_to_host_layout(const std::vector<z64>& h, double* out, int ink, 
       std::memcpy(o + ((int64_t)s * ink + ik) * n2, h.data() + ((int64_t)ik * ns + s) * n2, n2 * sizeof(z64));
 }
 
-void HfPlan::stage_dm() {
+void HfPlan::stage_dm(const z64* dm_dev) {
   if (x2c) {
-    rot.run(dm_host_dev.get(), dm2c_fbz.get(), stream);   // (nk, 1, 1, nso, nso)
+    rot.run(dm_dev, dm2c_fbz.get(), stream);              // (nk, 1, 1, nso, nso)
     spinor_copy(x2c_split, dm2c_fbz.get(), dm2c_fbz.size(), nso, 0, dm_fbz.get(), dm_fbz.size(), nao, 0, nao, 1, 1.0,
                 stream);
   } else {
-    rot.run(dm_host_dev.get(), dm_fbz.get(), stream);     // (nk, ns, 1, nao, nao)
+    rot.run(dm_dev, dm_fbz.get(), stream);                // (nk, ns, 1, nao, nao)
   }
 }
 
@@ -1463,12 +1555,43 @@ void HfPlan::fetch(double* out) {
   ibz_to_host_layout(h, out, sym.ink, ns, n2);
 }
 
+// E_2 = (1/nk) sum_{s, k in IBZ} w_k (1/2) Re Tr[D_s(k) F_s(k)], w_k = |star(k)|:
+// the two-electron HF energy of the full Brillouin zone (Tr[D F] is unchanged
+// by the unitary k rotation and only conjugated under time reversal).  The
+// reference returns only the Fock matrix (hf_gpu_kernel.cpp:66-104); the
+// energy is green-mbpt's.  With ns = 1 the density is the spin-summed one, so
+// the same formula holds.  Traces accumulate in FP64 on the device.
+void HfPlan::compute_energy() {
+  const int n = x2c ? nso : nao;
+  const int64_t n2 = (int64_t)n * n;
+  const int nsp = x2c ? 1 : ns;
+  traces.resize((size_t)nsp * sym.ink);
+  GemmDesc g;   // tr[s][ik] = sum_a sum_b D[a][b] F[b][a]: K index b, K-batch a
+  g.M = 1; g.N = 1; g.K = n; g.KB = n; g.nb0 = nsp; g.nb1 = sym.ink;
+  g.A.ptr = dm_host_dev.get(); g.A.rs = 0; g.A.cs = 1; g.A.kbs = n;
+  g.A.bs0 = (int64_t)sym.ink * n2; g.A.bs1 = n2;
+  const z64* F = x2c ? F2c.get() : Kout.get();
+  g.B.ptr = F; g.B.rs = n; g.B.cs = 0; g.B.kbs = 1;
+  g.B.bs0 = x2c ? 0 : n2; g.B.bs1 = x2c ? n2 : (int64_t)ns * n2;
+  g.C.ptr = traces.get(); g.C.rs = 0; g.C.cs = 0; g.C.bs0 = sym.ink; g.C.bs1 = 1;
+  if (x2c) gemm_on(g, dm_host_dev, F2c, traces, stream, kSiteHF);
+  else gemm_on(g, dm_host_dev, Kout, traces, stream, kSiteHF);
+  std::vector<z64> h(traces.size());
+  GGPU_HIP(hipMemcpyAsync(h.data(), traces.get(), traces.bytes(), hipMemcpyDeviceToHost, stream));
+  GGPU_HIP(hipStreamSynchronize(stream));
+  double e = 0.0;
+  for (int s = 0; s < nsp; ++s)
+    for (int ik = 0; ik < sym.ink; ++ik) e += (double)sym.k_stars[ik].size() * 0.5 * h[(size_t)s * sym.ink + ik].x;
+  energy = e / sym.nk;
+  energy_valid = true;
+}
+
 void HfPlan::exchange(const double* dm, double* K) {
   require(store.loaded(), "hf: integrals not loaded", kErrState);
   require(dm && K, "hf: NULL pointer");
   GGPU_HIP(hipMemcpyAsync(dm_host_dev.get(), dm, dm_host_dev.bytes(), hipMemcpyHostToDevice, stream));
-  stage_dm();
-  exchange_device();
+  stage_dm(dm_host_dev.get());
+  exchange_device(Kpart.get(), nsplit);
   fetch(K);
 }
 
@@ -1480,8 +1603,8 @@ void HfPlan::solve(const double* dm, const double* S_k, double madelung, double*
   const int64_t n2 = (int64_t)nao * nao, stride = (int64_t)NQ * n2, so2 = (int64_t)nso * nso;
   GGPU_HIP(hipMemcpyAsync(dm_host_dev.get(), dm, dm_host_dev.bytes(), hipMemcpyHostToDevice, stream));
   GGPU_HIP(hipMemcpyAsync(S_dev.get(), S_k, S_dev.bytes(), hipMemcpyHostToDevice, stream));
-  stage_dm();
-  exchange_device();                                   // Kout[ik][s]
+  stage_dm(dm_host_dev.get());
+  exchange_device(Kpart.get(), nsplit);                // Kout[ik][s]
   // Hartree: jvec[Q] = (1/nk) sum_{k', s} sum_{a,b} V^Q(k',k')[a][b] D_s(k')[b][a]
   // (X2C: the density is D_aa + D_bb, blocks 0 and 1)
   const int nj = x2c ? 2 : ns;
@@ -1558,7 +1681,9 @@ void HfPlan::solve(const double* dm, const double* S_k, double madelung, double*
   if (share_size > 1)
     for (int ik = 0; ik < sym.ink; ++ik)
       if (ik % share_size != share_rank) zero(Kout.get() + (int64_t)ik * nps * n2, (int64_t)nps * n2, stream);
+  energy_valid = false;
   fetch(F);
+  compute_energy();
 }
 
 }  // namespace ggpu

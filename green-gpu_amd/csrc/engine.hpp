@@ -174,6 +174,7 @@ struct GwPlan {
   int p0_split = 8;
   DeviceBuffer<z64> P0, P0w, Amat, Pt, Prot, Ptmp, Uq_dev;
   DeviceBuffer<int> info;
+  DeviceBuffer<int> ipiv;                // LU pivots, (nw_b, NQ)
   DeviceBuffer<z64> work;
   DeviceBuffer<int64_t> bidx_off;       // b * NQ * nao^2
   DeviceBuffer<int64_t> sig_src, sig_dst;
@@ -200,6 +201,7 @@ struct GwPlan {
   std::vector<std::vector<double>> tune_ms;   // [candidate][site]
   double tune_wall_ms = 0;
   int batch_limit = 0;                  // pair batches per q (and star member); 0: all
+  int last_q = -1;                      // IBZ q whose intermediates P0 / P0w / Pt hold
   // Graph replay: the iteration's launches captured once into a HIP graph
   // (for the G / Sigma addresses, variants and q share of the capture) and
   // replayed by every later iterate; needs the integrals resident in HBM.
@@ -221,6 +223,9 @@ struct GwPlan {
   void iterate_device(const z64* G_dev_hostlayout, z64* Sigma_dev_hostlayout);
   void iterate(const double* G, double* Sigma);
   void polarization(const double* G, int iq, int which, double* out);
+  // The intermediates the last iterate (or polarization call) left for its
+  // last q: which = 0 P0(tau), 1 P(Omega), 2 P(tau); *iq = that IBZ q.
+  void last_polarization(int which, int* iq, double* out);
   // Time every launch variant of the contraction call sites (P0 and Sigma
   // chains) on the first q of this plan's share, `batches` pair batches per
   // chain, best of `rounds`, and keep the fastest per site.
@@ -237,6 +242,8 @@ struct GwPlan {
   void drop_graph();
   void p0_for_q(int iq);
   void dyson_for_q();
+  void cholesky_solve_dyson();
+  void lu_solve_dyson();
   void sigma_for_q(int iq);
 };
 
@@ -288,17 +295,25 @@ struct HfPlan {
   // all shares sum to the full ones.
   int share_rank = 0, share_size = 1;
 
+  // Two-electron HF energy of the last solve (FP64 traces on the device,
+  // star-weighted sum on the host); valid after solve().
+  double energy = 0.0;
+  bool energy_valid = false;
+
   HfPlan(int nao, int nso, int ns, int NQ, const ggpu_symmetry& s);
   ~HfPlan();
   void exchange(const double* dm, double* K);
   void solve(const double* dm, const double* S_k, double madelung, double* F);
 
  private:
-  void stage_dm();          // dm_host_dev -> dm_fbz (full BZ, density blocks)
-  void exchange_device();   // Kout[ik][s] from dm_fbz
+  void stage_dm(const z64* dm_dev);   // caller-layout dm on the device -> dm_fbz (full BZ, density blocks)
+  // Kout[ik][s] from dm_fbz; the K partial sums of the split (k2, Q)
+  // reduction go to `part` (nsp slices of nps nao^2)
+  void exchange_device(z64* part, int nsp);
   void fetch(double* out);  // Kout -> caller layout (ns, ink, nso, nso)
   std::vector<PairTable> diag_tables;  // (k, k) pairs: [0] all full k, [1] IBZ representatives
-  DeviceBuffer<z64> jvec, S_dev, Ftmp, dm2c_fbz, F2c;
+  DeviceBuffer<z64> jvec, S_dev, Ftmp, dm2c_fbz, F2c, traces;
+  void compute_energy();    // from dm_host_dev and the Fock matrices of solve()
   SpinorTables x2c_split, x2c_merge;
 };
 

@@ -213,47 +213,161 @@ __global__ __launch_bounds__(64) void chol_panel_kernel(z64* A, int n, int j0, i
   }
 }
 
-// Forward substitution with the nb x nb lower block, 64 right-hand-side
-// columns per workgroup (one column per thread).
-template <bool UPPER_H>
-__global__ __launch_bounds__(64) void trsm_block_kernel(const z64* L, z64* B, int n, int nrhs, int i0, int nb) {
+// Triangular solve with the nb x nb diagonal block T at (i0, i0) of the
+// matrix A (row stride lda) on rows i0 .. i0+nb of B, columns bcol0 ..
+// bcol0+ncols; 64 right-hand-side columns per workgroup (one per thread).
+// KIND: kTriLower  T lower, non-unit (Cholesky L, forward)
+//       kTriUpperH T^dagger upper from the lower L (Cholesky, backward)
+//       kTriUnitLower  T unit lower (LU L, forward)
+//       kTriUpper  T upper, non-unit (LU U, backward)
+enum { kTriLower = 0, kTriUpperH = 1, kTriUnitLower = 2, kTriUpper = 3 };
+
+__device__ inline z64 cdiv(z64 a, z64 d) {
+  const double m = 1.0 / (d.x * d.x + d.y * d.y);
+  return z64{(a.x * d.x + a.y * d.y) * m, (a.y * d.x - a.x * d.y) * m};
+}
+
+template <int KIND>
+__global__ __launch_bounds__(64) void trsm_block_kernel(const z64* A, int lda, int64_t a_ms, z64* B, int ldb,
+                                                        int64_t b_ms, int bcol0, int ncols, int i0, int nb) {
   __shared__ z64 T[NBMAX][NBMAX + 1];
   __shared__ z64 Y[NBMAX][64 + 1];
   const int64_t w = blockIdx.y;
-  const z64* Lw = L + w * (int64_t)n * n;
-  z64* Bw = B + w * (int64_t)n * nrhs;
+  const z64* Aw = A + w * a_ms;
+  z64* Bw = B + w * b_ms + bcol0;
   const int tid = threadIdx.x;
   const int c0 = blockIdx.x * 64;
   for (int e = tid; e < nb * nb; e += 64) {
     const int r = e / nb, c = e - r * nb;
-    T[r][c] = Lw[(int64_t)(i0 + r) * n + i0 + c];
+    T[r][c] = Aw[(int64_t)(i0 + r) * lda + i0 + c];
   }
   for (int e = tid; e < nb * 64; e += 64) {
     const int r = e / 64, c = e - r * 64;
-    if (c0 + c < nrhs) Y[r][c] = Bw[(int64_t)(i0 + r) * nrhs + c0 + c];
+    if (c0 + c < ncols) Y[r][c] = Bw[(int64_t)(i0 + r) * ldb + c0 + c];
   }
   __syncthreads();
-  if (c0 + tid < nrhs) {
-    if (!UPPER_H) {
+  if (c0 + tid < ncols) {
+    if (KIND == kTriLower || KIND == kTriUnitLower) {
       for (int r = 0; r < nb; ++r) {
         z64 acc = Y[r][tid];
         for (int s = 0; s < r; ++s) acc = csub(acc, cmul(T[r][s], Y[s][tid]));
-        const double inv = 1.0 / T[r][r].x;
-        Y[r][tid] = z64{acc.x * inv, acc.y * inv};
+        if (KIND == kTriLower) {
+          const double inv = 1.0 / T[r][r].x;
+          acc = z64{acc.x * inv, acc.y * inv};
+        }
+        Y[r][tid] = acc;
       }
     } else {
       for (int r = nb - 1; r >= 0; --r) {
         z64 acc = Y[r][tid];
-        for (int s = r + 1; s < nb; ++s) acc = csub(acc, cmul(cj(T[s][r]), Y[s][tid]));
-        const double inv = 1.0 / T[r][r].x;
-        Y[r][tid] = z64{acc.x * inv, acc.y * inv};
+        if (KIND == kTriUpperH) {
+          for (int s = r + 1; s < nb; ++s) acc = csub(acc, cmul(cj(T[s][r]), Y[s][tid]));
+          const double inv = 1.0 / T[r][r].x;
+          Y[r][tid] = z64{acc.x * inv, acc.y * inv};
+        } else {
+          for (int s = r + 1; s < nb; ++s) acc = csub(acc, cmul(T[r][s], Y[s][tid]));
+          Y[r][tid] = cdiv(acc, T[r][r]);
+        }
       }
     }
   }
   __syncthreads();
   for (int e = tid; e < nb * 64; e += 64) {
     const int r = e / 64, c = e - r * 64;
-    if (c0 + c < nrhs) Bw[(int64_t)(i0 + r) * nrhs + c0 + c] = Y[r][c];
+    if (c0 + c < ncols) Bw[(int64_t)(i0 + r) * ldb + c0 + c] = Y[r][c];
+  }
+}
+
+// LU with partial pivoting of the panel of columns j0 .. j0+nb, rows j0 .. n
+// (LAPACK zgetf2 semantics on the panel): one workgroup per matrix, the
+// panel in global memory.  Pivot = first row of largest |re| + |im| (izamax);
+// the row swaps are applied inside the panel only (lu_row_swaps applies them
+// to the other columns).  ipiv[w * n + c] = pivot row of column c (0-based);
+// info[w] = column + 1 of the first exactly zero pivot (0 = none).
+constexpr int LU_THREADS = 256;
+
+__global__ __launch_bounds__(LU_THREADS) void lu_panel_kernel(z64* A, int n, int j0, int nb, int* ipiv, int* info) {
+  __shared__ double sval[LU_THREADS];
+  __shared__ int sidx[LU_THREADS];
+  __shared__ z64 srow[NBMAX];
+  const int64_t w = blockIdx.x;
+  z64* Aw = A + w * (int64_t)n * n;
+  const int tid = threadIdx.x;
+  for (int c = j0; c < j0 + nb; ++c) {
+    double best = -1.0;
+    int bi = c;                                // also the pivot if every entry is NaN
+    for (int r = c + tid; r < n; r += LU_THREADS) {
+      const z64 v = Aw[(int64_t)r * n + c];
+      const double a = fabs(v.x) + fabs(v.y);
+      if (a > best) { best = a; bi = r; }      // rows visited in increasing order: ties keep the first
+    }
+    sval[tid] = best;
+    sidx[tid] = bi;
+    __syncthreads();
+    for (int h = LU_THREADS / 2; h > 0; h >>= 1) {
+      if (tid < h) {
+        const double o = sval[tid + h];
+        const int oi = sidx[tid + h];
+        if (o > sval[tid] || (o == sval[tid] && oi < sidx[tid])) { sval[tid] = o; sidx[tid] = oi; }
+      }
+      __syncthreads();
+    }
+    const int p = sidx[0];
+    if (tid == 0) ipiv[w * n + c] = p;
+    // swap rows c and p inside the panel
+    if (p != c && tid < nb) {
+      const int64_t a = (int64_t)c * n + j0 + tid, b = (int64_t)p * n + j0 + tid;
+      const z64 t = Aw[a];
+      Aw[a] = Aw[b];
+      Aw[b] = t;
+    }
+    __syncthreads();
+    const z64 piv = Aw[(int64_t)c * n + c];
+    const bool zero_piv = piv.x == 0.0 && piv.y == 0.0;
+    if (zero_piv) {
+      if (tid == 0 && info[w] == 0) info[w] = c + 1;
+    } else {
+      const double m = 1.0 / (piv.x * piv.x + piv.y * piv.y);
+      const z64 inv = z64{piv.x * m, -piv.y * m};
+      for (int r = c + 1 + tid; r < n; r += LU_THREADS) {
+        const int64_t a = (int64_t)r * n + c;
+        Aw[a] = cmul(Aw[a], inv);
+      }
+    }
+    if (tid < j0 + nb - c - 1) srow[tid] = Aw[(int64_t)c * n + c + 1 + tid];
+    __syncthreads();
+    // rank-1 update of the panel's remaining columns
+    const int m = j0 + nb - c - 1;
+    if (m > 0 && !zero_piv) {
+      const int rows = n - c - 1;
+      for (int e = tid; e < rows * m; e += LU_THREADS) {
+        const int r = c + 1 + e / m, cc = e - (e / m) * m;
+        const int64_t a = (int64_t)r * n + c + 1 + cc;
+        Aw[a] = csub(Aw[a], cmul(Aw[(int64_t)r * n + c], srow[cc]));
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Apply the row interchanges ipiv[w * ldp + i], i = i_begin .. i_end (in
+// order), to the columns of B outside [skip_lo, skip_hi): one thread per
+// column.  B: nw matrices of ldb columns, row stride ldb, matrix stride b_ms.
+__global__ __launch_bounds__(256) void lu_row_swaps_kernel(z64* B, int ldb, int64_t b_ms, int ncols,
+                                                           const int* ipiv, int ldp, int i_begin, int i_end,
+                                                           int skip_lo, int skip_hi) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= ncols || (c >= skip_lo && c < skip_hi)) return;
+  const int64_t w = blockIdx.y;
+  z64* Bw = B + w * b_ms;
+  const int* pw = ipiv + w * ldp;
+  for (int i = i_begin; i < i_end; ++i) {
+    const int p = pw[i];
+    if (p != i) {
+      const z64 t = Bw[(int64_t)i * ldb + c];
+      Bw[(int64_t)i * ldb + c] = Bw[(int64_t)p * ldb + c];
+      Bw[(int64_t)p * ldb + c] = t;
+    }
   }
 }
 
@@ -383,16 +497,50 @@ void chol_panel(z64* A, int n, int nw, int j0, int nb, hipStream_t s) {
   GGPU_HIP(hipGetLastError());
 }
 
+template <int KIND>
+static void trsm_launch(const z64* A, int lda, int64_t a_ms, z64* B, int ldb, int64_t b_ms, int bcol0, int ncols,
+                        int nw, int i0, int nb, hipStream_t s) {
+  if (ncols <= 0) return;
+  hipLaunchKernelGGL((trsm_block_kernel<KIND>), dim3((ncols + 63) / 64, nw), dim3(64), 0, s, A, lda, a_ms, B, ldb,
+                     b_ms, bcol0, ncols, i0, nb);
+  GGPU_HIP(hipGetLastError());
+}
+
 void trsm_lower_block(const z64* L, z64* B, int n, int nrhs, int nw, int i0, int nb, hipStream_t s) {
   require(nb >= 1 && nb <= NBMAX && i0 >= 0 && i0 + nb <= n && nw <= 65535, "trsm_lower_block: bad block");
-  hipLaunchKernelGGL((trsm_block_kernel<false>), dim3((nrhs + 63) / 64, nw), dim3(64), 0, s, L, B, n, nrhs, i0, nb);
-  GGPU_HIP(hipGetLastError());
+  trsm_launch<kTriLower>(L, n, (int64_t)n * n, B, nrhs, (int64_t)n * nrhs, 0, nrhs, nw, i0, nb, s);
 }
 
 void trsm_upper_h_block(const z64* L, z64* B, int n, int nrhs, int nw, int i0, int nb, hipStream_t s) {
   require(nb >= 1 && nb <= NBMAX && i0 >= 0 && i0 + nb <= n && nw <= 65535, "trsm_upper_h_block: bad block");
-  hipLaunchKernelGGL((trsm_block_kernel<true>), dim3((nrhs + 63) / 64, nw), dim3(64), 0, s, L, B, n, nrhs, i0, nb);
+  trsm_launch<kTriUpperH>(L, n, (int64_t)n * n, B, nrhs, (int64_t)n * nrhs, 0, nrhs, nw, i0, nb, s);
+}
+
+void lu_panel(z64* A, int n, int nw, int j0, int nb, int* ipiv, int* info, hipStream_t s) {
+  require(nb >= 1 && nb <= NBMAX && j0 >= 0 && j0 + nb <= n, "lu_panel: bad block");
+  hipLaunchKernelGGL(lu_panel_kernel, dim3(nw), dim3(LU_THREADS), 0, s, A, n, j0, nb, ipiv, info);
   GGPU_HIP(hipGetLastError());
+}
+
+void lu_row_swaps(z64* B, int ldb, int ncols, int nw, const int* ipiv, int ldp, int i_begin, int i_end, int skip_lo,
+                  int skip_hi, hipStream_t s) {
+  require(i_begin >= 0 && i_end <= ldp && nw <= 65535 && ncols <= ldb, "lu_row_swaps: bad range");
+  if (ncols <= 0 || i_end <= i_begin) return;
+  hipLaunchKernelGGL(lu_row_swaps_kernel, dim3((ncols + 255) / 256, nw), dim3(256), 0, s, B, ldb,
+                     (int64_t)ldp * ldb, ncols, ipiv, ldp, i_begin, i_end, skip_lo, skip_hi);
+  GGPU_HIP(hipGetLastError());
+}
+
+void trsm_unit_lower_cols(const z64* A, z64* B, int n, int nw, int i0, int nb, int bcol0, int ncols, hipStream_t s) {
+  require(nb >= 1 && nb <= NBMAX && i0 >= 0 && i0 + nb <= n && nw <= 65535 && bcol0 >= 0 && bcol0 + ncols <= n,
+          "trsm_unit_lower_cols: bad block");
+  trsm_launch<kTriUnitLower>(A, n, (int64_t)n * n, B, n, (int64_t)n * n, bcol0, ncols, nw, i0, nb, s);
+}
+
+void trsm_upper_cols(const z64* A, z64* B, int n, int nw, int i0, int nb, int bcol0, int ncols, hipStream_t s) {
+  require(nb >= 1 && nb <= NBMAX && i0 >= 0 && i0 + nb <= n && nw <= 65535 && bcol0 >= 0 && bcol0 + ncols <= n,
+          "trsm_upper_cols: bad block");
+  trsm_launch<kTriUpper>(A, n, (int64_t)n * n, B, n, (int64_t)n * n, bcol0, ncols, nw, i0, nb, s);
 }
 
 void synth_integrals(z64* V, int64_t p0, int64_t np, const int* pair_is_diag, int NQ, int nao, uint64_t seed,

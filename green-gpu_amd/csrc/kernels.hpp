@@ -59,6 +59,22 @@ void trsm_lower_block(const z64* L, z64* B, int n, int nrhs, int nw, int i0, int
 // B[i0:i0+nb, :] <- (L[i0:i0+nb, i0:i0+nb]^dagger)^-1 B[i0:i0+nb, :] (backward)
 void trsm_upper_h_block(const z64* L, z64* B, int n, int nrhs, int nw, int i0, int nb, hipStream_t s);
 
+// ---- batched blocked LU with partial pivoting (getrf / getrs), nw matrices n x n ----
+// Factor the panel of columns j0 .. j0+nb (rows j0 .. n) in place with row
+// interchanges inside the panel; ipiv[w * n + c] = pivot row of column c;
+// info[w] = column + 1 of the first exactly zero pivot (0 = none).
+void lu_panel(z64* A, int n, int nw, int j0, int nb, int* ipiv, int* info, hipStream_t s);
+// Apply the interchanges ipiv[w * ldp + i], i = i_begin .. i_end in order, to
+// the columns [0, ncols) of B (row stride ldb, matrix stride ldp * ldb)
+// outside [skip_lo, skip_hi).
+void lu_row_swaps(z64* B, int ldb, int ncols, int nw, const int* ipiv, int ldp, int i_begin, int i_end, int skip_lo,
+                  int skip_hi, hipStream_t s);
+// B[i0:i0+nb, bcol0:bcol0+ncols] <- T^-1 B[...], T the nb x nb diagonal block
+// of A at (i0, i0), unit lower (LU's L) or upper non-unit (LU's U); A and B
+// are n x n (B may be A itself when the column ranges do not overlap).
+void trsm_unit_lower_cols(const z64* A, z64* B, int n, int nw, int i0, int nb, int bcol0, int ncols, hipStream_t s);
+void trsm_upper_cols(const z64* A, z64* B, int n, int nw, int i0, int nb, int bcol0, int ncols, hipStream_t s);
+
 // ---- synthetic density-fitted integrals (counter-based, reproducible) ----
 // V[p][Q][i][j] for stored pairs p in [p0, p0+np) of the pair list k1 >= k2,
 // Re/Im ~ N(0, sigma^2); diagonal pairs (k1 == k2) are Hermitised per Q.

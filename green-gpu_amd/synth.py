@@ -43,16 +43,17 @@ def make_green(nk, nao, ns, beta, tau, seed):
     W, _ = np.linalg.qr(Z)
     eps = rng.uniform(-2.0, 2.0, size=(ns, nk, nao))
     f = fermi_weight(eps, tau, beta)                       # (ns, nk, nts, nao)
-    # G[t,s,k] = -W diag(f[s,k,t]) W^dagger, as batched products per k-chunk
+    # G[t,s,k][a][b] = -sum_m f[s,k,t][m] W[a][m] conj(W[b][m]): for each (s, k)
+    # one real GEMM of the weights (nts x nao) with the outer products
+    # P[m][(a, b)] = W[a][m] conj(W[b][m]) viewed as real (nao x 2 nao^2)
     nts = f.shape[2]
     G = np.empty((nts, ns, nk, nao, nao), dtype=np.complex128)
-    WH = np.conj(np.swapaxes(W, -1, -2))
-    kc = max(1, min(nk, (1 << 28) // (nts * nao * nao * 16)))
     for s in range(ns):
-        for k0 in range(0, nk, kc):
-            k1 = min(nk, k0 + kc)
-            X = W[s, k0:k1, None, :, :] * f[s, k0:k1, :, None, :]      # (kc, nts, nao, nao)
-            G[:, s, k0:k1] = -np.swapaxes(X @ WH[s, k0:k1, None], 0, 1)
+        for k in range(nk):
+            Wk = W[s, k]
+            P = np.ascontiguousarray((Wk.T[:, :, None] * np.conj(Wk.T)[:, None, :]).reshape(nao, nao * nao))
+            Gk = np.dot(f[s, k], P.view(np.float64)).view(np.complex128)    # (nts, nao^2)
+            G[:, s, k] = -Gk.reshape(nts, nao, nao)
     D = -G[-1]
     return np.ascontiguousarray(G), np.ascontiguousarray(D)
 

@@ -33,7 +33,9 @@ typedef enum {
   GGPU_ERR_DEVICE = 2,               /* HIP runtime failure                                            */
   GGPU_ERR_MEMORY = 3,               /* device or pinned allocation failed (cugw_qkpt.cu:39-75)       */
   GGPU_ERR_NOT_POSITIVE_DEFINITE = 4,/* 1 - P0(Omega) not HPD; reference aborts (cu_compute_Pq.cu:24-43) */
-  GGPU_ERR_STATE = 5                 /* call out of order (e.g. iterate before integrals are loaded)  */
+  GGPU_ERR_STATE = 5,                /* call out of order (e.g. iterate before integrals are loaded)  */
+  GGPU_ERR_SINGULAR = 6              /* LU solver: exactly zero pivot in 1 - P0(Omega) (getrf info > 0;
+                                        reference aborts, cu_compute_Pq.cu:24-43)                    */
 } ggpu_status;
 
 /* Linear solver for the Dyson equation, same order as LinearSolverType
@@ -165,6 +167,18 @@ int ggpu_hf_exchange(ggpu_hf_plan* plan, const double* dm, double* K_out);
  * adjoint on ba. */
 int ggpu_hf_solve(ggpu_hf_plan* plan, const double* dm, const double* S_k, double madelung, double* F_out);
 
+/* Two-electron HF energy of the last ggpu_hf_solve on this plan:
+ *   E = (1/nk) sum_{s, k in IBZ} w_k (1/2) Re Tr[D_s(k) F_s(k)],  w_k = |star(k)|,
+ * i.e. the full-Brillouin-zone sum (Tr[D F] is invariant under the k rotation
+ * and only conjugated under time reversal).  F is the returned J + K - Ewald;
+ * with ns = 1 the density is the spin-summed one, so the formula is the same;
+ * for X2C the traces run over the nso x nso spinor matrices.  The traces
+ * accumulate in FP64 on the device in every precision mode; with a k share
+ * only the share's rows contribute (shares sum to the total).  The reference
+ * returns only the Fock matrix (hf_gpu_kernel.cpp:66-104); the energy is
+ * green-mbpt's.  GGPU_ERR_STATE before the first solve. */
+int ggpu_hf_energy(ggpu_hf_plan* plan, double* energy);
+
 void ggpu_hf_plan_destroy(ggpu_hf_plan* plan);
 
 /* ------------------------------------------------------------------------
@@ -245,6 +259,18 @@ void ggpu_gw_batch_destroy(ggpu_gw_batch* batch);
  * which = 1: P(Omega)  (nw_b, NQ, NQ)  after the Dyson solve
  * which = 2: P(tau)    (nts, NQ, NQ)   dressed polarization */
 int ggpu_gw_polarization(ggpu_gw_plan* plan, const double* G, int iq, int which, double* out);
+
+/* The same intermediates, read back without recomputation, for the last q
+ * the last ggpu_gw_iterate* (or ggpu_gw_polarization) processed: *iq is that
+ * IBZ q.  With a pair-batch limit they are partial.  GGPU_ERR_STATE before
+ * any q has run.  No counterpart in the reference (its buffers are private). */
+int ggpu_gw_last_polarization(ggpu_gw_plan* plan, int which, int* iq, double* out);
+
+/* Restrict every q of the next iterates to its first `batches` pair batches
+ * of the P0 chain and of the Sigma chain (per star member); 0 (default)
+ * runs all pairs.  A partial iteration for warm-up and profiling runs: Sigma
+ * is then not the full self-energy.  No counterpart in the reference. */
+int ggpu_gw_set_pair_batch_limit(ggpu_gw_plan* plan, int batches);
 
 /* Wall time (ms, device events) of the phases of the last iterate:
  * [0] total, [1] G staging, [2] P0, [3] IR + Dyson, [4] Sigma, [5] Sigma staging. */

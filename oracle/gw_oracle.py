@@ -28,9 +28,16 @@ Pinning (what checks this oracle independently of its own formulas):
   system (the q-rotation U_q P U_q^dagger).
 * ``gw_dyson``: LAPACK semantics (residual of (I-P0) P = P0, Cholesky vs LU).
 
-Parity unpinned: the GW fixture ``GW/data.h5:/result/Sigma_tau`` needs the
-green-grids sparse-IR file ``ir/1e4.h5``, which is absent, so end-to-end GW
-agreement with the reference's stored Sigma is not checked.
+* ``gw_solve`` end to end: the reference's GW regression fixture
+  ``GW/data.h5:/result/Sigma_tau`` (test/cu_solver_test.cpp:226-315) on the
+  tau grid recovered from that fixture (tests/golden/make_gw_grid.py), to
+  ~1e-10 against the reference's 1e-6 (tests/test_oracle_fixtures.py); the
+  two-component variant on ``GW_X2C/data.h5``.
+* ``hf_energy``: the reference defines no energy (green-mbpt does), so its
+  definition here is fixed by the mathematics: a hand-computed one-orbital
+  system (J, K and the Ewald term as scalars) and the symmetry-reduced sum
+  with star weights against the direct full-Brillouin-zone sum on the HF
+  fixture (tests/test_oracle_physics.py).
 """
 from dataclasses import dataclass, field
 
@@ -241,9 +248,15 @@ def hf_solve(dm, V_stored, sym, ns, S_k, madelung):
 
 
 def hf_energy(dm, F, sym):
-    """E_HF-like scalar used to report the HF accumulation error:
-    (1/nk) sum_{s,k in IBZ} w_k Re Tr[D_s(k) F_s(k)] / 2.  Not defined by the
-    reference (its energy lives in green-mbpt); parity unpinned."""
+    """Two-electron HF energy (1/nk) sum_{s, k in IBZ} w_k Re Tr[D_s(k) F_s(k)] / 2
+    with the star weights w_k = |star(k)|, i.e. the sum over the full
+    Brillouin zone (Tr[D F] is invariant under the unitary k rotation and only
+    conjugated under time reversal).  F = J + K - Ewald as hf_solve returns
+    it; with ns = 1 the density is the spin-summed one (J uses it whole, the
+    exchange takes half of it), so the same formula holds.  The reference
+    returns only the Fock matrix (hf_gpu_kernel.cpp:66-104) and defines no
+    energy; this definition is pinned by a hand-computed one-orbital system
+    and by the reduced-vs-full-zone sums on the HF fixture."""
     w = np.array([len(s) for s in sym.k_stars], dtype=np.float64)
     return float(0.5 * np.einsum("k,skij,skji->", w, dm, F).real / sym.nk)
 
@@ -280,6 +293,12 @@ def p0_pair(Gm, G1, V, pref):
     return pref * np.matmul(A, np.swapaxes(B, -1, -2))
 
 
+# tau points per contraction chunk: every tau is an independent product, so
+# chunking changes no result and bounds the intermediates' memory at large
+# nao / NQ
+TAU_CHUNK = 16
+
+
 def gw_p0(Gf, V_stored, sym, q_ir, ns, nts, Gm_src=None):
     """P0(q, tau) for all nts points: k-sum over every full-BZ k (k-stars of the
     IBZ), t < nts/2 computed, scaled by 1/nk and mirrored
@@ -300,9 +319,9 @@ def gw_p0(Gf, V_stored, sym, q_ir, ns, nts, Gm_src=None):
                 NQ = V.shape[0]
                 P0 = np.zeros((nts, NQ, NQ), dtype=np.complex128)
             for s in range(ns):
-                Gm = Gm_src[k, s, nts - 1 - np.arange(half)]
-                G1 = Gf[k1, s, :half]
-                P0[:half] += p0_pair(Gm, G1, V, pref)
+                for c0 in range(0, half, TAU_CHUNK):
+                    t = np.arange(c0, min(half, c0 + TAU_CHUNK))
+                    P0[t] += p0_pair(Gm_src[k, s, nts - 1 - t], Gf[k1, s, t], V, pref)
     P0[:half] *= 1.0 / sym.nk
     P0[nts - 1 - np.arange(half)] = P0[:half]
     return P0
@@ -389,7 +408,9 @@ def _gw_core(Gf, Gm_src, ns, nts, sym, V_stored, T_wt, T_tw, solver, return_part
                 V = v_pair(V_stored, sym, k, k1)
                 Pq = q_rotate(Pt, sym, q_deg)
                 for s in range(ns):
-                    Sigma[:, s, ik] += sigma_pair(Pq, Gf[k1, s], V)
+                    for c0 in range(0, nts, TAU_CHUNK):
+                        t = slice(c0, min(nts, c0 + TAU_CHUNK))
+                        Sigma[t, s, ik] += sigma_pair(Pq[t], Gf[k1, s, t], V)
     Sigma /= sym.nk
     return Sigma, parts
 

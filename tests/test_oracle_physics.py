@@ -145,3 +145,38 @@ def test_linearity_and_spin_bookkeeping(oracle, synth):
     P0 = oracle.gw_p0(oracle.g_full(G1, sym), V, sym, 1, 1, tau.size)
     assert max_abs(P0, oracle.herm(P0)) < 1e-12
     assert max_abs(P0, P0[::-1]) == 0.0
+
+
+def test_hf_energy_one_orbital(oracle):
+    """One k, one orbital, one auxiliary vector: J, K and the Ewald term are
+    scalars (J = v^2 sum_s d_s, K_s = px v^2 d_s, Ewald_s = -c m s^2 d_s with
+    px = -1, c = 1 for ns = 2 and px = c = -1/2, 1/2 for ns = 1), and the
+    two-electron energy is E = 1/2 sum_s d_s F_s."""
+    v, m, s_ov = 0.3, 0.5, 1.1
+    sym = oracle.symmetry_none((1, 1, 1), 1, 1)
+    V = np.full((1, 1, 1, 1), v, dtype=complex)
+    for ns, d in ((2, (0.7, 0.4)), (1, (1.3,))):
+        dm = np.array(d, dtype=complex).reshape(ns, 1, 1, 1)
+        S = np.full((ns, 1, 1, 1), s_ov, dtype=complex)
+        F = oracle.hf_solve(dm, V, sym, ns, S, m)
+        px, c = (-1.0, 1.0) if ns == 2 else (-0.5, 0.5)
+        hand = sum(0.5 * ds * (v * v * sum(d) + px * v * v * ds - c * m * s_ov ** 2 * ds) for ds in d)
+        assert abs(oracle.hf_energy(dm, F, sym) - hand) < 1e-15
+
+
+def test_hf_energy_star_weights(oracle, fx):
+    """On the reference's HF fixture (27 k, 4 IBZ k): the IBZ sum with star
+    weights equals the direct sum over all 27 k of the rotated density and
+    Fock matrices, and an unweighted IBZ sum does not."""
+    NQ, ns = int(fx["params_NQ"]), int(fx["params_ns"])
+    sym = oracle.symmetry_from_fixture(fx, NQ)
+    S_ibz = fx["S_k"][:, sym.k_reduced_to_full]
+    dm = fx["hf_G_tau"][-1]
+    F = oracle.hf_solve(dm, fx["hf_VQ_0"], sym, ns, S_ibz, float(fx["madelung"]))
+    E = oracle.hf_energy(dm, F, sym)
+    Df, Ff = oracle.ibz_to_full(dm, sym), oracle.ibz_to_full(F, sym)
+    E_full = 0.5 * np.einsum("skij,skji->", Df, Ff).real / sym.nk
+    print(f"E_HF {E:.12f}, full-zone sum {E_full:.12f}")
+    assert abs(E - E_full) < 1e-10 * abs(E_full)
+    E_flat = 0.5 * np.einsum("skij,skji->", dm, F).real / sym.nk
+    assert abs(E_flat - E_full) > 1e-3 * abs(E_full)
