@@ -53,7 +53,7 @@ WORKLOADS = {
                            systems=[(26, 100), (40, 160), (52, 220), (68, 280), (80, 340), (104, 420)]),
     # the large system: one q per step, integrals generated per pair batch
     "gw_nk216_nao104_nq600": dict(mesh=(6, 6, 6), nao=104, NQ=600, ns=2, beta=100.0, lam=1e4, step="q",
-                                  integrals="generated", hf_k_share="0/216"),
+                                  integrals="generated"),
 }
 DEFAULT_WORKLOAD = "gw_nk216_nao104_nq600"
 # share of --budget-s each mode may spend on timed steps (the rest covers input
@@ -156,9 +156,9 @@ def hf_share_flops(nk, ns, NQ, nao, n_ik):
 def run_hf(a, w, pkg, sym, G, modes, seed, sigma, deadline, t_start):
     """HF build (K + J, Ewald-free) on the same system in every mode, the
     exchange products in the mode's arithmetic and K's partial sums, J and
-    E_HF in FP64: D = -G(beta), synthetic integrals with the same seed.  The
-    large workload builds one IBZ k-point's Fock rows (the reference's per-
-    device k share, cuhf_utils.cu:140) and extrapolates."""
+    E_HF in FP64: D = -G(beta), synthetic integrals with the same seed.  A
+    workload may name a k share (the reference's per-device IBZ-k loop,
+    cuhf_utils.cu:140) to build only those Fock rows."""
     nk, nao, NQ, ns = sym.nk, w["nao"], w["NQ"], w["ns"]
     D = -np.ascontiguousarray(G[-1])                                   # (ns, ink, nao, nao)
     k_rank, k_size = (int(x) for x in w.get("hf_k_share", "0/1").split("/"))
@@ -173,7 +173,8 @@ def run_hf(a, w, pkg, sym, G, modes, seed, sigma, deadline, t_start):
                           precision=mode)
         hk.synth_integrals(seed, sigma)
         hk.set_k_share(k_rank, k_size)
-        hk.solve(D)                                                     # warm-up
+        if n_ik * nk <= 4096:
+            hk.solve(D)                                                 # warm-up (small systems)
         t = time.perf_counter()
         F = hk.solve(D)
         hf_s = time.perf_counter() - t
@@ -199,8 +200,6 @@ def run_hf(a, w, pkg, sym, G, modes, seed, sigma, deadline, t_start):
            "includes": "K + J + E_HF, host-to-host",
            "e_hf_definition": "(1/nk) sum_{s, k in IBZ} |star(k)| Re Tr[D_s(k) F_s(k)] / 2",
            "k_share": f"{k_rank}/{k_size}", "ik_built": n_ik, "modes": res}
-    if n_ik < sym.ink:
-        out["extrapolated_build_s"] = round(h64["seconds"] * sym.ink / n_ik, 2)
     return out
 
 
