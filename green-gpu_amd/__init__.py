@@ -57,6 +57,7 @@ def lib():
         L.ggpu_hf_exchange.argtypes = [vp, pd, pd]
         L.ggpu_hf_solve.argtypes = [vp, pd, pd, f64, pd]
         L.ggpu_hf_energy.argtypes = [vp, pd]
+        L.ggpu_hf_exchange_device_noalloc.argtypes = [vp, vp, vp]
         L.ggpu_hf_plan_destroy.argtypes = [vp]
         L.ggpu_hf_plan_destroy.restype = None
         L.ggpu_gw_plan_create.argtypes = [C.POINTER(vp), i32, i32, i32, i32, i32, i32, C.POINTER(SymmetryC),
@@ -100,7 +101,8 @@ EXPORTED = ["ggpu_last_error", "ggpu_device_count", "ggpu_device_memory",
             "ggpu_hf_set_precision",
             "ggpu_hf_load_integrals",
             "ggpu_hf_synth_integrals",
-            "ggpu_hf_exchange", "ggpu_hf_solve", "ggpu_hf_energy", "ggpu_hf_plan_destroy",
+            "ggpu_hf_exchange", "ggpu_hf_exchange_device_noalloc", "ggpu_hf_solve", "ggpu_hf_energy",
+            "ggpu_hf_plan_destroy",
             "ggpu_gw_plan_create", "ggpu_gw_set_integral_residence", "ggpu_gw_streamed_bytes",
             "ggpu_gw_generated_bytes", "ggpu_gw_staging_ms", "ggpu_gw_set_q_share",
             "ggpu_gw_load_integrals", "ggpu_gw_synth_integrals",

@@ -130,6 +130,13 @@ int ggpu_hf_solve(ggpu_hf_plan* plan, const double* dm, const double* S_k, doubl
   });
 }
 
+int ggpu_hf_exchange_device_noalloc(ggpu_hf_plan* plan, const void* dm_device, void* K_device) {
+  return guarded([&] {
+    ggpu::require(plan && plan->impl, "NULL plan");
+    plan->impl->exchange_device_noalloc(static_cast<const ggpu::z64*>(dm_device), static_cast<ggpu::z64*>(K_device));
+  });
+}
+
 int ggpu_hf_energy(ggpu_hf_plan* plan, double* energy) {
   return guarded([&] {
     ggpu::require(plan && plan->impl && energy, "NULL argument");

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 namespace ggpu {
 
@@ -1377,6 +1378,35 @@ void GwPlan::last_polarization(int which, int* iq, double* out) {
 }
 
 // ---------------------------------------------------------------------------
+// Static scratch of the no-allocation HF exchange
+// ---------------------------------------------------------------------------
+// K partial sums of ggpu_hf_exchange_device_noalloc: 4 Mi complex doubles
+// (64 MiB) per GPU, held until the process ends.  Kernels never name the
+// array: host code passes the device address hipGetSymbolAddress returns for
+// the current device (each GPU has its own copy at its own address).
+constexpr int64_t kHfScratchElems = (int64_t)1 << 22;
+static __device__ __align__(256) double scratch[2 * kHfScratchElems];
+
+constexpr int kMaxScratchDevices = 64;
+static std::mutex g_scratch_mutex;
+static void* g_scratch_addr[kMaxScratchDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per device.
+static z64* scratch_on_current_device() {
+  int dev = 0;
+  GGPU_HIP(hipGetDevice(&dev));
+  require(dev >= 0 && dev < kMaxScratchDevices, "scratch: device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_addr[dev]) {
+    void* p = nullptr;
+    GGPU_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    require(p != nullptr, "scratch: no device address");
+    g_scratch_addr[dev] = p;
+  }
+  return static_cast<z64*>(g_scratch_addr[dev]);
+}
+
+// ---------------------------------------------------------------------------
 // HF plan
 // ---------------------------------------------------------------------------
 HfPlan::HfPlan(int nao_, int nso_, int ns_, int NQ_, const ggpu_symmetry& s)
@@ -1429,6 +1459,22 @@ HfPlan::HfPlan(int nao_, int nso_, int ns_, int NQ_, const ggpu_symmetry& s)
   for (int b = 0; b < kb; ++b) bo[b] = b * stride;
   bidx_off.resize(kb);
   GGPU_HIP(hipMemcpy(bidx_off.get(), bo.data(), bo.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+  // set-up of the no-allocation exchange: this device's scratch address and
+  // the Kout -> caller-layout tables
+  scratch = scratch_on_current_device();
+  scratch_elems = kHfScratchElems;
+  if (!x2c) {
+    std::vector<int64_t> src((size_t)ns * sym.ink), dst(src.size());
+    for (int ik = 0; ik < sym.ink; ++ik)
+      for (int sp = 0; sp < ns; ++sp) {
+        src[(size_t)ik * ns + sp] = ((int64_t)ik * ns + sp) * n2;
+        dst[(size_t)ik * ns + sp] = ((int64_t)sp * sym.ink + ik) * n2;
+      }
+    kout_src.resize(src.size());
+    kout_dst.resize(dst.size());
+    GGPU_HIP(hipMemcpy(kout_src.get(), src.data(), src.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    GGPU_HIP(hipMemcpy(kout_dst.get(), dst.data(), dst.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+  }
   if (x2c) {
     dm2c_fbz.resize((size_t)sym.nk * so2);
     F2c.resize((size_t)sym.ink * so2);
@@ -1593,6 +1639,31 @@ void HfPlan::exchange(const double* dm, double* K) {
   stage_dm(dm_host_dev.get());
   exchange_device(Kpart.get(), nsplit);
   fetch(K);
+}
+
+// The exchange on caller device buffers with the K partial sums in the static
+// scratch array: the same launches as exchange() (fewer partial slices when
+// the plan's would not fit the scratch), and no device allocation -- every
+// other buffer it touches was allocated when the plan was created.
+void HfPlan::exchange_device_noalloc(const z64* dm_dev, z64* K_dev) {
+  require(store.loaded(), "hf: integrals not loaded", kErrState);
+  require(dm_dev && K_dev, "hf: NULL pointer");
+  require(!store.staged(), "hf no-alloc exchange: needs the integrals resident in HBM", kErrState);
+  require(compute == Compute::F64 || dm32.size() == dm_fbz.size(), "hf: reduced-precision density buffer missing");
+  const int64_t n2 = (int64_t)nao * nao, so2 = (int64_t)nso * nso;
+  const int64_t slice = (int64_t)nps * n2;
+  require(slice <= scratch_elems,
+          "hf no-alloc exchange: one partial slice of " + std::to_string(slice) +
+              " complex elements exceeds the static scratch of " + std::to_string(scratch_elems));
+  require(scratch == scratch_on_current_device(), "hf no-alloc exchange: plan created on another device");
+  const int nsp = (int)std::max<int64_t>(1, std::min<int64_t>(nsplit, scratch_elems / slice));
+  stage_dm(dm_dev);
+  exchange_device(scratch, nsp);
+  if (x2c)
+    spinor_copy(x2c_merge, Kout.get(), Kout.size(), nao, 0, K_dev, (size_t)sym.ink * so2, nso, 0, nao, 1, 1.0, stream);
+  else
+    slab_gather(Kout.get(), kout_src.get(), K_dev, kout_dst.get(), (int)kout_src.size(), n2, 1.0, stream);
+  GGPU_HIP(hipStreamSynchronize(stream));
 }
 
 // F = J + K - c madelung S D S  (hf_gpu_kernel.cpp:66-104, :141-201); the

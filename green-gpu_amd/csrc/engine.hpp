@@ -303,6 +303,13 @@ struct HfPlan {
   HfPlan(int nao, int nso, int ns, int NQ, const ggpu_symmetry& s);
   ~HfPlan();
   void exchange(const double* dm, double* K);
+  // The exchange on caller device buffers (dm, K: (ns, ink, nso, nso)) with
+  // the K partial sums in the library's static per-GPU scratch array instead
+  // of the plan's partial buffer: no device allocation.  Needs the integrals
+  // resident in HBM.  Calls that overlap on one GPU race on the scratch.
+  void exchange_device_noalloc(const z64* dm_dev, z64* K_dev);
+  z64* scratch = nullptr;            // this device's copy of the static scratch array
+  int64_t scratch_elems = 0;
   void solve(const double* dm, const double* S_k, double madelung, double* F);
 
  private:
@@ -313,6 +320,7 @@ struct HfPlan {
   void fetch(double* out);  // Kout -> caller layout (ns, ink, nso, nso)
   std::vector<PairTable> diag_tables;  // (k, k) pairs: [0] all full k, [1] IBZ representatives
   DeviceBuffer<z64> jvec, S_dev, Ftmp, dm2c_fbz, F2c, traces;
+  DeviceBuffer<int64_t> kout_src, kout_dst;   // Kout (ik, s) -> caller layout (s, ik) slab offsets
   void compute_energy();    // from dm_host_dev and the Fock matrices of solve()
   SpinorTables x2c_split, x2c_merge;
 };

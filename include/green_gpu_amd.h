@@ -157,6 +157,22 @@ int ggpu_hf_synth_integrals(ggpu_hf_plan* plan, uint64_t seed, double sigma);
  * blocks with px = -1 and K_ba = K_ab^dagger (cuda_common.cu:176-205). */
 int ggpu_hf_exchange(ggpu_hf_plan* plan, const double* dm, double* K_out);
 
+/* The exchange K of ggpu_hf_exchange on caller device buffers, allocating
+ * nothing: dm_device and K_device are (ns, ink, nso, nso) complex in device
+ * memory (K_device overwritten); the K partial sums of the split (k2, Q)
+ * reduction live in a static per-GPU scratch array of 64 MiB (4 Mi complex
+ * doubles) that the library holds until the process ends, and every other
+ * buffer the call touches was allocated when the plan was created.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (graph capture, tight serving loops, allocation-free harnesses).
+ * Limits: one partial slice (nps * nao^2 complex elements, nps = ns, or 3
+ * for X2C) must fit the scratch, else GGPU_ERR_ARGUMENT; the integrals must
+ * be resident in HBM (GGPU_INTEGRALS_DEVICE), else GGPU_ERR_STATE.  Calls
+ * on one GPU share the scratch: calls that overlap on one GPU (several host
+ * threads, or plans on different streams) race and must be serialised by
+ * the caller.  Returns after the result is complete. */
+int ggpu_hf_exchange_device_noalloc(ggpu_hf_plan* plan, const void* dm_device, void* K_device);
+
 /* Full HF self-energy, as scalar_hf_gpu_kernel::solve returns it
  * (hf_gpu_kernel.cpp:66-104, without the MPI reduction):
  *   F = J + K - c * madelung * S D S,   c = 1 (ns = 2) or 0.5 (ns = 1),
