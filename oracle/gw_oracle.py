@@ -212,7 +212,7 @@ def hf_exchange(dm, V_stored, sym, ns):
             V = v_pair(V_stored, sym, k, k2)             # (NQ, nao, nao)
             VH = np.conj(np.swapaxes(V, -1, -2))
             for s in range(ns):
-                K[s, ik] += (px / sym.nk) * np.einsum("Qia,ab,Qbj->ij", V, dm_fbz[s, k2], VH)
+                K[s, ik] += (px / sym.nk) * np.einsum("Qia,ab,Qbj->ij", V, dm_fbz[s, k2], VH, optimize=True)
     return K
 
 
@@ -484,7 +484,7 @@ def _exchange_fbz(dm_fbz, V_stored, sym, px):
             V = v_pair(V_stored, sym, k, k2)
             VH = np.conj(np.swapaxes(V, -1, -2))
             for s in range(ns):
-                K[s, ik] += (px / sym.nk) * np.einsum("Qia,ab,Qbj->ij", V, dm_fbz[s, k2], VH)
+                K[s, ik] += (px / sym.nk) * np.einsum("Qia,ab,Qbj->ij", V, dm_fbz[s, k2], VH, optimize=True)
     return K
 
 

@@ -1,0 +1,120 @@
+"""GPU parity at the benchmark configurations' per-pair shapes, through the C
+ABI, against oracle values committed by tests/golden/make_workload_fixture.py
+(inputs regenerated from the same seeds; the device generator gives the same
+integrals as the fixture script's host generator):
+
+* nao = 52, NQ = 300 (the nk = 64 workload) in FP32 and split-bf16 x3;
+* nao = 104, NQ = 600 (the nk = 216 workload) in FP64, FP32 and split-bf16 x3
+  (K = 104 > 64 takes the split-bf16 kernels' multi-k-tile path);
+* the six batched systems at their full nao / NQ and nk = 27 as one graph
+  launch: each Sigma equals its own eager iteration bit for bit.
+
+Bounds: FP64 1e-10 max-abs on Sigma and K (relative to max |Sigma| where that
+exceeds 100, the FP64 rounding floor of sums of that size); FP32 2e-5 and
+split-bf16 x3 2e-4 of max |Sigma| (each product keeps ~24 / ~16 significant
+bits; the same relative bounds as the small-system tests).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, max_abs
+
+pytestmark = pytest.mark.gpu
+
+RTOL = {"fp32": 2e-5, "bf16x3": 2e-4}
+
+
+def fixture(nao, NQ):
+    return np.load(os.path.join(ROOT, "tests", "golden", "fixtures", f"gw_workload_nao{nao}_nq{NQ}_nk2.npz"))
+
+
+def fp64_tol(scale):
+    return 1e-10 * max(1.0, scale / 100.0)
+
+
+def run_fixture_system(ggpu, synth, f, precision):
+    mesh = tuple(int(x) for x in f["mesh"])
+    nao, NQ, ns, beta = int(f["nao"]), int(f["NQ"]), int(f["ns"]), float(f["beta"])
+    sym = ggpu.mesh_symmetry_none(mesh)
+    tau, T_wt, T_tw = synth.load_ir(beta, float(f["lam"]))
+    G, D = synth.make_green(sym.nk, nao, ns, beta, tau, int(f["seed_g"]))
+    print(f"fixture shape nao={nao} NQ={NQ} nts={tau.size} precision={precision}", flush=True)
+    k = ggpu.GwKernel(nao, ns, NQ, T_wt, T_tw, sym, precision=precision, solver="cholesky")
+    k.synth_integrals(int(f["seed_v"]), 0.1)
+    S = k.solve(G)[f["tau_idx"]]
+    k.close()
+    h = ggpu.HfKernel(nao, ns, NQ, sym, precision=precision)
+    h.synth_integrals(int(f["seed_v"]), 0.1)
+    K = h.exchange(D)
+    h.close()
+    return S, K
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+def test_reduced_precision_at_nk64_workload_shape(ggpu, synth, precision):
+    f = fixture(52, 300)
+    S, K = run_fixture_system(ggpu, synth, f, precision)
+    scale, kscale = float(f["sigma_abs_max"]), float(np.abs(f["K"]).max())
+    err, errk = max_abs(S, f["sigma"]), max_abs(K, f["K"])
+    print(f"{precision}: Sigma max-abs {err:.3e} (rel {err / scale:.2e}), K {errk:.3e} (rel {errk / kscale:.2e})")
+    assert err < RTOL[precision] * scale and errk < RTOL[precision] * kscale
+
+
+@pytest.mark.parametrize("NQ,precision", [(600, "fp64"), (600, "fp32"), (600, "bf16x3")])
+def test_nao104_shapes(ggpu, synth, NQ, precision):
+    f = fixture(104, NQ)
+    S, K = run_fixture_system(ggpu, synth, f, precision)
+    scale, kscale = float(f["sigma_abs_max"]), float(f["k_abs_max"])
+    err, errk = max_abs(S, f["sigma"]), max_abs(K, f["K"])
+    print(f"nao=104 NQ={NQ} {precision}: Sigma max-abs {err:.3e} (|Sigma| {scale:.3g}), "
+          f"K {errk:.3e} (|K| {kscale:.3g})")
+    if precision == "fp64":
+        assert err < fp64_tol(scale) and errk < fp64_tol(kscale)
+    else:
+        assert err < RTOL[precision] * scale and errk < RTOL[precision] * kscale
+
+
+def test_six_systems_full_size_one_graph_launch(ggpu, synth):
+    """The six systems of the batched workload at their full sizes (nk = 27,
+    (nao, NQ) = (26, 100) ... (104, 420), beta = 100, Lambda = 1e4) captured
+    into one HIP graph: each system's Sigma equals its own eager iteration bit
+    for bit on every replay, with eager work between replays, and is Hermitian
+    in (a, b) for every tau and k.  Each plan runs one q (q share 0/27) and its
+    first pair batch per chain, so the launches have the production shapes and
+    the test stays short."""
+    systems = [(26, 100), (40, 160), (52, 220), (68, 280), (80, 340), (104, 420)]
+    sym = ggpu.mesh_symmetry_none((3, 3, 3))
+    tau, T_wt, T_tw = synth.load_ir(100.0, 1e4)
+    kerns, Gd, Sd, eager = [], [], [], []
+    for i, (nao, NQ) in enumerate(systems):
+        G, _ = synth.make_green(sym.nk, nao, 2, 100.0, tau, 2000 + i)
+        k = ggpu.GwKernel(nao, 2, NQ, T_wt, T_tw, sym, precision="fp64", solver="cholesky", work_bytes=16 << 30)
+        k.synth_integrals(3000 + i, 0.1)
+        k.set_q_share(0, sym.inq)
+        k.set_pair_batch_limit(1)
+        g = ggpu.DeviceArray.from_numpy(G)
+        s = ggpu.DeviceArray.empty_like(g, zero=True)
+        k.solve_device(g.ptr, s.ptr)
+        eager.append(s.numpy())
+        kerns.append(k)
+        Gd.append(g)
+        Sd.append(ggpu.DeviceArray.empty_like(g, zero=True))
+        print(f"system {i} nao={nao} NQ={NQ}: eager done, max |Sigma| {np.abs(eager[-1]).max():.3g}", flush=True)
+    b = ggpu.GwBatch(kerns, [g.ptr for g in Gd], [s.ptr for s in Sd])
+    for rep in range(2):
+        assert b.run() > 0
+        for i in range(len(systems)):
+            assert np.array_equal(Sd[i].numpy(), eager[i]), f"replay {rep}, system {i}"
+        kerns[0].solve_device(Gd[0].ptr, Sd[0].ptr)           # eager work between replays
+    for i, S in enumerate(eager):
+        scale = np.abs(S).max()
+        assert scale > 0
+        herm = max_abs(S, np.conj(np.swapaxes(S, -1, -2)))
+        assert herm < 1e-12 * scale, f"system {i}: Sigma not Hermitian ({herm:.2e})"
+    b.close()
+    for x in Gd + Sd:
+        x.free()
+    for k in kerns:
+        k.close()
