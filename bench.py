@@ -34,12 +34,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "GW iteration wall-time (s) + TFLOP/s, mixed vs FP64 on MI355X at Σ err≤1e-6"
-PEAK = {"f64": 78.6, "f32": 157.3, "bf16x3": 2500.0}      # TFLOP/s, MI355X dense matrix
-DTYPE = {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16x3"}
-# Matrix-core FLOPs executed per algorithmic FLOP (8 per complex multiply-add):
-# FP64 forms 3 real products per complex one (Gauss), FP32 4, split-bf16 x3
-# 4 real products of 3 bf16 products each.
-MFMA_PER_ALG = {"fp64": 0.75, "fp32": 1.0, "bf16x3": 3.0}
+# MI355X dense matrix peaks, TFLOP/s (int8: TOP/s)
+PEAK = {"f64": 78.6, "f32": 157.3, "bf16x3": 2500.0, "i8x5": 5000.0}
+DTYPE = {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16x3", "i8x5": "i8x5"}
+# Matrix-core operations executed per algorithmic FLOP (8 per complex
+# multiply-add): FP64 forms 3 real products per complex one (Gauss), FP32 4,
+# split-bf16 x3 4 real products of 3 bf16 products each; the int8 emulation
+# 4 real products for each of its 19 slice pairs (152 int8 ops per complex
+# multiply-add).
+MFMA_PER_ALG = {"fp64": 0.75, "fp32": 1.0, "bf16x3": 3.0, "i8x5": 19.0}
 SIGMA_BOUND = 1e-6                                          # Ha, max-abs against FP64
 
 WORKLOADS = {
@@ -59,7 +62,7 @@ DEFAULT_WORKLOAD = "gw_nk216_nao104_nq600"
 # share of --budget-s each mode may spend on timed steps (the rest covers input
 # generation, plan set-up, autotuning, warm-up, error checks, HF and the CPU
 # sample)
-MODE_SHARE = {"fp64": 0.30, "bf16x3": 0.12, "fp32": 0.12}
+MODE_SHARE = {"fp64": 0.25, "i8x5": 0.12, "bf16x3": 0.08, "fp32": 0.07}
 WARMUP_PAIR_BATCHES = 4
 
 
@@ -209,7 +212,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS))
-    ap.add_argument("--precision", default="fp64,bf16x3,fp32",
+    ap.add_argument("--precision", default="fp64,i8x5,bf16x3,fp32",
                     help="comma-separated modes; fp64 always runs first (the error reference)")
     ap.add_argument("--solver", default="cholesky", choices=["cholesky", "lu"])
     ap.add_argument("--integrals", default=None, choices=["host", "device", "generated"],

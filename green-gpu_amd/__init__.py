@@ -22,7 +22,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgreen_gpu_amd.so")
 
-PRECISION = {"fp64": 0, "fp32": 1, "bf16x3": 2}
+PRECISION = {"fp64": 0, "fp32": 1, "bf16x3": 2, "i8x5": 3}
 SOLVER = {"cholesky": 0, "lu": 1}
 RESIDENCE = {"device": 0, "host": 1, "generated": 2}
 
@@ -118,7 +118,7 @@ EXPORTED = ["ggpu_last_error", "ggpu_device_count", "ggpu_device_memory",
 # Launch variants of the contraction GEMMs (ids as the library numbers them):
 # FP64 / FP32 LDS k-depth and waves-per-SIMD register allocation, split-bf16
 # workgroup tiles.
-VARIANTS_MFMA = {8: "bk8", 16: "bk16", 83: "bk8_w3", 84: "bk8_w4"}
+VARIANTS_MFMA = {8: "bk8", 16: "bk16", 83: "bk8_w3", 84: "bk8_w4", 200: "int8"}
 VARIANTS_BF16 = {0: "64x64", 1: "128x64", 4: "64x128", 5: "160x64", 6: "96x64_w3", 7: "64x96_w3", 8: "64x64_w3",
                  9: "64x64_astat"}
 

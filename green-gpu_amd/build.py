@@ -20,7 +20,7 @@ SITE_BENCH = os.path.join(HERE, "gemm_site_bench")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["zgemm.hip", "kernels.hip", "engine.hip", "capi.cpp"]
+SOURCES = ["zgemm.hip", "ozaki.hip", "kernels.hip", "engine.hip", "capi.cpp"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-result"]
 # Device code: no packed FP32 instructions (hipcc would otherwise form
 # v_pk_*_f32 from scalar float code).
@@ -40,7 +40,7 @@ def _compile(src):
 
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
-    with cf.ThreadPoolExecutor(max_workers=min(4, len(SOURCES))) as ex:
+    with cf.ThreadPoolExecutor(max_workers=min(5, len(SOURCES))) as ex:
         results = list(ex.map(_compile, SOURCES))
     for _, msgs in results:
         if verbose and msgs.strip():

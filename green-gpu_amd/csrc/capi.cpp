@@ -82,12 +82,7 @@ int ggpu_hf_set_integral_residence(ggpu_hf_plan* plan, int residence) {
 int ggpu_hf_set_precision(ggpu_hf_plan* plan, int precision) {
   return guarded([&] {
     ggpu::require(plan && plan->impl, "NULL plan");
-    ggpu::require(precision >= GGPU_PRECISION_FP64 && precision <= GGPU_PRECISION_BF16X3, "hf: unknown precision");
-    auto* p = plan->impl;
-    p->compute = precision == GGPU_PRECISION_FP64   ? ggpu::Compute::F64
-                 : precision == GGPU_PRECISION_FP32 ? ggpu::Compute::F32
-                                                    : ggpu::Compute::BF16X3;
-    p->dm32.resize(precision == GGPU_PRECISION_FP64 ? 0 : p->dm_fbz.size());
+    plan->impl->set_precision(precision);
   });
 }
 

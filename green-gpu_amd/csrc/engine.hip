@@ -9,6 +9,7 @@
 //   Pt        (nts, NQ, NQ)                dressed polarization of the current q
 //   work      pair-batch intermediates (X1, X2 for P0; Y1, Y2 for Sigma)
 #include "engine.hpp"
+#include "ozaki.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -536,11 +537,12 @@ GwPlan::GwPlan(int nao_, int nso_, int ns_, int NQ_, int nts_, int nw_b_, const 
   require(nw_b > 0, "gw: nw_b must be positive");
   require(solver == GGPU_SOLVER_CHOLESKY || solver == GGPU_SOLVER_LU, "gw: unknown linear solver");
   require(precision == GGPU_PRECISION_FP64 || precision == GGPU_PRECISION_FP32 ||
-              precision == GGPU_PRECISION_BF16X3,
+              precision == GGPU_PRECISION_BF16X3 || precision == GGPU_PRECISION_INT8X5,
           "gw: unknown precision");
   require(T_wt && T_tw_h, "gw: NULL transform matrices");
-  compute = precision == GGPU_PRECISION_FP64 ? Compute::F64
-            : precision == GGPU_PRECISION_FP32 ? Compute::F32 : Compute::BF16X3;
+  compute = precision == GGPU_PRECISION_FP64   ? Compute::F64
+            : precision == GGPU_PRECISION_FP32 ? Compute::F32
+            : precision == GGPU_PRECISION_INT8X5 ? Compute::I8X5 : Compute::BF16X3;
   sym.load(s, nso, NQ);
   if (x2c)
     for (auto t : sym.k_tr)
@@ -639,6 +641,13 @@ GwPlan::GwPlan(int nao_, int nso_, int ns_, int NQ_, int nts_, int nw_b_, const 
   kb = (int)std::max<int64_t>(1, std::min<int64_t>(budget / per_pair, std::max(sym.nk, sym.ink)));
   kb = std::min(kb, 4096);
   work.resize(((size_t)kb * per_pair + sizeof(z64) - 1) / sizeof(z64));
+  if (compute == Compute::I8X5) {
+    // int8 slices of the operands of one contraction launch (launches whose
+    // slices do not fit are split over their batches)
+    size_t fr = 0, tot = 0;
+    GGPU_HIP(hipMemGetInfo(&fr, &tot));
+    ozwork.resize(std::min<size_t>((size_t)24 << 30, fr / 4));
+  }
   std::vector<int64_t> bo(kb);
   for (int b = 0; b < kb; ++b) bo[b] = b * stride;
   bidx_off.resize(kb);
@@ -1038,8 +1047,14 @@ struct ProfilerGuard {
   ~ProfilerGuard() { set_gemm_profiler(nullptr); }
 };
 struct VariantGuard {
-  explicit VariantGuard(const int* v) { set_site_variants(v); }
-  ~VariantGuard() { set_site_variants(nullptr); }
+  explicit VariantGuard(const int* v, DeviceBuffer<char>* ws = nullptr) {
+    set_site_variants(v);
+    if (ws) set_gemm_workspace(ws->get(), ws->size());
+  }
+  ~VariantGuard() {
+    set_site_variants(nullptr);
+    set_gemm_workspace(nullptr, 0);
+  }
 };
 
 // Sites whose launch variant the autotuner picks: the six contraction sites,
@@ -1085,7 +1100,7 @@ void GwPlan::autotune(const z64* G_dev, int rounds, int batches) {
   for (int r = 0; r < rounds; ++r)
     for (int c = 0; c < nc; ++c) {
       for (int s = 0; s < kNumSites; ++s) trial[s] = tunable_site(s) ? cand[c] : -1;
-      VariantGuard vg(trial);
+      VariantGuard vg(trial, &ozwork);
       tp.reset();
       p0_for_q(iq);
       sigma_for_q(iq);
@@ -1186,7 +1201,7 @@ void GwPlan::iterate_device(const z64* G_dev, z64* S_dev) {
     require(!store.staged(), "gw: graph replay needs the integrals resident in HBM", kErrState);
     require(!profiling, "gw: per-site profiling does not apply to graph replay", kErrState);
     if (!gexec || graph_G != G_dev || graph_S != S_dev) {
-      VariantGuard vguard(variants);
+      VariantGuard vguard(variants, &ozwork);
       capture(G_dev, S_dev);
     }
     GGPU_HIP(hipEventRecord(ev[0], stream));
@@ -1202,7 +1217,7 @@ void GwPlan::iterate_device(const z64* G_dev, z64* S_dev) {
     return;
   }
   ProfilerGuard guard(profiling ? &prof : nullptr);
-  VariantGuard vguard(variants);
+  VariantGuard vguard(variants, &ozwork);
   if (profiling) prof.reset();
   const double streamed0 = store.bytes_streamed(), generated0 = store.bytes_generated();
   (void)store.collect_staging_ms();
@@ -1271,7 +1286,7 @@ void GwBatch::capture() {
     for (size_t i = 0; i < plans.size(); ++i) {
       GwPlan& p = *plans[i];
       GGPU_HIP(hipStreamWaitEvent(p.stream, fork, 0));
-      VariantGuard vg(p.variants);
+      VariantGuard vg(p.variants, &p.ozwork);
       p.enqueue_iteration(G[i], S[i], false);
       GGPU_HIP(hipEventRecord(join[i], p.stream));
       GGPU_HIP(hipStreamWaitEvent(stream, join[i], 0));
@@ -1331,8 +1346,7 @@ void GwPlan::polarization(const double* G, int iq, int which, double* out) {
   require(iq >= 0 && iq < sym.inq, "gw: iq out of range");
   require(which >= 0 && which <= 2, "gw: which must be 0, 1 or 2");
   require(G && out, "gw: NULL pointer");
-  const int64_t Q2 = (int64_t)NQ * NQ;
-  VariantGuard vguard(variants);
+  VariantGuard vguard(variants, &ozwork);
   GGPU_HIP(hipMemcpyAsync(G_host_dev.get(), G, G_host_dev.bytes(), hipMemcpyHostToDevice, stream));
   stage_G(G_host_dev.get());
   zero_int(info.get(), nw_b, stream);
@@ -1497,6 +1511,20 @@ HfPlan::HfPlan(int nao_, int nso_, int ns_, int NQ_, const ggpu_symmetry& s)
   }
 }
 
+void HfPlan::set_precision(int precision) {
+  require(precision >= GGPU_PRECISION_FP64 && precision <= GGPU_PRECISION_INT8X5, "hf: unknown precision");
+  compute = precision == GGPU_PRECISION_FP64     ? Compute::F64
+            : precision == GGPU_PRECISION_FP32   ? Compute::F32
+            : precision == GGPU_PRECISION_BF16X3 ? Compute::BF16X3
+                                                 : Compute::I8X5;
+  dm32.resize(complex64_operands(compute) ? dm_fbz.size() : 0);
+  if (compute == Compute::I8X5 && ozwork.size() == 0) {
+    size_t fr = 0, tot = 0;
+    GGPU_HIP(hipMemGetInfo(&fr, &tot));
+    ozwork.resize(std::min<size_t>((size_t)8 << 30, fr / 8));
+  }
+}
+
 HfPlan::~HfPlan() {
   if (stream) {
     (void)hipStreamSynchronize(stream);
@@ -1512,10 +1540,12 @@ HfPlan::~HfPlan() {
 void HfPlan::exchange_device(z64* kpart, int nsp) {
   const int64_t n2 = (int64_t)nao * nao, stride = (int64_t)NQ * n2;
   const double px = (nps == 1) ? -0.5 : -1.0;   // -1 for ns = 2 and the X2C blocks
-  const bool lp = compute != Compute::F64;
+  const bool lp = complex64_operands(compute);
   const int64_t es = lp ? (int64_t)sizeof(c32) : (int64_t)sizeof(z64);
   char* Vdir = reinterpret_cast<char*>(work.get());
   char* X = Vdir + kb * stride * es;
+  set_gemm_workspace(ozwork.get(), ozwork.size());
+  struct WsReset { ~WsReset() { set_gemm_workspace(nullptr, 0); } } ws_reset;
   const int64_t xs = (int64_t)kb * stride;          // one spin slab of X (elements)
   const Span wk{work.get(), work.bytes() / (size_t)es};
   if (lp) {
@@ -1649,7 +1679,8 @@ void HfPlan::exchange_device_noalloc(const z64* dm_dev, z64* K_dev) {
   require(store.loaded(), "hf: integrals not loaded", kErrState);
   require(dm_dev && K_dev, "hf: NULL pointer");
   require(!store.staged(), "hf no-alloc exchange: needs the integrals resident in HBM", kErrState);
-  require(compute == Compute::F64 || dm32.size() == dm_fbz.size(), "hf: reduced-precision density buffer missing");
+  require(!complex64_operands(compute) || dm32.size() == dm_fbz.size(),
+          "hf: reduced-precision density buffer missing");
   const int64_t n2 = (int64_t)nao * nao, so2 = (int64_t)nso * nso;
   const int64_t slice = (int64_t)nps * n2;
   require(slice <= scratch_elems,

@@ -176,6 +176,7 @@ struct GwPlan {
   DeviceBuffer<int> info;
   DeviceBuffer<int> ipiv;                // LU pivots, (nw_b, NQ)
   DeviceBuffer<z64> work;
+  DeviceBuffer<char> ozwork;            // slices and scales of the int8-emulated FP64 products
   DeviceBuffer<int64_t> bidx_off;       // b * NQ * nao^2
   DeviceBuffer<int64_t> sig_src, sig_dst;
   std::vector<PairTable> p0_tables;     // per IBZ q
@@ -219,7 +220,7 @@ struct GwPlan {
          const double* T_tw, int precision, int solver, size_t work_bytes);
   ~GwPlan();
 
-  bool lowp() const { return compute != Compute::F64; }
+  bool lowp() const { return complex64_operands(compute); }
   void iterate_device(const z64* G_dev_hostlayout, z64* Sigma_dev_hostlayout);
   void iterate(const double* G, double* Sigma);
   void polarization(const double* G, int iq, int which, double* out);
@@ -290,6 +291,8 @@ struct HfPlan {
   // partial sums of K, J and the Ewald term are always FP64.
   Compute compute = Compute::F64;
   DeviceBuffer<c32> dm32;               // complex64 copy of dm_fbz (reduced-precision modes)
+  DeviceBuffer<char> ozwork;            // slices and scales of the int8-emulated FP64 products
+  void set_precision(int precision);
   // This device's share of the IBZ k-points: ik = share_rank, share_rank +
   // share_size, ...; the other rows of K and F are zero, so the results of
   // all shares sum to the full ones.

@@ -20,6 +20,7 @@
 // the launch variant per site comes from the built-in choice, a per-thread
 // table (the plan's autotuned choice) or a global override.
 #include "zgemm.hpp"
+#include "ozaki.hpp"
 
 #include <algorithm>
 #include <cstdio>
@@ -970,6 +971,8 @@ void launch_typed(const GemmDesc& d, hipStream_t stream, int v) {
 }
 
 thread_local GemmProfiler* g_profiler = nullptr;
+thread_local void* g_workspace = nullptr;
+thread_local size_t g_workspace_bytes = 0;
 thread_local const int* g_site_variants = nullptr;   // [kNumSites], -1: built-in choice
 
 // Range of element offsets (relative to ptr) touched by a 2-D view with the
@@ -1013,7 +1016,7 @@ void gemm_check_bounds(const GemmDesc& d) {
   extend(rb, d.nb0, d.B.bs0); if (!d.B.off1) extend(rb, d.nb1, d.B.bs1);
   extend(rc, d.M, d.C.rs); extend(rc, d.N, d.C.cs);
   extend(rc, d.nb0, d.C.bs0); if (!d.C.off1) extend(rc, d.nb1, d.C.bs1);
-  const size_t ein = (d.compute == Compute::F64) ? sizeof(double2) : sizeof(float2);
+  const size_t ein = complex64_operands(d.compute) ? sizeof(float2) : sizeof(double2);
   const size_t eout = d.out_f32 ? sizeof(float2) : sizeof(double2);
   require(!d.A.off1 || d.A.off1_host || !d.A.buf, "gemm: A offset table without host copy");
   require(!d.B.off1 || d.B.off1_host || !d.B.buf, "gemm: B offset table without host copy");
@@ -1027,12 +1030,21 @@ void gemm_check_bounds(const GemmDesc& d) {
 
 void set_gemm_profiler(GemmProfiler* p) { g_profiler = p; }
 void set_site_variants(const int* table) { g_site_variants = table; }
+void set_gemm_workspace(void* ptr, size_t bytes) {
+  g_workspace = ptr;
+  g_workspace_bytes = ptr ? bytes : 0;
+}
 
 int gemm_variant_candidates(Compute c, int* out, int cap) {
   static const int f64[] = {83, 8, 16};
   static const int f32[] = {8, 16, 83};
   static const int bf[] = {kBf16Tile128x64, kBf16Tile64x128, kBf16Tile160x64, kBf16Tile96x64W3,
                            kBf16Tile64x96W3, kBf16Tile64x64W3, kBf16Tile64x64, kBf16TileAstat64x64};
+  if (c == Compute::I8X5) {              // int8 emulation or the FP64 kernel
+    static const int i8[] = {kVariantInt8, 83};
+    for (int i = 0; i < 2 && i < cap; ++i) out[i] = i8[i];
+    return 2;
+  }
   const int* v = c == Compute::F64 ? f64 : c == Compute::F32 ? f32 : bf;
   const int n = c == Compute::F64 ? 3 : c == Compute::F32 ? 3 : 8;
   for (int i = 0; i < n && i < cap; ++i) out[i] = v[i];
@@ -1068,6 +1080,19 @@ void gemm(const GemmDesc& dd, hipStream_t stream) {
       if (d.out_f32) launch_bf16x3<float2>(d, stream, v);
       else launch_bf16x3<double2>(d, stream, v);
       break;
+    case Compute::I8X5: {
+      const bool emulate =
+          v == kVariantInt8 || (v == -1 && (d.site == kSiteP0 || d.site == kSiteSigY2 || d.site == kSiteHF));
+      if (emulate) {
+        gemm_int8_emulated(d, g_workspace, g_workspace_bytes, stream);
+      } else {
+        require(!d.out_f32, "gemm: FP64 kernel writes complex128");
+        GemmDesc f = d;
+        f.compute = Compute::F64;
+        launch_typed<double, double2>(f, stream, v == kVariantInt8 ? -1 : v);
+      }
+      break;
+    }
     default:
       throw Error(kErrArgument, "gemm: compute mode not available");
   }

@@ -20,7 +20,14 @@ enum class Compute : int {
   F32 = 1,     // complex64 operands, v_mfma_f32_16x16x4_f32, FP32 accumulation
   BF16X3 = 2,  // complex64 operands split hi+lo into bf16, 3 products on
                // v_mfma_f32_16x16x32_bf16, FP32 accumulation
+  I8X5 = 3,    // complex128 operands split into five signed 7-bit slices,
+               // exact int32 slice products (s + t <= 5) on
+               // v_mfma_i32_16x16x64_i8, FP64 recombination: ~2^-36 of the
+               // operand scale
 };
+
+// Operands of this compute mode are complex64 (else complex128).
+inline bool complex64_operands(Compute c) { return c == Compute::F32 || c == Compute::BF16X3; }
 
 struct Operand {
   const void* ptr = nullptr;   // element 0 of batch (0,0)
@@ -85,6 +92,12 @@ enum {
   kBf16TileAstat64x64 = 9
 };
 void set_bf16_tiles(int mode);
+// Variant of a Compute::I8X5 launch that runs the int8 emulation; the FP64
+// kernel variants (8, 16, 83, 84) run the launch on the FP64 kernel instead
+// (both FP64-accurate).  Built-in choice: the emulation at the two large-K
+// contractions (P0 contraction, Sigma P.Y1) and the HF products, the FP64
+// kernel at the other GW sites.
+enum { kVariantInt8 = 200 };
 // Variant of the FP64 / FP32 kernel: 0 picks per call site; 8 or 16 forces
 // the LDS k-depth; 83 or 84 forces k-depth 8 with registers allocated for 3 or
 // 4 waves per SIMD.  All variants give the same sums.
@@ -94,6 +107,9 @@ void set_mfma_bk(int variant);
 // an LDS k-depth / waves-per-SIMD value of set_mfma_bk for FP64 and FP32,
 // a kBf16Tile* value for split-bf16.  The global switches above win.
 void set_site_variants(const int* table);
+// Device workspace of the Compute::I8X5 launches of the calling thread
+// (nullptr: none; such launches then fail with an error).
+void set_gemm_workspace(void* ptr, size_t bytes);
 // Variants worth timing for a compute mode (the autotuner's candidates);
 // returns their count, writes at most cap of them.
 int gemm_variant_candidates(Compute c, int* out, int cap);

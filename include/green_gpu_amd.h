@@ -47,7 +47,12 @@ typedef enum { GGPU_SOLVER_CHOLESKY = 0, GGPU_SOLVER_LU = 1 } ggpu_linear_solver
 typedef enum {
   GGPU_PRECISION_FP64 = 0,    /* reference "double" path (gw_gpu_kernel.cpp:278-284, _sp = false) */
   GGPU_PRECISION_FP32 = 1,    /* complex64 contractions (reference _sp = P_sp && Sigma_sp)        */
-  GGPU_PRECISION_BF16X3 = 2   /* complex64 data, split-bf16 x3 products, FP32 accumulation        */
+  GGPU_PRECISION_BF16X3 = 2,  /* complex64 data, split-bf16 x3 products, FP32 accumulation        */
+  GGPU_PRECISION_INT8X5 = 3   /* complex128 data; products emulated on the int8 matrix cores: five
+                                 signed 7-bit slices per element (one power-of-two scale per row /
+                                 column), exact int32 slice products up to slice diagonal 5 (19
+                                 pairs), FP64 recombination and accumulation; ~1e-10 relative per
+                                 product.  No counterpart in the reference (FP64 or complex64)   */
 } ggpu_precision;
 
 /* Where a plan keeps the stored integrals.  HOST: in pinned host memory,
