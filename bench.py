@@ -402,9 +402,14 @@ def main():
         traffic = json.load(open(tf)).get(a.workload, {}).get(head, {}).get(dom_name)
     roofline = {"bound": "mfma", "kernel": f"gemm site {dom_name}", "achieved": round(achieved, 3),
                 "peak": PEAK[dtype], "unit": "TFLOP/s", "frac": round(achieved / PEAK[dtype], 4),
+                # matrix-core operations the kernel executes per algorithmic FLOP
+                "frac_executed": round(achieved * MFMA_PER_ALG[head] / PEAK[dtype], 4),
                 "traffic": None if traffic is None else round(traffic["bytes"]),
                 "traffic_source": None if traffic is None else traffic["source"],
                 "launches": dom_n, "avg_launch_ms": round(dom_ms / dom_n, 4)}
+    if head == "i8x5":
+        # FP64-accurate products: the same FLOPs against the FP64 matrix peak
+        roofline["fp64_equivalent_frac"] = round(achieved / PEAK["f64"], 4)
     step_name = f"one q of {sym.inq}" if per_q else "one GW iteration"
     if a.pair_batches:
         step_name += f", first {a.pair_batches} pair batches per chain only (partial, profiling)"

@@ -51,6 +51,7 @@ def lib():
         L.ggpu_hf_plan_create.argtypes = [C.POINTER(vp), i32, i32, i32, i32, C.POINTER(SymmetryC)]
         L.ggpu_hf_set_integral_residence.argtypes = [vp, i32]
         L.ggpu_hf_set_k_share.argtypes = [vp, i32, i32]
+        L.ggpu_hf_set_variants.argtypes = [vp, C.POINTER(C.c_int)]
         L.ggpu_hf_set_precision.argtypes = [vp, i32]
         L.ggpu_hf_load_integrals.argtypes = [vp, pd]
         L.ggpu_hf_synth_integrals.argtypes = [vp, u64, f64]
@@ -97,7 +98,7 @@ def lib():
 
 
 EXPORTED = ["ggpu_last_error", "ggpu_device_count", "ggpu_device_memory",
-            "ggpu_hf_plan_create", "ggpu_hf_set_integral_residence", "ggpu_hf_set_k_share",
+            "ggpu_hf_plan_create", "ggpu_hf_set_integral_residence", "ggpu_hf_set_k_share", "ggpu_hf_set_variants",
             "ggpu_hf_set_precision",
             "ggpu_hf_load_integrals",
             "ggpu_hf_synth_integrals",
@@ -441,6 +442,12 @@ class HfKernel:
         """Compute only the rows of IBZ k = rank, rank + size, ...
         (cuhf_utils.cu:140); the other rows are zero."""
         _check(lib().ggpu_hf_set_k_share(self._h, int(rank), int(size)))
+
+    def set_variants(self, variants):
+        """Launch variant per call site (only site "hf"; 200: int8 emulation,
+        83: FP64 kernel, -1: built-in) for the exchange products."""
+        v = (C.c_int * len(SITES))(*[int(x) for x in variants])
+        _check(lib().ggpu_hf_set_variants(self._h, v))
 
     def exchange(self, dm):
         dm = _c128(dm, self.mat_shape)

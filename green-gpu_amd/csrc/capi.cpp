@@ -86,6 +86,18 @@ int ggpu_hf_set_precision(ggpu_hf_plan* plan, int precision) {
   });
 }
 
+int ggpu_hf_set_variants(ggpu_hf_plan* plan, const int* variants) {
+  return guarded([&] {
+    ggpu::require(plan && plan->impl && variants, "NULL argument");
+    ggpu::HfPlan& p = *plan->impl;
+    for (int s = 0; s < ggpu::kNumSites; ++s)
+      ggpu::require(variants[s] == -1 || (s == ggpu::kSiteHF && ggpu::gemm_variant_valid(p.compute, variants[s])),
+                    "hf: launch variant " + std::to_string(variants[s]) + " not available for call site " +
+                        std::to_string(s) + " in this plan's precision");
+    std::copy(variants, variants + ggpu::kNumSites, p.variants);
+  });
+}
+
 int ggpu_hf_set_k_share(ggpu_hf_plan* plan, int rank, int size) {
   return guarded([&] {
     ggpu::require(plan && plan->impl, "NULL plan");

@@ -1431,6 +1431,7 @@ HfPlan::HfPlan(int nao_, int nso_, int ns_, int NQ_, const ggpu_symmetry& s)
   x2c = nso == 2 * nao;
   require(!x2c || ns == 1, "hf: a two-component (nso = 2 nao) plan needs ns = 1");
   if (x2c) nps = 3;   // density blocks aa, bb, ab (cuhf_utils with pseudo spin 3)
+  std::fill(variants, variants + kNumSites, -1);
   sym.load(s, nso, NQ);
   if (x2c)
     for (auto t : sym.k_tr)
@@ -1544,8 +1545,7 @@ void HfPlan::exchange_device(z64* kpart, int nsp) {
   const int64_t es = lp ? (int64_t)sizeof(c32) : (int64_t)sizeof(z64);
   char* Vdir = reinterpret_cast<char*>(work.get());
   char* X = Vdir + kb * stride * es;
-  set_gemm_workspace(ozwork.get(), ozwork.size());
-  struct WsReset { ~WsReset() { set_gemm_workspace(nullptr, 0); } } ws_reset;
+  VariantGuard vguard(variants, &ozwork);
   const int64_t xs = (int64_t)kb * stride;          // one spin slab of X (elements)
   const Span wk{work.get(), work.bytes() / (size_t)es};
   if (lp) {

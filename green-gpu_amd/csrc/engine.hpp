@@ -293,6 +293,8 @@ struct HfPlan {
   DeviceBuffer<c32> dm32;               // complex64 copy of dm_fbz (reduced-precision modes)
   DeviceBuffer<char> ozwork;            // slices and scales of the int8-emulated FP64 products
   void set_precision(int precision);
+  // Launch variant per call site of the exchange products (-1: built-in).
+  int variants[kNumSites];
   // This device's share of the IBZ k-points: ik = share_rank, share_rank +
   // share_size, ...; the other rows of K and F are zero, so the results of
   // all shares sum to the full ones.

@@ -1081,8 +1081,7 @@ void gemm(const GemmDesc& dd, hipStream_t stream) {
       else launch_bf16x3<double2>(d, stream, v);
       break;
     case Compute::I8X5: {
-      const bool emulate =
-          v == kVariantInt8 || (v == -1 && (d.site == kSiteP0 || d.site == kSiteSigY2 || d.site == kSiteHF));
+      const bool emulate = v == kVariantInt8 || (v == -1 && d.site == kSiteSigY2);
       if (emulate) {
         gemm_int8_emulated(d, g_workspace, g_workspace_bytes, stream);
       } else {

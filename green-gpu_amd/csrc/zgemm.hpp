@@ -94,9 +94,9 @@ enum {
 void set_bf16_tiles(int mode);
 // Variant of a Compute::I8X5 launch that runs the int8 emulation; the FP64
 // kernel variants (8, 16, 83, 84) run the launch on the FP64 kernel instead
-// (both FP64-accurate).  Built-in choice: the emulation at the two large-K
-// contractions (P0 contraction, Sigma P.Y1) and the HF products, the FP64
-// kernel at the other GW sites.
+// (both FP64-accurate).  Built-in choice: the emulation at the Sigma P.Y1
+// product (the one site where it is faster on the benchmark shapes), the
+// FP64 kernel everywhere else; per-site variants override it.
 enum { kVariantInt8 = 200 };
 // Variant of the FP64 / FP32 kernel: 0 picks per call site; 8 or 16 forces
 // the LDS k-depth; 83 or 84 forces k-depth 8 with registers allocated for 3 or

@@ -10,8 +10,8 @@ Dyson solve (with cancellation in the sums); measured 4e-10 .. 9e-9 of
 max |Sigma| (the strongly coupled two-k fixture systems amplify most), so
 Sigma and K are held to 2e-8 of their maximum -- three orders of magnitude
 tighter than complex64 arithmetic on the same systems.  The plans emulate
-every contraction site here (the built-in choice emulates only the two
-large-K ones)."""
+every contraction site here (the built-in choice emulates only the Sigma
+P.Y1 product)."""
 import os
 
 import numpy as np
@@ -103,6 +103,9 @@ def test_int8_emulation_at_workload_shapes(ggpu, synth, nao, NQ):
     k.close()
     h = ggpu.HfKernel(nao, ns, NQ, sym, precision="i8x5")
     h.synth_integrals(int(f["seed_v"]), 0.1)
+    v = [-1] * len(ggpu.SITES)
+    v[ggpu.SITES.index("hf")] = 200
+    h.set_variants(v)
     K = h.exchange(D)
     h.close()
     scale, kscale = float(f["sigma_abs_max"]), float(np.abs(f["K"]).max())
@@ -122,6 +125,9 @@ def test_hf_int8_emulation_fixture(ggpu, oracle, fx):
     dm = fx["hf_G_tau"][-1]
     h = ggpu.HfKernel(nao, ns, NQ, sym, madelung=float(fx["madelung"]), S_k=S_ibz, precision="i8x5")
     h.load_integrals(fx["hf_VQ_0"])
+    v = [-1] * len(ggpu.SITES)
+    v[ggpu.SITES.index("hf")] = 200            # emulate the exchange products
+    h.set_variants(v)
     F = h.solve(dm)
     F_or = oracle.hf_solve(dm, fx["hf_VQ_0"], sym, ns, S_ibz, float(fx["madelung"]))
     Kf = oracle.hf_exchange(dm, fx["hf_VQ_0"], sym, ns)
