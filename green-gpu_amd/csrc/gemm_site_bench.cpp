@@ -1,13 +1,16 @@
 // GEMM call-site benchmark: launches the six GEMM call sites of one GW
-// iteration with the shapes, strides and batching the engine uses at
-// nao = 52, NQ = 300, ns = 2, nts = 110 (9 k-pairs per batch), through the
-// library's own dispatch, and prints the per-launch median time of each
-// (HIP events on the launch stream).  Profilers attribute the same kernels
-// (one kernel symbol per call site), so their per-kernel durations can be
-// compared with these.
+// iteration with the shapes, strides and batching the engine uses (pairs per
+// batch kb and P0 partial slices as the plan chooses them; defaults: the
+// nk = 216 system, nao = 104, NQ = 600, ns = 2, nts = 110, one pair per
+// batch, 8 P0 slices), through the library's own dispatch, and prints the
+// per-launch median time of each (HIP events on the launch stream).
+// Profilers attribute the same kernels (one kernel symbol per call site), so
+// their per-kernel durations can be compared with these.
 //
-//   gemm_site_bench <fp64|fp32|bf16x3> [repetitions] [--calibrate-copy]
+//   gemm_site_bench <fp64|fp32|bf16x3|i8x5> [--nao N] [--nq N] [--nts N] [--kb N]
+//                   [--p0-split S] [--reps R] [--sites a,b,..] [--calibrate-copy]
 //
+// i8x5 forces the int8 emulation at every site (the FP64 kernel is fp64).
 // --calibrate-copy ends with one plain 1 GiB device-to-device copy, the
 // reference point for scaling HBM byte counters.
 #include <hip/hip_runtime.h>
@@ -71,26 +74,50 @@ int64_t extent(int64_t r, int64_t nr, int64_t c, int64_t nc, int64_t kb, int64_t
 
 int main(int argc, char** argv) {
   if (argc < 2) {
-    std::fprintf(stderr, "usage: %s <fp64|fp32|bf16x3> [repetitions] [--calibrate-copy]\n", argv[0]);
+    std::fprintf(stderr,
+                 "usage: %s <fp64|fp32|bf16x3|i8x5> [--nao N] [--nq N] [--nts N] [--kb N] [--p0-split S] "
+                 "[--reps R] [--sites a,b] [--calibrate-copy]\n",
+                 argv[0]);
     return 2;
   }
   const std::string mode = argv[1];
-  const Compute comp = mode == "fp64" ? Compute::F64 : mode == "fp32" ? Compute::F32 : Compute::BF16X3;
-  if (mode != "fp64" && mode != "fp32" && mode != "bf16x3") {
+  if (mode != "fp64" && mode != "fp32" && mode != "bf16x3" && mode != "i8x5") {
     std::fprintf(stderr, "unknown mode %s\n", mode.c_str());
     return 2;
   }
-  int reps = 20;
+  const Compute comp = mode == "fp64" ? Compute::F64 : mode == "fp32" ? Compute::F32
+                       : mode == "i8x5" ? Compute::I8X5 : Compute::BF16X3;
+  int reps = 10, nao = 104, NQ = 600, nts = 110, kb = 1, S = 8;
+  std::string only;
   bool calibrate = false;
   for (int i = 2; i < argc; ++i) {
-    if (!std::strcmp(argv[i], "--calibrate-copy")) calibrate = true;
-    else reps = std::max(1, std::atoi(argv[i]));
+    const std::string a = argv[i];
+    auto next = [&]() { return i + 1 < argc ? std::atoi(argv[++i]) : 0; };
+    if (a == "--calibrate-copy") calibrate = true;
+    else if (a == "--nao") nao = next();
+    else if (a == "--nq") NQ = next();
+    else if (a == "--nts") nts = next();
+    else if (a == "--kb") kb = next();
+    else if (a == "--p0-split") S = next();
+    else if (a == "--reps") reps = std::max(1, next());
+    else if (a == "--sites" && i + 1 < argc) only = argv[++i];
+    else {
+      std::fprintf(stderr, "unknown argument %s\n", a.c_str());
+      return 2;
+    }
   }
-
-  const int nao = 52, NQ = 300, nts = 110, half = nts / 2, ns = 2, kb = 9, S = 4;
+  if (nao < 1 || NQ < 1 || nts < 4 || nts % 2 || kb < 1 || S < 1) {
+    std::fprintf(stderr, "bad sizes\n");
+    return 2;
+  }
+  const int half = nts / 2, ns = 2;
   const int64_t n2 = (int64_t)nao * nao, stride = (int64_t)NQ * n2, Q2 = (int64_t)NQ * NQ;
   const int64_t xs = (int64_t)half * stride, ys = (int64_t)nts * stride;
-  const int full = kb * ns / S;   // P0 partial slices: K-batch per slice
+  // P0 partial slices as GwPlan::p0_for_q forms them: E = kb ns (pair, spin)
+  // entries, slice j sums entries j, j + S, ...; with fewer entries than
+  // slices every entry is its own slice
+  const int E = kb * ns, full = E / S;
+  const int p0_kb = full > 0 ? full : 1, p0_slices = full > 0 ? S : E;
   const std::vector<CallSite> sites = {
       // X1 = G(k, beta - tau) V (one pair, one spin)
       {"p0_gv", kSiteP0X1, half * nao, nao * NQ, nao, 1, 1, 1, nao, 1, 0, 0, 0, (int64_t)nao * NQ, 1, 0, 0, 0,
@@ -99,8 +126,8 @@ int main(int argc, char** argv) {
       {"p0_gvh", kSiteP0X2, NQ * nao, nao, nao, 1, half, 1, nao, 1, 0, 0, 0, 1, nao, 0, n2, 0, nao, 1, stride, 0, 1,
        1, 1.0, 0.0},
       // P0 partial slices += X1 . X2 over the (pair, spin) K-batch
-      {"p0_contract", kSiteP0, NQ, NQ, (int)n2, full, half, S, 1, NQ, S * xs, stride, xs, 1, n2, S * xs, stride, xs,
-       NQ, 1, Q2, half * Q2, 0, 0, -1.0, 1.0},
+      {"p0_contract", kSiteP0, NQ, NQ, (int)n2, p0_kb, half, p0_slices, 1, NQ, (int64_t)S * xs, stride, xs, 1, n2,
+       (int64_t)S * xs, stride, xs, NQ, 1, Q2, half * Q2, 0, 0, -1.0, 1.0},
       // Y1 = V G(k-q, tau) per tau
       {"sig_vg", kSiteSigY1, NQ * nao, nao, nao, 1, nts, 1, nao, 1, 0, 0, 0, nao, 1, 0, n2, 0, nao, 1, stride, 0, 0,
        1, 1.0, 0.0},
@@ -111,12 +138,26 @@ int main(int argc, char** argv) {
       {"sig_contract", kSiteSigma, nts * nao, nao, nao, NQ, kb, ns, nao, 1, (int64_t)nts * n2, ns * ys, ys, 1, nao,
        n2, stride, 0, nao, 1, (int64_t)ns * nts * n2, (int64_t)nts * n2, 2, 0, -1.0, 1.0},
   };
+  // the int8 emulation: a workspace and the emulated variant at every site
+  void* work = nullptr;
+  size_t work_bytes = 0;
+  int variants[kNumSites];
+  std::fill(variants, variants + kNumSites, comp == Compute::I8X5 ? (int)kVariantInt8 : -1);
+  if (comp == Compute::I8X5) {
+    size_t fr = 0, tot = 0;
+    check(hipMemGetInfo(&fr, &tot), "meminfo");
+    work_bytes = std::min<size_t>((size_t)24 << 30, fr / 4);
+    check(hipMalloc(&work, work_bytes), "malloc workspace");
+    set_gemm_workspace(work, work_bytes);
+    set_site_variants(variants);
+  }
 
-  std::printf("GEMM call sites, nao=%d NQ=%d ns=%d nts=%d, %d pairs per batch, mode %s, %d repetitions\n", nao, NQ,
-              ns, nts, kb, mode.c_str(), reps);
+  std::printf("GEMM call sites, nao=%d NQ=%d ns=%d nts=%d, %d pairs per batch, %d P0 slices, mode %s, %d repetitions\n",
+              nao, NQ, ns, nts, kb, S, mode.c_str(), reps);
   std::printf("%-13s %8s %8s %8s %5s %6s %12s %10s\n", "site", "M", "N", "K", "KB", "batch", "ms/launch", "TFLOP/s");
   for (const CallSite& s : sites) {
-    const bool low = comp != Compute::F64;
+    if (!only.empty() && ("," + only + ",").find("," + std::string(s.name) + ",") == std::string::npos) continue;
+    const bool low = complex64_operands(comp);
     const size_t ein = low ? 8 : 16, eout = (low && s.out_low) ? 8 : 16;
     const int64_t ea = extent(s.ars, s.M, s.acs, s.K, s.akbs, s.KB, s.abs0, s.nb0, s.abs1, s.nb1);
     const int64_t eb = extent(s.brs, s.K, s.bcs, s.N, s.bkbs, s.KB, s.bbs0, s.nb0, s.bbs1, s.nb1);
@@ -178,6 +219,11 @@ int main(int argc, char** argv) {
     hipFree(A);
     hipFree(B);
     hipFree(C);
+  }
+  if (work) {
+    set_gemm_workspace(nullptr, 0);
+    set_site_variants(nullptr);
+    hipFree(work);
   }
   if (calibrate) {
     const size_t bytes = (size_t)1 << 30;
