@@ -11,14 +11,22 @@
 // The dropped terms (s + t > 5, and the slice remainders) leave a relative
 // error of about 2^-36 of max|A_m| max|B_n| K per element (a numpy model on
 // Gaussian operands: ~2e-10 of max|C|), far below complex64 arithmetic.
-// Complex products use four real products per slice pair with the
-// imaginary part of A also stored negated, so re and im accumulate exactly:
+// Complex products use four real products per slice pair; the negated
+// imaginary slice of A is formed in registers (bytewise), so re and im
+// accumulate exactly:
 //   re += Ar_s Br_t + (-Ai_s) Bi_t,   im += Ar_s Bi_t + Ai_s Br_t.
 //
-// A launch runs three kernels: the per-row / per-column maximum binary
-// exponent (over K and the K-batch), the slicing of each operand into a
-// canonical int8 layout, and the int8 GEMM with the FP64 epilogue (alpha,
-// beta, strided or complex64 output).
+// Slices are stored fragment-native: for every (batch, k block of 64, block
+// of 16 rows, plane) one 1 KiB fragment whose lane l holds row l % 16 and the
+// 16 k bytes 16 (l / 16) .. + 15 -- the operand layout of the MFMA, so a
+// fragment moves from HBM to LDS and from LDS to registers as one contiguous
+// 1 KiB wave access (LDS reads conflict-free, no padding).  Planes: re and im
+// of every slice (10).
+//
+// A launch runs the slicing of both operands (per-row / per-column maximum
+// binary exponent, then the digits; one fused pass when the summed k extent
+// is short, an exponent pass and a digit pass otherwise) and the int8 GEMM
+// with the FP64 epilogue (alpha, beta, strided or complex64 output).
 #include "ozaki.hpp"
 
 #include <algorithm>
@@ -31,18 +39,21 @@ namespace {
 
 constexpr int NSL = 5;              // slices per operand
 constexpr int DMAX = 5;             // kept slice pairs: s + t <= DMAX (19 pairs)
-constexpr int PA = 3 * NSL;         // A planes per k block: re, im, -im of every slice
-constexpr int PB = 2 * NSL;         // B planes: re, im of every slice
+constexpr int NP = 2 * NSL;         // planes per operand: re and im of every slice
 constexpr int KBLK = 64;            // k per block: one v_mfma_i32_16x16x64_i8
 constexpr int TILE = 64;            // rows of a workgroup tile (both operands)
+constexpr int RB = 16;              // rows of a fragment
+constexpr int FRAG = 1024;          // bytes of a fragment (64 lanes x 16 bytes)
 constexpr int NT = 256;
-constexpr int LDS_PAD = 16;         // bytes of padding per LDS row (bank spread)
-constexpr int ROWA = PA * KBLK;     // bytes of one A row of a k block in global memory
-constexpr int ROWB = PB * KBLK;
+constexpr int STAGE_A = (TILE / RB) * NP * FRAG;   // one operand of a tile for one k block (40 KiB)
+constexpr int STAGE = 2 * STAGE_A;                 // A and B (80 KiB); two stages fill the LDS
 // Largest K * KB one launch may sum: every int32 diagonal accumulator holds at
 // most NSL pairs * 2 products * K * 127^2 < 2^31 (no diagonal has more than
 // NSL slice pairs).
 constexpr int64_t KMAX = 12800;
+// Largest summed k extent sliced in one fused pass (the rows' second read
+// then comes from cache).
+constexpr int FUSE_KT = 2048;
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 
@@ -52,105 +63,134 @@ struct SliceView {
   const int64_t* off1 = nullptr;  // device: offset of batch b1 (replaces bs1)
   int rows = 0, K = 0, KB = 1, nb1 = 1;
   int conj = 0;
-  int nplanes = 0;                // PA (A side, re / im / -im) or PB (B side)
   int Rp = 0, nkb = 0;            // padded rows, k blocks
   int* E = nullptr;               // [batch][Rp]: max binary exponent (INT_MIN: all zero)
-  int8_t* out = nullptr;          // [batch][kblock][Rp][nplanes][64]
+  int8_t* out = nullptr;          // [batch][kblock][Rp / 16][NP][64 lanes][16 bytes]
 };
 
-// Load the 64 x 64 tile (rows r0.., flattened k k0..) of batch b into LDS,
-// zero outside the operand; lanes run along whichever index is contiguous.
-// The flattened k of one launch stays below 2^31 (K * KB <= 12800).
-__device__ inline void load_tile(const SliceView& v, int64_t b, int r0, int k0, double2 (*T)[KBLK + 1]) {
-  const int t = threadIdx.x;
-  const int kt = v.K * v.KB;
-  const bool k_fast = v.k_s == 1 || v.row_s != 1;
+__device__ inline const double2* batch_base(const SliceView& v, int64_t b) {
   const int64_t b0 = b / v.nb1, b1 = b - b0 * v.nb1;
-  const double2* base = v.src + b0 * v.bs0 + (v.off1 ? v.off1[b1] : b1 * v.bs1);
-  for (int it = 0; it < TILE * KBLK / NT; ++it) {
-    int r, c;
-    if (k_fast) { r = it * (NT / KBLK) + t / KBLK; c = t % KBLK; }
-    else { c = it * (NT / TILE) + t / TILE; r = t % TILE; }
-    const int row = r0 + r, kk = k0 + c;
-    double2 x = make_double2(0.0, 0.0);
-    if (row < v.rows && kk < kt) {
-      const int kb = kk / v.K, k = kk - kb * v.K;
-      x = base[(int64_t)row * v.row_s + (int64_t)k * v.k_s + (int64_t)kb * v.kb_s];
-    }
-    T[r][c] = x;
-  }
+  return v.src + b0 * v.bs0 + (v.off1 ? v.off1[b1] : b1 * v.bs1);
 }
 
-__device__ inline int max_exponent(double a) {
+// Element (row, flattened k) of one batch; zero outside the operand.  The
+// flattened k of one launch stays below 2^31 (K * KB <= 12800).
+__device__ inline double2 element(const SliceView& v, const double2* base, int row, int kk) {
+  if (row >= v.rows || kk >= v.K * v.KB) return make_double2(0.0, 0.0);
+  const int kb = kk / v.K, k = kk - kb * v.K;
+  return base[(int64_t)row * v.row_s + (int64_t)k * v.k_s + (int64_t)kb * v.kb_s];
+}
+
+__device__ inline int max_exponent(double2 x) {
+  const double a = fmax(fabs(x.x), fabs(x.y));
   if (a == 0.0) return INT_MIN;
   int e;
   (void)frexp(a, &e);
   return e;
 }
 
-// E[b][row] = max over (k, kb) of the binary exponent of max(|re|, |im|).
-// Grid: (k blocks, row tiles, batches); E starts at INT_MIN.
+__device__ inline int sigma_of(int e) { return e == INT_MIN ? 0 : 7 - e; }
+
+// Lanes run along whichever index is contiguous in memory.
+__device__ inline bool k_fast(const SliceView& v) { return v.k_s == 1 || v.row_s != 1; }
+
+// E[b][row] = max over (k, kb) of the binary exponent of max(|re|, |im|),
+// for operands with a long k extent.  Grid: (k blocks, row tiles of 64,
+// batches); E starts at INT_MIN.
 __global__ __launch_bounds__(NT) void oz_exponent_kernel(const SliceView v) {
-  __shared__ double2 T[TILE][KBLK + 1];
-  __shared__ int red[NT];
+  __shared__ int red[TILE][KBLK + 1];
   const int64_t b = blockIdx.z;
-  const int r0 = blockIdx.y * TILE;
-  const int k0 = blockIdx.x * KBLK;
-  load_tile(v, b, r0, k0, T);
-  __syncthreads();
-  const int t = threadIdx.x, r = t / 4, q = t % 4;
-  int e = INT_MIN;
-  for (int j = 0; j < 16; ++j) {
-    const double2 x = T[r][16 * q + j];
-    e = max(e, max_exponent(fmax(fabs(x.x), fabs(x.y))));
+  const double2* base = batch_base(v, b);
+  const int r0 = blockIdx.y * TILE, k0 = blockIdx.x * KBLK, t = threadIdx.x;
+  // a wave reads 64 consecutive elements of one row (k contiguous) or of
+  // one k (rows contiguous)
+  const bool kf = k_fast(v);
+  for (int j = 0; j < TILE * KBLK / NT; ++j) {
+    const int r = kf ? (t >> 6) + 4 * j : t & 63, c = kf ? t & 63 : (t >> 6) + 4 * j;
+    red[r][c] = max_exponent(element(v, base, r0 + r, k0 + c));
   }
-  red[t] = e;
   __syncthreads();
-  if (q == 0) {
-    e = max(max(red[t], red[t + 1]), max(red[t + 2], red[t + 3]));
-    if (e != INT_MIN && r0 + r < v.rows) atomicMax(&v.E[b * v.Rp + r0 + r], e);
+  if (t < TILE) {
+    int e = INT_MIN;
+    for (int c = 0; c < KBLK; ++c) e = max(e, red[t][c]);
+    if (e != INT_MIN && r0 + t < v.rows) atomicMax(&v.E[b * v.Rp + r0 + t], e);
   }
 }
 
-__device__ inline int sigma_of(int e) { return e == INT_MIN ? 0 : 7 - e; }
-
-// Canonical slices: thread (row r, quarter q) turns 16 consecutive k of its
-// row into NSL digits of re and im, packed 16 bytes per plane.
+// Digits of one block of 16 rows: wave w of the workgroup takes k blocks
+// w, w + 4, .. (FUSED) or the one k block 4 blockIdx.x + w; lane (lr, lk)
+// loads the 16 k of its fragment lane (row lr, k 16 lk ..) straight into
+// registers, turns them into NSL digits of re and im and stores one 16-byte
+// lane of each of the NP fragments (every store a contiguous 1 KiB per wave).
+// FUSED: the workgroup first finds its rows' exponents itself (grid: (1,
+// Rp / 16, batches)); otherwise they come from E (grid: (k groups of 4,
+// Rp / 16, batches)).
+template <bool FUSED>
 __global__ __launch_bounds__(NT) void oz_slice_kernel(const SliceView v) {
-  __shared__ double2 T[TILE][KBLK + 1];
+  __shared__ int red[NT];
+  __shared__ int rowexp[RB];
   const int64_t b = blockIdx.z;
-  const int r0 = blockIdx.y * TILE;
-  const int kblk = blockIdx.x;
-  load_tile(v, b, r0, kblk * KBLK, T);
-  __syncthreads();
-  const int t = threadIdx.x, r = t / 4, q = t % 4;
-  const int row = r0 + r;
-  const int sg = row < v.rows ? sigma_of(v.E[b * v.Rp + row]) : 0;
-  unsigned int wre[NSL][4], wim[NSL][4], wneg[NSL][4];
-  for (int s = 0; s < NSL; ++s)
-    for (int w = 0; w < 4; ++w) { wre[s][w] = 0u; wim[s][w] = 0u; wneg[s][w] = 0u; }
-  for (int j = 0; j < 16; ++j) {
-    const double2 x = T[r][16 * q + j];
-    double xr = ldexp(x.x, sg), xi = ldexp(v.conj ? -x.y : x.y, sg);
-    for (int s = 0; s < NSL; ++s) {
-      const double dr = trunc(xr), di = trunc(xi);
-      xr = (xr - dr) * 128.0;
-      xi = (xi - di) * 128.0;
-      const int ir = (int)dr, ii = (int)di;
-      const int sh = 8 * (j & 3);
-      wre[s][j >> 2] |= (unsigned int)(ir & 0xff) << sh;
-      wim[s][j >> 2] |= (unsigned int)(ii & 0xff) << sh;
-      wneg[s][j >> 2] |= (unsigned int)((-ii) & 0xff) << sh;
+  const int rblk = blockIdx.y, r0 = rblk * RB, t = threadIdx.x;
+  const double2* base = batch_base(v, b);
+  const int kt = v.K * v.KB;
+  if (FUSED) {
+    // thread t: row t / 16 (k contiguous) or t % 16 (rows contiguous), every 16th k
+    const bool kf = k_fast(v);
+    const int rr = kf ? t / RB : t % RB, kc = kf ? t % RB : t / RB;
+    int e = INT_MIN;
+    for (int kk = kc; kk < kt; kk += RB) e = max(e, max_exponent(element(v, base, r0 + rr, kk)));
+    red[rr * RB + kc] = e;
+    __syncthreads();
+    if (t < RB) {
+      e = INT_MIN;
+      for (int j = 0; j < RB; ++j) e = max(e, red[t * RB + j]);
+      if (r0 + t < v.rows) v.E[b * v.Rp + r0 + t] = e;
+      rowexp[t] = e;
     }
+    __syncthreads();
   }
-  int8_t* dst = v.out + (((int64_t)b * v.nkb + kblk) * v.Rp + row) * ((int64_t)v.nplanes * KBLK) + 16 * q;
-  const int per = v.nplanes / NSL;   // 3 (re, im, -im) or 2 (re, im)
-  for (int s = 0; s < NSL; ++s) {
-    *reinterpret_cast<uint4*>(dst + (per * s + 0) * KBLK) = make_uint4(wre[s][0], wre[s][1], wre[s][2], wre[s][3]);
-    *reinterpret_cast<uint4*>(dst + (per * s + 1) * KBLK) = make_uint4(wim[s][0], wim[s][1], wim[s][2], wim[s][3]);
-    if (per == 3)
-      *reinterpret_cast<uint4*>(dst + (per * s + 2) * KBLK) =
-          make_uint4(wneg[s][0], wneg[s][1], wneg[s][2], wneg[s][3]);
+  const int lane = t & 63, w = t >> 6, lr = lane & 15, lk = lane >> 4;
+  const int row = r0 + lr;
+  const bool live = row < v.rows;
+  const int sg = live ? sigma_of(FUSED ? rowexp[lr] : v.E[b * v.Rp + row]) : 0;
+  const int rblocks = v.Rp / RB;
+  const double2* rowp = base + (int64_t)(live ? row : 0) * v.row_s;
+  const int kb_first = FUSED ? w : (int)blockIdx.x * 4 + w;
+  const int kb_end = FUSED ? v.nkb : min(v.nkb, kb_first + 1);
+  for (int kb = kb_first; kb < kb_end; kb += 4) {
+    // flattened k of element j: kk0 + j = q K + k
+    const int kk0 = kb * KBLK + 16 * lk;
+    int q = kk0 / v.K, k = kk0 - q * v.K;
+    double2 x[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      x[j] = (live && kk0 + j < kt) ? rowp[(int64_t)k * v.k_s + (int64_t)q * v.kb_s] : make_double2(0.0, 0.0);
+      if (++k == v.K) { k = 0; ++q; }
+    }
+    unsigned int wre[NSL][4], wim[NSL][4];
+#pragma unroll
+    for (int s = 0; s < NSL; ++s)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) { wre[s][c] = 0u; wim[s][c] = 0u; }
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      double xr = ldexp(x[j].x, sg), xi = ldexp(v.conj ? -x[j].y : x[j].y, sg);
+      const int sh = 8 * (j & 3);
+#pragma unroll
+      for (int s = 0; s < NSL; ++s) {
+        const double dr = trunc(xr), di = trunc(xi);
+        xr = (xr - dr) * 128.0;
+        xi = (xi - di) * 128.0;
+        wre[s][j >> 2] |= (unsigned int)((int)dr & 0xff) << sh;
+        wim[s][j >> 2] |= (unsigned int)((int)di & 0xff) << sh;
+      }
+    }
+    int8_t* dst = v.out + (((int64_t)b * v.nkb + kb) * rblocks + rblk) * ((int64_t)NP * FRAG) + 16 * lane;
+#pragma unroll
+    for (int s = 0; s < NSL; ++s) {
+      *reinterpret_cast<uint4*>(dst + (2 * s) * FRAG) = make_uint4(wre[s][0], wre[s][1], wre[s][2], wre[s][3]);
+      *reinterpret_cast<uint4*>(dst + (2 * s + 1) * FRAG) = make_uint4(wim[s][0], wim[s][1], wim[s][2], wim[s][3]);
+    }
   }
 }
 
@@ -192,41 +232,106 @@ __device__ inline void oz_store(const OzGemm& g, int64_t b0, int64_t b1, int row
   *C = out;
 }
 
-// 64 x 64 output tile per workgroup, four waves of 32 x 32 (2 x 2 blocks of
-// the 16 x 16 MFMA output).  Every k block stages all A and B planes in LDS;
-// the next k block is loaded from global memory into registers while the
-// current one is multiplied (one wave per SIMD, so the MFMAs must cover the
-// load latency).
+// Bytewise two's-complement negation of 16 packed int8 digits (|digit| <= 127).
+__device__ inline v4i neg_bytes(v4i x) {
+  v4i r;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const unsigned int n = ~(unsigned int)x[c];
+    r[c] = (int)(((n & 0x7f7f7f7fu) + 0x01010101u) ^ (n & 0x80808080u));
+  }
+  return r;
+}
+
+__device__ inline v4i lds_frag(const int8_t* p) { return *reinterpret_cast<const v4i*>(p); }
+
+// One 64 x 64 output tile per workgroup, four waves of 32 x 32 (2 x 2
+// blocks of the 16 x 16 MFMA output).  Both operands' fragments of a k block
+// (80 KiB) are copied HBM -> LDS by the load-to-LDS path into one of two
+// stages while the other stage is multiplied, one barrier per k block.
+// Within a k block every wave walks its A slices from the last to the first,
+// so the B fragments it needs grow by one slice per step and are read from
+// LDS one step ahead of their first use.
+//
+// Tile order: workgroups are dispatched round-robin over the 8 XCDs; the
+// linear id is remapped so every XCD works on a contiguous range of tiles,
+// and within a batch the tiles run in groups of GM row tiles, column after
+// column, so the ~32 tiles an XCD runs at once cover GM row panels of A (kept
+// in its L2 while the group sweeps the columns) and ~32 / GM column panels of B.
+constexpr int GM = 4;
+
 template <class TO>
 __global__ __launch_bounds__(NT) void oz_gemm_kernel(const OzGemm g) {
-  __shared__ __attribute__((aligned(16))) int8_t sA[TILE * (ROWA + LDS_PAD)];
-  __shared__ __attribute__((aligned(16))) int8_t sB[TILE * (ROWB + LDS_PAD)];
+  __shared__ __attribute__((aligned(1024))) int8_t lds[2 * STAGE];
   const int tiles_m = (g.M + TILE - 1) / TILE, tiles_n = (g.N + TILE - 1) / TILE;
-  int64_t lg = oz_xcd_logical(blockIdx.x, gridDim.x);
+  int64_t L = oz_xcd_logical(blockIdx.x, gridDim.x);
   int mt, nt;
   int64_t b0, b1;
   if (g.batch_major) {
-    const int64_t nb = (int64_t)g.nb0 * g.nb1, bb = lg % nb;
-    lg /= nb;
-    mt = (int)(lg % tiles_m);
-    nt = (int)(lg / tiles_m);
+    const int64_t nb = (int64_t)g.nb0 * g.nb1, bb = L % nb;
+    L /= nb;
+    mt = (int)(L % tiles_m);
+    nt = (int)(L / tiles_m);
     b1 = bb % g.nb1;
     b0 = bb / g.nb1;
   } else {
-    mt = (int)(lg % tiles_m);
-    lg /= tiles_m;
-    nt = (int)(lg % tiles_n);
-    lg /= tiles_n;
-    b1 = lg % g.nb1;
-    b0 = lg / g.nb1;
+    const int per_batch = tiles_m * tiles_n;
+    const int64_t bb = L / per_batch;
+    const int r = (int)(L - bb * per_batch);
+    const int mg = r / (GM * tiles_n), rem = r - mg * (GM * tiles_n);
+    const int gm = min(GM, tiles_m - mg * GM);   // row tiles in this group
+    nt = rem / gm;
+    mt = mg * GM + (rem - nt * gm);
+    b1 = bb % g.nb1;
+    b0 = bb / g.nb1;
   }
   const int m0 = mt * TILE, n0 = nt * TILE;
   const int64_t bA = g.a_shared ? 0 : b0 * g.nb1 + b1, bB = g.b_shared ? 0 : b0 * g.nb1 + b1;
-  const int8_t* gA = g.Aq + ((bA * g.nkb) * g.Mp + m0) * (int64_t)ROWA;
-  const int8_t* gB = g.Bq + ((bB * g.nkb) * g.Np + n0) * (int64_t)ROWB;
+  // the tile's 40 A fragments (4 row blocks x NP planes) of one k block are
+  // contiguous, and so are its 40 B fragments; consecutive k blocks are
+  // kstrideA / kstrideB bytes apart
+  const int64_t kstrideA = (int64_t)(g.Mp / RB) * NP * FRAG, kstrideB = (int64_t)(g.Np / RB) * NP * FRAG;
+  const int8_t* gA = g.Aq + bA * g.nkb * kstrideA + (int64_t)(m0 / RB) * NP * FRAG;
+  const int8_t* gB = g.Bq + bB * g.nkb * kstrideB + (int64_t)(n0 / RB) * NP * FRAG;
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
   const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
   const int lr = lane & 15, lk = lane >> 4;
+
+  // wave w copies fragments w, w + 4, .. of the stage: 10 of A, then 10 of B
+  auto copy_stage = [&](int kb, int st) {
+    const int8_t* srcA = gA + (int64_t)kb * kstrideA + 16 * lane;
+    const int8_t* srcB = gB + (int64_t)kb * kstrideB + 16 * lane;
+    int8_t* dst = lds + st * STAGE;
+#pragma unroll
+    for (int it = 0; it < STAGE_A / FRAG / 4; ++it) {
+      const int f = w + 4 * it;
+      __builtin_amdgcn_global_load_lds((const void*)(srcA + f * FRAG),
+                                       (__attribute__((address_space(3))) void*)(dst + f * FRAG), 16, 0, 0);
+    }
+#pragma unroll
+    for (int it = 0; it < STAGE_A / FRAG / 4; ++it) {
+      const int f = w + 4 * it;
+      __builtin_amdgcn_global_load_lds((const void*)(srcB + f * FRAG),
+                                       (__attribute__((address_space(3))) void*)(dst + STAGE_A + f * FRAG), 16, 0,
+                                       0);
+    }
+  };
+
+  // exponents of the rows / columns this lane stores (fetched with the first
+  // k block)
+  int ea[2][4], eb[2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = m0 + wm + 16 * i + 4 * lk + r;
+      ea[i][r] = row < g.M ? g.EA[bA * g.Mp + row] : INT_MIN;
+    }
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int col = n0 + wn + 16 * j + lr;
+    eb[j] = col < g.N ? g.EB[bB * g.Np + col] : INT_MIN;
+  }
 
   v4i re[2][2][DMAX + 1], im[2][2][DMAX + 1];
 #pragma unroll
@@ -236,81 +341,68 @@ __global__ __launch_bounds__(NT) void oz_gemm_kernel(const OzGemm g) {
 #pragma unroll
       for (int d = 0; d <= DMAX; ++d) { re[i][j][d] = v4i{0, 0, 0, 0}; im[i][j][d] = v4i{0, 0, 0, 0}; }
 
-  // The next k block's A and B tiles (15 and 10 16-byte chunks per thread)
-  // are held in named registers between the loads and the LDS stores (an
-  // array of them is not kept in registers by the compiler).
-  static_assert(TILE * ROWA / 16 / NT == 15 && TILE * ROWB / 16 / NT == 10, "chunk counts");
-#define OZ_A15(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14)
-#define OZ_B10(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9)
-#define OZ_DECL_A(c) uint4 ra##c;
-#define OZ_DECL_B(c) uint4 rb##c;
-#define OZ_LOAD_A(c) ra##c = srcA[(c) * NT + t];
-#define OZ_LOAD_B(c) rb##c = srcB[(c) * NT + t];
-#define OZ_STORE_A(c) *reinterpret_cast<uint4*>(sA + (((c) * NT + t) / (ROWA / 16)) * (ROWA + LDS_PAD) + \
-                                                 16 * (((c) * NT + t) % (ROWA / 16))) = ra##c;
-#define OZ_STORE_B(c) *reinterpret_cast<uint4*>(sB + (((c) * NT + t) / (ROWB / 16)) * (ROWB + LDS_PAD) + \
-                                                 16 * (((c) * NT + t) % (ROWB / 16))) = rb##c;
-  OZ_A15(OZ_DECL_A)
-  OZ_B10(OZ_DECL_B)
-  {
-    const uint4* srcA = reinterpret_cast<const uint4*>(gA);
-    const uint4* srcB = reinterpret_cast<const uint4*>(gB);
-    OZ_A15(OZ_LOAD_A)
-    OZ_B10(OZ_LOAD_B)
-  }
-  const int8_t* aRow[2];
-  aRow[0] = sA + (wm + lr) * (ROWA + LDS_PAD) + 16 * lk;
-  aRow[1] = aRow[0] + 16 * (ROWA + LDS_PAD);
+  copy_stage(0, 0);
   for (int kb = 0; kb < g.nkb; ++kb) {
-    OZ_A15(OZ_STORE_A)
-    OZ_B10(OZ_STORE_B)
+    const int st = kb & 1;
+    // stage st has landed for every wave, and every wave is done with the
+    // stage the next copy overwrites
     __syncthreads();
-    if (kb + 1 < g.nkb) {
-      const uint4* srcA = reinterpret_cast<const uint4*>(gA + (int64_t)(kb + 1) * g.Mp * ROWA);
-      const uint4* srcB = reinterpret_cast<const uint4*>(gB + (int64_t)(kb + 1) * g.Np * ROWB);
-      OZ_A15(OZ_LOAD_A)
-      OZ_B10(OZ_LOAD_B)
-    }
-    v4i bf[2][PB];
+    if (kb + 1 < g.nkb) copy_stage(kb + 1, st ^ 1);
+    const int8_t* sA = lds + st * STAGE + (wm / RB) * NP * FRAG + 16 * lane;
+    const int8_t* sB = lds + st * STAGE + STAGE_A + (wn / RB) * NP * FRAG + 16 * lane;
+    // B fragments of slice u: bf[j][2u] (re), bf[j][2u + 1] (im), block column j
+    v4i bf[2][NP];
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int u = 0; u < NSL && u <= DMAX - (NSL - 1); ++u)
 #pragma unroll
-      for (int p = 0; p < PB; ++p)
-        bf[j][p] = *reinterpret_cast<const v4i*>(sB + (wn + 16 * j + lr) * (ROWB + LDS_PAD) + p * KBLK + 16 * lk);
-    // A fragments (re, im, -im) of slice s for block row i, fetched one
-    // (i, s) step ahead of their MFMAs; within a step the products of one
-    // accumulator are four MFMAs apart.
-    v4i cur[3], nxt[3];
-#pragma unroll
-    for (int q = 0; q < 3; ++q) cur[q] = *reinterpret_cast<const v4i*>(aRow[0] + q * KBLK);
+      for (int j = 0; j < 2; ++j) {
+        bf[j][2 * u] = lds_frag(sB + (j * NP + 2 * u) * FRAG);
+        bf[j][2 * u + 1] = lds_frag(sB + (j * NP + 2 * u + 1) * FRAG);
+      }
+    v4i ar = lds_frag(sA + (2 * (NSL - 1)) * FRAG), ai = lds_frag(sA + (2 * (NSL - 1) + 1) * FRAG);
 #pragma unroll
     for (int step = 0; step < 2 * NSL; ++step) {
-      const int i = step / NSL, s = step % NSL;
+      const int i = step / NSL, s = NSL - 1 - step % NSL;
+      v4i ar2 = ar, ai2 = ai;
       if (step + 1 < 2 * NSL) {
-        const int i2 = (step + 1) / NSL, s2 = (step + 1) % NSL;
+        const int i2 = (step + 1) / NSL, s2 = NSL - 1 - (step + 1) % NSL;
+        ar2 = lds_frag(sA + (i2 * NP + 2 * s2) * FRAG);
+        ai2 = lds_frag(sA + (i2 * NP + 2 * s2 + 1) * FRAG);
+      }
+      // the slice of B the next step uses first
+      if (i == 0 && s >= 1 && DMAX - s + 1 < NSL) {
+        const int u = DMAX - s + 1;
 #pragma unroll
-        for (int q = 0; q < 3; ++q) nxt[q] = *reinterpret_cast<const v4i*>(aRow[i2] + (3 * s2 + q) * KBLK);
+        for (int j = 0; j < 2; ++j) {
+          bf[j][2 * u] = lds_frag(sB + (j * NP + 2 * u) * FRAG);
+          bf[j][2 * u + 1] = lds_frag(sB + (j * NP + 2 * u + 1) * FRAG);
+        }
+      }
+      const v4i nai = neg_bytes(ai);
+      // the products with Ar for every slice u of the step, then those with Ai
+#pragma unroll
+      for (int u = 0; u < NSL && u + s <= DMAX; ++u) {
+        const int d = s + u;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          re[i][j][d] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ar, bf[j][2 * u], re[i][j][d], 0, 0, 0);
+          im[i][j][d] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ar, bf[j][2 * u + 1], im[i][j][d], 0, 0, 0);
+        }
       }
 #pragma unroll
       for (int u = 0; u < NSL && u + s <= DMAX; ++u) {
         const int d = s + u;
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-          re[i][j][d] = __builtin_amdgcn_mfma_i32_16x16x64_i8(cur[0], bf[j][2 * u], re[i][j][d], 0, 0, 0);
-          im[i][j][d] = __builtin_amdgcn_mfma_i32_16x16x64_i8(cur[0], bf[j][2 * u + 1], im[i][j][d], 0, 0, 0);
-        }
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          re[i][j][d] = __builtin_amdgcn_mfma_i32_16x16x64_i8(cur[2], bf[j][2 * u + 1], re[i][j][d], 0, 0, 0);
-          im[i][j][d] = __builtin_amdgcn_mfma_i32_16x16x64_i8(cur[1], bf[j][2 * u], im[i][j][d], 0, 0, 0);
+          re[i][j][d] = __builtin_amdgcn_mfma_i32_16x16x64_i8(nai, bf[j][2 * u + 1], re[i][j][d], 0, 0, 0);
+          im[i][j][d] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ai, bf[j][2 * u], im[i][j][d], 0, 0, 0);
         }
       }
-      if (step + 1 < 2 * NSL) {
-#pragma unroll
-        for (int q = 0; q < 3; ++q) cur[q] = nxt[q];
-      }
+      ar = ar2;
+      ai = ai2;
+      // keep each step's MFMAs together, behind the reads issued for the next step
+      __builtin_amdgcn_sched_barrier(0);
     }
-    __syncthreads();
   }
   // epilogue: lane (lr, lk), register r holds row 4 lk + r, column lr of each 16 x 16 block
 #pragma unroll
@@ -327,18 +419,9 @@ __global__ __launch_bounds__(NT) void oz_gemm_kernel(const OzGemm g) {
           vr = vr * 0.0078125 + (double)re[i][j][d][r];
           vi = vi * 0.0078125 + (double)im[i][j][d][r];
         }
-        const int ea = g.EA[bA * g.Mp + row], eb = g.EB[bB * g.Np + col];
-        const int sc = -(sigma_of(ea) + sigma_of(eb));
+        const int sc = -(sigma_of(ea[i][r]) + sigma_of(eb[j]));
         oz_store<TO>(g, b0, b1, row, col, ldexp(vr, sc), ldexp(vi, sc));
       }
-#undef OZ_A15
-#undef OZ_B10
-#undef OZ_DECL_A
-#undef OZ_DECL_B
-#undef OZ_LOAD_A
-#undef OZ_LOAD_B
-#undef OZ_STORE_A
-#undef OZ_STORE_B
 }
 
 __global__ void oz_fill_kernel(int* p, int64_t n, int v) {
@@ -363,8 +446,8 @@ Plan plan_for(int M, int N, int64_t kt, int nb0, int nb1, bool a_shared, bool b_
   p.nbB = b_shared ? 1 : (int64_t)nb0 * nb1;
   p.bytesEA = round_up((size_t)p.nbA * p.Mp * sizeof(int), 256);
   p.bytesEB = round_up((size_t)p.nbB * p.Np * sizeof(int), 256);
-  p.bytesA = round_up((size_t)p.nbA * p.nkb * p.Mp * ROWA, 256);
-  p.bytesB = round_up((size_t)p.nbB * p.nkb * p.Np * ROWB, 256);
+  p.bytesA = round_up((size_t)p.nbA * p.nkb * p.Mp * NP * KBLK, 256);
+  p.bytesB = round_up((size_t)p.nbB * p.nkb * p.Np * NP * KBLK, 256);
   return p;
 }
 
@@ -375,12 +458,20 @@ void fill_int(int* p, int64_t n, int v, hipStream_t s) {
 }
 
 void slice_operand(SliceView v, int64_t nb, hipStream_t s) {
-  require(nb >= 1 && nb <= 65535 && v.Rp / TILE <= 65535 && v.nkb >= 1, "int8 emulation: operand too large");
+  require(nb >= 1 && nb <= 65535 && v.Rp / RB <= 65535 && v.nkb >= 1, "int8 emulation: operand too large");
+  const int64_t kt = (int64_t)v.K * v.KB;
+  if (kt <= FUSE_KT) {
+    hipLaunchKernelGGL(oz_slice_kernel<true>, dim3(1, (unsigned)(v.Rp / RB), (unsigned)nb), dim3(NT), 0, s, v);
+    GGPU_HIP(hipGetLastError());
+    return;
+  }
   fill_int(v.E, nb * v.Rp, INT_MIN, s);
-  const dim3 grid((unsigned)v.nkb, (unsigned)(v.Rp / TILE), (unsigned)nb);
-  hipLaunchKernelGGL(oz_exponent_kernel, grid, dim3(NT), 0, s, v);
+  hipLaunchKernelGGL(oz_exponent_kernel, dim3((unsigned)v.nkb, (unsigned)(v.Rp / TILE), (unsigned)nb), dim3(NT),
+                     0, s, v);
   GGPU_HIP(hipGetLastError());
-  hipLaunchKernelGGL(oz_slice_kernel, grid, dim3(NT), 0, s, v);
+  const int ngroups = (v.nkb + 3) / 4;
+  hipLaunchKernelGGL(oz_slice_kernel<false>, dim3((unsigned)ngroups, (unsigned)(v.Rp / RB), (unsigned)nb),
+                     dim3(NT), 0, s, v);
   GGPU_HIP(hipGetLastError());
 }
 
@@ -401,13 +492,13 @@ void run_part(const GemmDesc& d, int b0s, int nb0, int kb0, int KBc, bool first_
   a.src = static_cast<const double2*>(d.A.ptr) + (int64_t)b0s * d.A.bs0 + (int64_t)kb0 * d.A.kbs;
   a.row_s = d.A.rs; a.k_s = d.A.cs; a.kb_s = d.A.kbs; a.bs0 = d.A.bs0; a.bs1 = d.A.bs1; a.off1 = d.A.off1;
   a.rows = d.M; a.K = d.K; a.KB = KBc; a.nb1 = a_shared ? 1 : d.nb1;
-  a.conj = d.A.conj; a.nplanes = PA; a.Rp = (int)p.Mp; a.nkb = (int)p.nkb; a.E = EA; a.out = Aq;
+  a.conj = d.A.conj; a.Rp = (int)p.Mp; a.nkb = (int)p.nkb; a.E = EA; a.out = Aq;
   slice_operand(a, p.nbA, s);
   SliceView b;
   b.src = static_cast<const double2*>(d.B.ptr) + (int64_t)b0s * d.B.bs0 + (int64_t)kb0 * d.B.kbs;
   b.row_s = d.B.cs; b.k_s = d.B.rs; b.kb_s = d.B.kbs; b.bs0 = d.B.bs0; b.bs1 = d.B.bs1; b.off1 = d.B.off1;
   b.rows = d.N; b.K = d.K; b.KB = KBc; b.nb1 = b_shared ? 1 : d.nb1;
-  b.conj = d.B.conj; b.nplanes = PB; b.Rp = (int)p.Np; b.nkb = (int)p.nkb; b.E = EB; b.out = Bq;
+  b.conj = d.B.conj; b.Rp = (int)p.Np; b.nkb = (int)p.nkb; b.E = EB; b.out = Bq;
   slice_operand(b, p.nbB, s);
 
   OzGemm g;
