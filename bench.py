@@ -35,14 +35,18 @@ sys.path.insert(0, ROOT)
 
 METRIC = "GW iteration wall-time (s) + TFLOP/s, mixed vs FP64 on MI355X at Σ err≤1e-6"
 # MI355X dense matrix peaks, TFLOP/s (int8: TOP/s)
-PEAK = {"f64": 78.6, "f32": 157.3, "bf16x3": 2500.0, "i8x5": 5000.0}
-DTYPE = {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16x3", "i8x5": "i8x5"}
+PEAK = {"f64": 78.6, "f32": 157.3, "bf16x3": 2500.0, "i8x5": 5000.0, "i8x4": 5000.0}
+DTYPE = {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16x3", "i8x5": "i8x5", "i8x4": "i8x4"}
 # Matrix-core operations executed per algorithmic FLOP (8 per complex
 # multiply-add): FP64 forms 3 real products per complex one (Gauss), FP32 4,
 # split-bf16 x3 4 real products of 3 bf16 products each; the int8 emulation
-# 4 real products for each of its 19 slice pairs (152 int8 ops per complex
-# multiply-add).
-MFMA_PER_ALG = {"fp64": 0.75, "fp32": 1.0, "bf16x3": 3.0, "i8x5": 19.0}
+# 4 real products for each of its 19 (five slices) or 13 (four slices) slice
+# pairs.
+MFMA_PER_ALG = {"fp64": 0.75, "fp32": 1.0, "bf16x3": 3.0, "i8x5": 19.0, "i8x4": 13.0}
+INT8_MODES = ("i8x5", "i8x4")
+# call sites the int8 modes emulate when no autotuner picks (the library's
+# built-in choice)
+INT8_DEFAULT_SITES = ("p0_contract", "sig_py")
 SIGMA_BOUND = 1e-6                                          # Ha, max-abs against FP64
 
 WORKLOADS = {
@@ -62,7 +66,7 @@ DEFAULT_WORKLOAD = "gw_nk216_nao104_nq600"
 # share of --budget-s each mode may spend on timed steps (the rest covers input
 # generation, plan set-up, autotuning, warm-up, error checks, HF and the CPU
 # sample)
-MODE_SHARE = {"fp64": 0.25, "i8x5": 0.12, "bf16x3": 0.08, "fp32": 0.07}
+MODE_SHARE = {"fp64": 0.22, "i8x5": 0.09, "i8x4": 0.09, "bf16x3": 0.06, "fp32": 0.06}
 WARMUP_PAIR_BATCHES = 4
 
 
@@ -79,16 +83,30 @@ def gw_flops(nk, nao, NQ, ns, nts, nw_b):
     return {"p0": p0, "sigma": sig, "ir": ir, "dyson": dyson, "total": p0 + sig + ir + dyson}
 
 
-def mode_roofline(mode, sites):
-    """Dominant GEMM site of one precision mode against that mode's matrix
-    peak: algorithmic TFLOP/s, matrix-core TFLOP/s executed, and the
-    fraction of the peak the latter is."""
+def site_engine(mode, site, tune):
+    """(matrix-core ops per algorithmic FLOP, peak TFLOP/s, engine) of one
+    call site in one precision mode: the int8 modes run each site either on
+    the int8 emulation or on the FP64 kernel (autotuner's pick, else the
+    built-in choice)."""
+    if mode in INT8_MODES:
+        chosen = (tune or {}).get("chosen", {}).get(site)
+        emulated = chosen == "int8" if chosen is not None else site in INT8_DEFAULT_SITES
+        if not emulated:
+            return MFMA_PER_ALG["fp64"], PEAK["f64"], "fp64 kernel"
+        return MFMA_PER_ALG[mode], PEAK[DTYPE[mode]], "int8 emulation"
+    return MFMA_PER_ALG[mode], PEAK[DTYPE[mode]], mode
+
+
+def mode_roofline(mode, sites, tune=None):
+    """Dominant GEMM site of one precision mode against the matrix peak of the
+    engine that ran it: algorithmic TFLOP/s, matrix-core TFLOP/s executed, and
+    the fraction of the peak the latter is."""
     name, (ms, flops, n) = max(sites.items(), key=lambda kv: kv[1][0])
     alg = flops / (ms * 1e-3) / 1e12
-    mfma = alg * MFMA_PER_ALG[mode]
-    peak = PEAK[DTYPE[mode]]
-    return {"kernel": f"gemm site {name}", "achieved_algorithmic": round(alg, 2), "achieved_mfma": round(mfma, 2),
-            "peak": peak, "unit": "TFLOP/s", "frac_mfma": round(mfma / peak, 4)}
+    per_alg, peak, engine = site_engine(mode, name, tune)
+    mfma = alg * per_alg
+    return {"kernel": f"gemm site {name}", "engine": engine, "achieved_algorithmic": round(alg, 2),
+            "achieved_mfma": round(mfma, 2), "peak": peak, "unit": "TFLOP/s", "frac_mfma": round(mfma / peak, 4)}
 
 
 def host_info():
@@ -212,7 +230,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS))
-    ap.add_argument("--precision", default="fp64,i8x5,bf16x3,fp32",
+    ap.add_argument("--precision", default="fp64,i8x5,i8x4,bf16x3,fp32",
                     help="comma-separated modes; fp64 always runs first (the error reference)")
     ap.add_argument("--solver", default="cholesky", choices=["cholesky", "lu"])
     ap.add_argument("--integrals", default=None, choices=["host", "device", "generated"],
@@ -236,6 +254,13 @@ def main():
     ap.add_argument("--graph", action="store_true",
                     help="time the steps as captured HIP graph replays (needs --integrals device); per-site "
                          "times then come from one profiled eager step before the timed steps")
+    ap.add_argument("--error-budget", action="store_true",
+                    help="instead of timing: Sigma and P0 errors against FP64 on the first q (its first "
+                         "--pair-batches pair batches, default all) with the int8 emulation at one contraction "
+                         "site at a time, at the built-in sites and at all sites, and the whole reduced modes")
+    ap.add_argument("--budget-cases", default=None,
+                    help="error budget: comma-separated cases of the int8 modes (site names, built-in, all; "
+                         "default every one)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one plain 1 GiB device-to-device copy kernel (scale for HBM counters)")
     a = ap.parse_args()
@@ -272,6 +297,8 @@ def main():
     fl_step = {k: v / sym.inq for k, v in fl_iter.items()} if per_q else fl_iter
     modes = [m.strip() for m in a.precision.split(",") if m.strip()]
     modes = ["fp64"] + [m for m in modes if m != "fp64"]
+    if a.error_budget:
+        return run_error_budget(a, w, pkg, sym, Gd, T_wt, T_tw, modes, seed, sigma, t_start)
     runs = {}
     S64 = P064 = None
     for mode in modes:
@@ -400,15 +427,16 @@ def main():
     tf = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     if os.path.exists(tf):
         traffic = json.load(open(tf)).get(a.workload, {}).get(head, {}).get(dom_name)
-    roofline = {"bound": "mfma", "kernel": f"gemm site {dom_name}", "achieved": round(achieved, 3),
-                "peak": PEAK[dtype], "unit": "TFLOP/s", "frac": round(achieved / PEAK[dtype], 4),
+    per_alg, peak, engine = site_engine(head, dom_name, r["autotune"])
+    roofline = {"bound": "mfma", "kernel": f"gemm site {dom_name}", "engine": engine,
+                "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
                 # matrix-core operations the kernel executes per algorithmic FLOP
-                "frac_executed": round(achieved * MFMA_PER_ALG[head] / PEAK[dtype], 4),
+                "frac_executed": round(achieved * per_alg / peak, 4),
                 "traffic": None if traffic is None else round(traffic["bytes"]),
                 "traffic_source": None if traffic is None else traffic["source"],
                 "launches": dom_n, "avg_launch_ms": round(dom_ms / dom_n, 4)}
-    if head == "i8x5":
-        # FP64-accurate products: the same FLOPs against the FP64 matrix peak
+    if engine == "int8 emulation":
+        # products with FP64 operands: the same FLOPs against the FP64 matrix peak
         roofline["fp64_equivalent_frac"] = round(achieved / PEAK["f64"], 4)
     step_name = f"one q of {sym.inq}" if per_q else "one GW iteration"
     if a.pair_batches:
@@ -436,7 +464,7 @@ def main():
                       "phases_ms": {k: round(x, 1) for k, x in v["phases"].items()},
                       "sites_ms": {k: round(x[0] / v["steps"], 1) for k, x in v["sites"].items()},
                       "autotune": v["autotune"],
-                      "roofline": mode_roofline(m, v["sites"])} for m, v in runs.items()},
+                      "roofline": mode_roofline(m, v["sites"], v["autotune"])} for m, v in runs.items()},
         "sigma_abs_max_q0": scale,
         "error_reference": "FP64 run of the same q (q = 0) on the same inputs" if per_q else
                            "FP64 iteration on the same inputs",
@@ -471,6 +499,75 @@ def main():
                        f"{fl_iter['total'] / rate:.0f}s")}
     out["wall_s"] = round(time.perf_counter() - t_start, 1)
     print(json.dumps(out), flush=True)
+
+
+CONTRACTION_SITES = ("p0_gv", "p0_gvh", "p0_contract", "sig_vg", "sig_py", "sig_contract")
+
+
+def max_abs_diff(x, y):
+    err = 0.0
+    fa, fb = x.reshape(-1), y.reshape(-1)
+    for c0 in range(0, fa.size, 1 << 26):
+        err = max(err, float(np.abs(fa[c0:c0 + (1 << 26)] - fb[c0:c0 + (1 << 26)]).max()))
+    return err
+
+
+def run_error_budget(a, w, pkg, sym, Gd, T_wt, T_tw, modes, seed, sigma, t_start):
+    """Per-stage error budget: Sigma and P0 of q = 0 against FP64 with the
+    int8 emulation at one contraction site at a time (every other site on
+    the FP64 kernel), at the built-in sites and at all six, and for the
+    whole reduced-precision modes (complex64 data at every stage)."""
+    nao, NQ, ns = w["nao"], w["NQ"], w["ns"]
+
+    def one(kern):
+        kern.set_q_share(0, sym.inq)
+        kern.set_pair_batch_limit(a.pair_batches)
+        Sd = pkg.DeviceArray.empty_like(Gd)
+        t = time.perf_counter()
+        kern.solve_device(Gd.ptr, Sd.ptr)
+        pkg.synchronize()
+        dt = time.perf_counter() - t
+        return Sd.numpy(), kern.last_polarization(0)[1], dt
+
+    ref = pkg.GwKernel(nao, ns, NQ, T_wt, T_tw, sym, precision="fp64", solver=a.solver, integrals=a.integrals)
+    ref.synth_integrals(seed, sigma)
+    S64, P64, dt = one(ref)
+    ref.close()
+    scale, pscale = float(np.abs(S64).max()), float(np.abs(P64).max())
+    log(f"[budget] fp64 reference q 0: {dt:.1f}s, max|Sigma| {scale:.4g}, max|P0| {pscale:.4g}")
+    out = {}
+
+    def record(mode, case, S, P, dt):
+        e, pe = max_abs_diff(S, S64), max_abs_diff(P, P64)
+        out.setdefault(mode, {})[case] = {"sigma_max_abs_err": e, "sigma_rel_err": e / scale,
+                                          "p0_rel_err": pe / pscale, "meets_sigma_bound": e <= SIGMA_BOUND,
+                                          "s": round(dt, 2)}
+        log(f"[budget] {mode} {case}: Sigma {e:.3e} (rel {e / scale:.2e}), P0 rel {pe / pscale:.2e}, {dt:.1f}s")
+
+    for mode in modes[1:]:
+        kern = pkg.GwKernel(nao, ns, NQ, T_wt, T_tw, sym, precision=mode, solver=a.solver, integrals=a.integrals)
+        kern.synth_integrals(seed, sigma)
+        if mode in INT8_MODES:
+            cases = [(s, (s,)) for s in CONTRACTION_SITES] + [("built-in", None), ("all", CONTRACTION_SITES)]
+            if a.budget_cases:
+                keep = [c.strip() for c in a.budget_cases.split(",")]
+                cases = [c for c in cases if c[0] in keep]
+            for case, emulated in cases:
+                v = [-1] * len(pkg.SITES)
+                if emulated is not None:
+                    for s_ in CONTRACTION_SITES:
+                        v[pkg.SITES.index(s_)] = 200 if s_ in emulated else 83
+                kern.set_variants(v)
+                record(mode, case, *one(kern))
+        elif a.pair_batches:
+            # complex64 plans batch twice the pairs: a partial q sums other pairs
+            log(f"[budget] {mode}: needs the whole q (no --pair-batches); skipped")
+        else:
+            record(mode, "all", *one(kern))
+        kern.close()
+    print(json.dumps({"error_budget": out, "workload": a.workload, "q": 0,
+                      "pair_batches": a.pair_batches or "all", "sigma_abs_max": scale, "p0_abs_max": pscale,
+                      "sigma_bound": SIGMA_BOUND, "wall_s": round(time.perf_counter() - t_start, 1)}))
 
 
 def run_batch(a, w, pkg, synth):

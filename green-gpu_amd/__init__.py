@@ -22,7 +22,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgreen_gpu_amd.so")
 
-PRECISION = {"fp64": 0, "fp32": 1, "bf16x3": 2, "i8x5": 3}
+PRECISION = {"fp64": 0, "fp32": 1, "bf16x3": 2, "i8x5": 3, "i8x4": 4}
 SOLVER = {"cholesky": 0, "lu": 1}
 RESIDENCE = {"device": 0, "host": 1, "generated": 2}
 

@@ -537,12 +537,15 @@ GwPlan::GwPlan(int nao_, int nso_, int ns_, int NQ_, int nts_, int nw_b_, const 
   require(nw_b > 0, "gw: nw_b must be positive");
   require(solver == GGPU_SOLVER_CHOLESKY || solver == GGPU_SOLVER_LU, "gw: unknown linear solver");
   require(precision == GGPU_PRECISION_FP64 || precision == GGPU_PRECISION_FP32 ||
-              precision == GGPU_PRECISION_BF16X3 || precision == GGPU_PRECISION_INT8X5,
+              precision == GGPU_PRECISION_BF16X3 || precision == GGPU_PRECISION_INT8X5 ||
+              precision == GGPU_PRECISION_INT8X4,
           "gw: unknown precision");
   require(T_wt && T_tw_h, "gw: NULL transform matrices");
-  compute = precision == GGPU_PRECISION_FP64   ? Compute::F64
-            : precision == GGPU_PRECISION_FP32 ? Compute::F32
-            : precision == GGPU_PRECISION_INT8X5 ? Compute::I8X5 : Compute::BF16X3;
+  compute = precision == GGPU_PRECISION_FP64     ? Compute::F64
+            : precision == GGPU_PRECISION_FP32   ? Compute::F32
+            : precision == GGPU_PRECISION_INT8X5 ? Compute::I8X5
+            : precision == GGPU_PRECISION_INT8X4 ? Compute::I8X4
+                                                 : Compute::BF16X3;
   sym.load(s, nso, NQ);
   if (x2c)
     for (auto t : sym.k_tr)
@@ -641,7 +644,7 @@ GwPlan::GwPlan(int nao_, int nso_, int ns_, int NQ_, int nts_, int nw_b_, const 
   kb = (int)std::max<int64_t>(1, std::min<int64_t>(budget / per_pair, std::max(sym.nk, sym.ink)));
   kb = std::min(kb, 4096);
   work.resize(((size_t)kb * per_pair + sizeof(z64) - 1) / sizeof(z64));
-  if (compute == Compute::I8X5) {
+  if (int8_slices(compute)) {
     // int8 slices of the operands of one contraction launch (launches whose
     // slices do not fit are split over their batches)
     size_t fr = 0, tot = 0;
@@ -1097,9 +1100,13 @@ void GwPlan::autotune(const z64* G_dev, int rounds, int batches) {
   } limit(batch_limit, batches);
   stage_G(G_dev);
   int trial[kNumSites];
+  // the int8 emulation is a candidate only where int8_site_tunable allows it
+  auto tried = [&](int c, int s) {
+    return tunable_site(s) && (cand[c] != kVariantInt8 || int8_site_tunable(compute, s));
+  };
   for (int r = 0; r < rounds; ++r)
     for (int c = 0; c < nc; ++c) {
-      for (int s = 0; s < kNumSites; ++s) trial[s] = tunable_site(s) ? cand[c] : -1;
+      for (int s = 0; s < kNumSites; ++s) trial[s] = tried(c, s) ? cand[c] : -1;
       VariantGuard vg(trial, &ozwork);
       tp.reset();
       p0_for_q(iq);
@@ -1107,7 +1114,7 @@ void GwPlan::autotune(const z64* G_dev, int rounds, int batches) {
       GGPU_HIP(hipStreamSynchronize(stream));
       tp.collect();
       for (int s = 0; s < kNumSites; ++s) {
-        if (!tunable_site(s) || tp.launches[s] == 0) continue;
+        if (!tried(c, s) || tp.launches[s] == 0) continue;
         double& best = tune_ms[c][s];
         best = best < 0 ? tp.ms[s] : std::min(best, tp.ms[s]);
       }
@@ -1513,13 +1520,14 @@ HfPlan::HfPlan(int nao_, int nso_, int ns_, int NQ_, const ggpu_symmetry& s)
 }
 
 void HfPlan::set_precision(int precision) {
-  require(precision >= GGPU_PRECISION_FP64 && precision <= GGPU_PRECISION_INT8X5, "hf: unknown precision");
+  require(precision >= GGPU_PRECISION_FP64 && precision <= GGPU_PRECISION_INT8X4, "hf: unknown precision");
   compute = precision == GGPU_PRECISION_FP64     ? Compute::F64
             : precision == GGPU_PRECISION_FP32   ? Compute::F32
             : precision == GGPU_PRECISION_BF16X3 ? Compute::BF16X3
-                                                 : Compute::I8X5;
+            : precision == GGPU_PRECISION_INT8X5 ? Compute::I8X5
+                                                 : Compute::I8X4;
   dm32.resize(complex64_operands(compute) ? dm_fbz.size() : 0);
-  if (compute == Compute::I8X5 && ozwork.size() == 0) {
+  if (int8_slices(compute) && ozwork.size() == 0) {
     size_t fr = 0, tot = 0;
     GGPU_HIP(hipMemGetInfo(&fr, &tot));
     ozwork.resize(std::min<size_t>((size_t)8 << 30, fr / 8));

@@ -7,10 +7,10 @@
 // Profilers attribute the same kernels (one kernel symbol per call site), so
 // their per-kernel durations can be compared with these.
 //
-//   gemm_site_bench <fp64|fp32|bf16x3|i8x5> [--nao N] [--nq N] [--nts N] [--kb N]
+//   gemm_site_bench <fp64|fp32|bf16x3|i8x5|i8x4> [--nao N] [--nq N] [--nts N] [--kb N]
 //                   [--p0-split S] [--reps R] [--sites a,b,..] [--calibrate-copy]
 //
-// i8x5 forces the int8 emulation at every site (the FP64 kernel is fp64).
+// i8x5 / i8x4 force the int8 emulation at every site (the FP64 kernel is fp64).
 // --calibrate-copy ends with one plain 1 GiB device-to-device copy, the
 // reference point for scaling HBM byte counters.
 #include <hip/hip_runtime.h>
@@ -75,18 +75,19 @@ int64_t extent(int64_t r, int64_t nr, int64_t c, int64_t nc, int64_t kb, int64_t
 int main(int argc, char** argv) {
   if (argc < 2) {
     std::fprintf(stderr,
-                 "usage: %s <fp64|fp32|bf16x3|i8x5> [--nao N] [--nq N] [--nts N] [--kb N] [--p0-split S] "
+                 "usage: %s <fp64|fp32|bf16x3|i8x5|i8x4> [--nao N] [--nq N] [--nts N] [--kb N] [--p0-split S] "
                  "[--reps R] [--sites a,b] [--calibrate-copy]\n",
                  argv[0]);
     return 2;
   }
   const std::string mode = argv[1];
-  if (mode != "fp64" && mode != "fp32" && mode != "bf16x3" && mode != "i8x5") {
+  if (mode != "fp64" && mode != "fp32" && mode != "bf16x3" && mode != "i8x5" && mode != "i8x4") {
     std::fprintf(stderr, "unknown mode %s\n", mode.c_str());
     return 2;
   }
   const Compute comp = mode == "fp64" ? Compute::F64 : mode == "fp32" ? Compute::F32
-                       : mode == "i8x5" ? Compute::I8X5 : Compute::BF16X3;
+                       : mode == "i8x5" ? Compute::I8X5
+                       : mode == "i8x4" ? Compute::I8X4 : Compute::BF16X3;
   int reps = 10, nao = 104, NQ = 600, nts = 110, kb = 1, S = 8;
   std::string only;
   bool calibrate = false;
@@ -142,8 +143,8 @@ int main(int argc, char** argv) {
   void* work = nullptr;
   size_t work_bytes = 0;
   int variants[kNumSites];
-  std::fill(variants, variants + kNumSites, comp == Compute::I8X5 ? (int)kVariantInt8 : -1);
-  if (comp == Compute::I8X5) {
+  std::fill(variants, variants + kNumSites, int8_slices(comp) ? (int)kVariantInt8 : -1);
+  if (int8_slices(comp)) {
     size_t fr = 0, tot = 0;
     check(hipMemGetInfo(&fr, &tot), "meminfo");
     work_bytes = std::min<size_t>((size_t)24 << 30, fr / 4);

@@ -1,16 +1,17 @@
-// FP64-accurate complex GEMM on the gfx950 int8 matrix cores
-// (v_mfma_i32_16x16x64_i8): every FP64 operand is split into five signed
-// 7-bit slices with one power-of-two scale per row of A and per column of B
-// (Ozaki-style splitting), the slice products with s + t <= 5 are summed
-// exactly in int32 (one accumulator per slice diagonal s + t), and the
-// diagonals are recombined in FP64:
+// Complex GEMM with FP64 operands on the gfx950 int8 matrix cores
+// (v_mfma_i32_16x16x64_i8): every FP64 operand is split into NSL signed
+// 7-bit slices (NSL = 5 or 4) with one power-of-two scale per row of A and per
+// column of B (Ozaki-style splitting), the slice products with s + t <= NSL
+// are summed exactly in int32 (one accumulator per slice diagonal s + t), and
+// the diagonals are recombined in FP64:
 //
-//   x 2^sigma = sum_s x_s 128^-s + r,  |x_s| <= 127,  |r| < 128^-4 (relative to the row maximum)
+//   x 2^sigma = sum_s x_s 128^-s + r,  |x_s| <= 127,  |r| < 128^-(NSL-1) (in units of 2^-7 of the row maximum)
 //   (A B)[m][n] = 2^-(sigma_m + sigma_n) sum_d 128^-d sum_{s+t=d} (A_s B_t)[m][n]
 //
-// The dropped terms (s + t > 5, and the slice remainders) leave a relative
-// error of about 2^-36 of max|A_m| max|B_n| K per element (a numpy model on
-// Gaussian operands: ~2e-10 of max|C|), far below complex64 arithmetic.
+// The slice remainders (and the dropped pairs s + t > NSL, smaller still)
+// leave an error of about 2^-(7 NSL - 1) of max|A_m| max|B_n| per term of the
+// k sum: 2^-34 with five slices (FP64-accurate products for the contractions
+// here), 2^-27 with four.
 // Complex products use four real products per slice pair; the negated
 // imaginary slice of A is formed in registers (bytewise), so re and im
 // accumulate exactly:
@@ -21,7 +22,7 @@
 // 16 k bytes 16 (l / 16) .. + 15 -- the operand layout of the MFMA, so a
 // fragment moves from HBM to LDS and from LDS to registers as one contiguous
 // 1 KiB wave access (LDS reads conflict-free, no padding).  Planes: re and im
-// of every slice (10).
+// of every slice (2 NSL).
 //
 // A launch runs the slicing of both operands (per-row / per-column maximum
 // binary exponent, then the digits; one fused pass when the summed k extent
@@ -37,19 +38,20 @@
 namespace ggpu {
 namespace {
 
-constexpr int NSL = 5;              // slices per operand
-constexpr int DMAX = 5;             // kept slice pairs: s + t <= DMAX (19 pairs)
-constexpr int NP = 2 * NSL;         // planes per operand: re and im of every slice
+// NSL (a template argument): slices per operand, 5 or 4; the slice pairs
+// s + t <= NSL are kept (19 or 13 pairs), one int32 accumulator per diagonal.
 constexpr int KBLK = 64;            // k per block: one v_mfma_i32_16x16x64_i8
 constexpr int TILE = 64;            // rows of a workgroup tile (both operands)
 constexpr int RB = 16;              // rows of a fragment
 constexpr int FRAG = 1024;          // bytes of a fragment (64 lanes x 16 bytes)
 constexpr int NT = 256;
-constexpr int STAGE_A = (TILE / RB) * NP * FRAG;   // one operand of a tile for one k block (40 KiB)
-constexpr int STAGE = 2 * STAGE_A;                 // A and B (80 KiB); two stages fill the LDS
+// one operand of a tile for one k block: 4 row blocks x 2 NSL planes (re and
+// im of every slice); a stage holds both operands, two stages the LDS
+template <int NSL>
+constexpr int stage_bytes() { return 2 * (TILE / RB) * 2 * NSL * FRAG; }
 // Largest K * KB one launch may sum: every int32 diagonal accumulator holds at
-// most NSL pairs * 2 products * K * 127^2 < 2^31 (no diagonal has more than
-// NSL slice pairs).
+// most 5 pairs * 2 products * K * 127^2 < 2^31 (no diagonal has more than 5
+// slice pairs).
 constexpr int64_t KMAX = 12800;
 // Largest summed k extent sliced in one fused pass (the rows' second read
 // then comes from cache).
@@ -65,7 +67,7 @@ struct SliceView {
   int conj = 0;
   int Rp = 0, nkb = 0;            // padded rows, k blocks
   int* E = nullptr;               // [batch][Rp]: max binary exponent (INT_MIN: all zero)
-  int8_t* out = nullptr;          // [batch][kblock][Rp / 16][NP][64 lanes][16 bytes]
+  int8_t* out = nullptr;          // [batch][kblock][Rp / 16][2 NSL planes][64 lanes][16 bytes]
 };
 
 __device__ inline const double2* batch_base(const SliceView& v, int64_t b) {
@@ -125,8 +127,9 @@ __global__ __launch_bounds__(NT) void oz_exponent_kernel(const SliceView v) {
 // FUSED: the workgroup first finds its rows' exponents itself (grid: (1,
 // Rp / 16, batches)); otherwise they come from E (grid: (k groups of 4,
 // Rp / 16, batches)).
-template <bool FUSED>
+template <bool FUSED, int NSL>
 __global__ __launch_bounds__(NT) void oz_slice_kernel(const SliceView v) {
+  constexpr int NP = 2 * NSL;
   __shared__ int red[NT];
   __shared__ int rowexp[RB];
   const int64_t b = blockIdx.z;
@@ -260,8 +263,9 @@ __device__ inline v4i lds_frag(const int8_t* p) { return *reinterpret_cast<const
 // in its L2 while the group sweeps the columns) and ~32 / GM column panels of B.
 constexpr int GM = 4;
 
-template <class TO>
+template <class TO, int NSL>
 __global__ __launch_bounds__(NT) void oz_gemm_kernel(const OzGemm g) {
+  constexpr int DMAX = NSL, NP = 2 * NSL, STAGE = stage_bytes<NSL>(), STAGE_A = STAGE / 2;
   __shared__ __attribute__((aligned(1024))) int8_t lds[2 * STAGE];
   const int tiles_m = (g.M + TILE - 1) / TILE, tiles_n = (g.N + TILE - 1) / TILE;
   int64_t L = oz_xcd_logical(blockIdx.x, gridDim.x);
@@ -287,8 +291,8 @@ __global__ __launch_bounds__(NT) void oz_gemm_kernel(const OzGemm g) {
   }
   const int m0 = mt * TILE, n0 = nt * TILE;
   const int64_t bA = g.a_shared ? 0 : b0 * g.nb1 + b1, bB = g.b_shared ? 0 : b0 * g.nb1 + b1;
-  // the tile's 40 A fragments (4 row blocks x NP planes) of one k block are
-  // contiguous, and so are its 40 B fragments; consecutive k blocks are
+  // the tile's A fragments (4 row blocks x NP planes) of one k block are
+  // contiguous, and so are its B fragments; consecutive k blocks are
   // kstrideA / kstrideB bytes apart
   const int64_t kstrideA = (int64_t)(g.Mp / RB) * NP * FRAG, kstrideB = (int64_t)(g.Np / RB) * NP * FRAG;
   const int8_t* gA = g.Aq + bA * g.nkb * kstrideA + (int64_t)(m0 / RB) * NP * FRAG;
@@ -297,7 +301,7 @@ __global__ __launch_bounds__(NT) void oz_gemm_kernel(const OzGemm g) {
   const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
   const int lr = lane & 15, lk = lane >> 4;
 
-  // wave w copies fragments w, w + 4, .. of the stage: 10 of A, then 10 of B
+  // wave w copies fragments w, w + 4, .. of the stage: NSL of A, then NSL of B
   auto copy_stage = [&](int kb, int st) {
     const int8_t* srcA = gA + (int64_t)kb * kstrideA + 16 * lane;
     const int8_t* srcB = gB + (int64_t)kb * kstrideB + 16 * lane;
@@ -437,7 +441,8 @@ struct Plan {
   size_t total() const { return bytesEA + bytesEB + bytesA + bytesB + 4 * 256; }
 };
 
-Plan plan_for(int M, int N, int64_t kt, int nb0, int nb1, bool a_shared, bool b_shared) {
+Plan plan_for(int M, int N, int64_t kt, int nb0, int nb1, bool a_shared, bool b_shared, int nsl) {
+  const int np = 2 * nsl;
   Plan p;
   p.Mp = round_up(M, TILE);
   p.Np = round_up(N, TILE);
@@ -446,8 +451,8 @@ Plan plan_for(int M, int N, int64_t kt, int nb0, int nb1, bool a_shared, bool b_
   p.nbB = b_shared ? 1 : (int64_t)nb0 * nb1;
   p.bytesEA = round_up((size_t)p.nbA * p.Mp * sizeof(int), 256);
   p.bytesEB = round_up((size_t)p.nbB * p.Np * sizeof(int), 256);
-  p.bytesA = round_up((size_t)p.nbA * p.nkb * p.Mp * NP * KBLK, 256);
-  p.bytesB = round_up((size_t)p.nbB * p.nkb * p.Np * NP * KBLK, 256);
+  p.bytesA = round_up((size_t)p.nbA * p.nkb * p.Mp * np * KBLK, 256);
+  p.bytesB = round_up((size_t)p.nbB * p.nkb * p.Np * np * KBLK, 256);
   return p;
 }
 
@@ -457,11 +462,12 @@ void fill_int(int* p, int64_t n, int v, hipStream_t s) {
   GGPU_HIP(hipGetLastError());
 }
 
+template <int NSL>
 void slice_operand(SliceView v, int64_t nb, hipStream_t s) {
   require(nb >= 1 && nb <= 65535 && v.Rp / RB <= 65535 && v.nkb >= 1, "int8 emulation: operand too large");
   const int64_t kt = (int64_t)v.K * v.KB;
   if (kt <= FUSE_KT) {
-    hipLaunchKernelGGL(oz_slice_kernel<true>, dim3(1, (unsigned)(v.Rp / RB), (unsigned)nb), dim3(NT), 0, s, v);
+    hipLaunchKernelGGL((oz_slice_kernel<true, NSL>), dim3(1, (unsigned)(v.Rp / RB), (unsigned)nb), dim3(NT), 0, s, v);
     GGPU_HIP(hipGetLastError());
     return;
   }
@@ -470,18 +476,19 @@ void slice_operand(SliceView v, int64_t nb, hipStream_t s) {
                      0, s, v);
   GGPU_HIP(hipGetLastError());
   const int ngroups = (v.nkb + 3) / 4;
-  hipLaunchKernelGGL(oz_slice_kernel<false>, dim3((unsigned)ngroups, (unsigned)(v.Rp / RB), (unsigned)nb),
+  hipLaunchKernelGGL((oz_slice_kernel<false, NSL>), dim3((unsigned)ngroups, (unsigned)(v.Rp / RB), (unsigned)nb),
                      dim3(NT), 0, s, v);
   GGPU_HIP(hipGetLastError());
 }
 
 // One launch over batches [b0s, b0s + nb0) and K-batch [kb0, kb0 + KBc).
+template <int NSL>
 void run_part(const GemmDesc& d, int b0s, int nb0, int kb0, int KBc, bool first_k, char* work, size_t work_bytes,
               hipStream_t s) {
   const bool a_shared = d.A.bs0 == 0 && d.A.bs1 == 0 && d.A.off1 == nullptr;
   const bool b_shared = d.B.bs0 == 0 && d.B.bs1 == 0 && d.B.off1 == nullptr;
   const int64_t kt = (int64_t)d.K * KBc;
-  const Plan p = plan_for(d.M, d.N, kt, nb0, d.nb1, a_shared, b_shared);
+  const Plan p = plan_for(d.M, d.N, kt, nb0, d.nb1, a_shared, b_shared, NSL);
   require(p.total() <= work_bytes, "int8 emulation: workspace too small for one batch");
   int* EA = reinterpret_cast<int*>(work);
   int* EB = reinterpret_cast<int*>(work + p.bytesEA);
@@ -493,13 +500,13 @@ void run_part(const GemmDesc& d, int b0s, int nb0, int kb0, int KBc, bool first_
   a.row_s = d.A.rs; a.k_s = d.A.cs; a.kb_s = d.A.kbs; a.bs0 = d.A.bs0; a.bs1 = d.A.bs1; a.off1 = d.A.off1;
   a.rows = d.M; a.K = d.K; a.KB = KBc; a.nb1 = a_shared ? 1 : d.nb1;
   a.conj = d.A.conj; a.Rp = (int)p.Mp; a.nkb = (int)p.nkb; a.E = EA; a.out = Aq;
-  slice_operand(a, p.nbA, s);
+  slice_operand<NSL>(a, p.nbA, s);
   SliceView b;
   b.src = static_cast<const double2*>(d.B.ptr) + (int64_t)b0s * d.B.bs0 + (int64_t)kb0 * d.B.kbs;
   b.row_s = d.B.cs; b.k_s = d.B.rs; b.kb_s = d.B.kbs; b.bs0 = d.B.bs0; b.bs1 = d.B.bs1; b.off1 = d.B.off1;
   b.rows = d.N; b.K = d.K; b.KB = KBc; b.nb1 = b_shared ? 1 : d.nb1;
   b.conj = d.B.conj; b.Rp = (int)p.Np; b.nkb = (int)p.nkb; b.E = EB; b.out = Bq;
-  slice_operand(b, p.nbB, s);
+  slice_operand<NSL>(b, p.nbB, s);
 
   OzGemm g;
   g.Aq = Aq; g.Bq = Bq; g.EA = EA; g.EB = EB;
@@ -513,21 +520,22 @@ void run_part(const GemmDesc& d, int b0s, int nb0, int kb0, int KBc, bool first_
   g.beta_im = first_k ? d.beta_im : 0.0;
   const int64_t tiles = (int64_t)(p.Mp / TILE) * (p.Np / TILE) * nb0 * d.nb1;
   require(tiles < ((int64_t)1 << 31), "int8 emulation: grid too large");
-  if (d.out_f32) hipLaunchKernelGGL((oz_gemm_kernel<float2>), dim3((unsigned)tiles), dim3(NT), 0, s, g);
-  else hipLaunchKernelGGL((oz_gemm_kernel<double2>), dim3((unsigned)tiles), dim3(NT), 0, s, g);
+  if (d.out_f32) hipLaunchKernelGGL((oz_gemm_kernel<float2, NSL>), dim3((unsigned)tiles), dim3(NT), 0, s, g);
+  else hipLaunchKernelGGL((oz_gemm_kernel<double2, NSL>), dim3((unsigned)tiles), dim3(NT), 0, s, g);
   GGPU_HIP(hipGetLastError());
 }
 
 }  // namespace
 
-size_t gemm_int8_emulated_workspace(const GemmDesc& d) {
+size_t gemm_int8_emulated_workspace(const GemmDesc& d, int slices) {
   const bool a_shared = d.A.bs0 == 0 && d.A.bs1 == 0 && d.A.off1 == nullptr;
   const bool b_shared = d.B.bs0 == 0 && d.B.bs1 == 0 && d.B.off1 == nullptr;
   const int KBc = (int)std::max<int64_t>(1, std::min<int64_t>(d.KB, KMAX / std::max(1, d.K)));
-  return plan_for(d.M, d.N, (int64_t)d.K * KBc, 1, d.nb1, a_shared, b_shared).total();
+  return plan_for(d.M, d.N, (int64_t)d.K * KBc, 1, d.nb1, a_shared, b_shared, slices).total();
 }
 
-void gemm_int8_emulated(const GemmDesc& d, void* work, size_t work_bytes, hipStream_t s) {
+void gemm_int8_emulated(const GemmDesc& d, int slices, void* work, size_t work_bytes, hipStream_t s) {
+  require(slices == 4 || slices == 5, "int8 emulation: 4 or 5 slices");
   require(work != nullptr && work_bytes > 0, "int8 emulation: no workspace installed");
   require(d.K <= KMAX, "int8 emulation: K above " + std::to_string(KMAX));
   const bool a_shared = d.A.bs0 == 0 && d.A.bs1 == 0 && d.A.off1 == nullptr;
@@ -536,12 +544,17 @@ void gemm_int8_emulated(const GemmDesc& d, void* work, size_t work_bytes, hipStr
   const int KBc = (int)std::max<int64_t>(1, std::min<int64_t>(d.KB, KMAX / std::max(1, d.K)));
   // batch ranges (along b0) whose slices fit the workspace
   int nb0c = d.nb0;
-  while (nb0c > 1 && plan_for(d.M, d.N, (int64_t)d.K * KBc, nb0c, d.nb1, a_shared, b_shared).total() > work_bytes)
+  while (nb0c > 1 &&
+         plan_for(d.M, d.N, (int64_t)d.K * KBc, nb0c, d.nb1, a_shared, b_shared, slices).total() > work_bytes)
     nb0c = (nb0c + 1) / 2;
   for (int kb0 = 0; kb0 < d.KB; kb0 += KBc) {
     const int kbn = std::min(KBc, d.KB - kb0);
-    for (int b0s = 0; b0s < d.nb0; b0s += nb0c)
-      run_part(d, b0s, std::min(nb0c, d.nb0 - b0s), kb0, kbn, kb0 == 0, static_cast<char*>(work), work_bytes, s);
+    for (int b0s = 0; b0s < d.nb0; b0s += nb0c) {
+      const int nb = std::min(nb0c, d.nb0 - b0s);
+      char* w = static_cast<char*>(work);
+      if (slices == 5) run_part<5>(d, b0s, nb, kb0, kbn, kb0 == 0, w, work_bytes, s);
+      else run_part<4>(d, b0s, nb, kb0, kbn, kb0 == 0, w, work_bytes, s);
+    }
   }
 }
 

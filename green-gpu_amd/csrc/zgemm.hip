@@ -1040,7 +1040,7 @@ int gemm_variant_candidates(Compute c, int* out, int cap) {
   static const int f32[] = {8, 16, 83};
   static const int bf[] = {kBf16Tile128x64, kBf16Tile64x128, kBf16Tile160x64, kBf16Tile96x64W3,
                            kBf16Tile64x96W3, kBf16Tile64x64W3, kBf16Tile64x64, kBf16TileAstat64x64};
-  if (c == Compute::I8X5) {              // int8 emulation or the FP64 kernel
+  if (int8_slices(c)) {                  // int8 emulation or the FP64 kernel
     static const int i8[] = {kVariantInt8, 83};
     for (int i = 0; i < 2 && i < cap; ++i) out[i] = i8[i];
     return 2;
@@ -1080,10 +1080,11 @@ void gemm(const GemmDesc& dd, hipStream_t stream) {
       if (d.out_f32) launch_bf16x3<float2>(d, stream, v);
       else launch_bf16x3<double2>(d, stream, v);
       break;
-    case Compute::I8X5: {
-      const bool emulate = v == kVariantInt8 || (v == -1 && d.site == kSiteSigY2);
+    case Compute::I8X5:
+    case Compute::I8X4: {
+      const bool emulate = v == kVariantInt8 || (v == -1 && (d.site == kSiteSigY2 || d.site == kSiteP0));
       if (emulate) {
-        gemm_int8_emulated(d, g_workspace, g_workspace_bytes, stream);
+        gemm_int8_emulated(d, int8_slices(d.compute), g_workspace, g_workspace_bytes, stream);
       } else {
         require(!d.out_f32, "gemm: FP64 kernel writes complex128");
         GemmDesc f = d;

@@ -22,12 +22,15 @@ enum class Compute : int {
                // v_mfma_f32_16x16x32_bf16, FP32 accumulation
   I8X5 = 3,    // complex128 operands split into five signed 7-bit slices,
                // exact int32 slice products (s + t <= 5) on
-               // v_mfma_i32_16x16x64_i8, FP64 recombination: ~2^-36 of the
+               // v_mfma_i32_16x16x64_i8, FP64 recombination: ~2^-34 of the
                // operand scale
+  I8X4 = 4,    // as I8X5 with four slices (s + t <= 4): ~2^-27 of the operand scale
 };
 
 // Operands of this compute mode are complex64 (else complex128).
 inline bool complex64_operands(Compute c) { return c == Compute::F32 || c == Compute::BF16X3; }
+// Slices of the int8 emulation of this compute mode (0: not emulated).
+inline int int8_slices(Compute c) { return c == Compute::I8X5 ? 5 : c == Compute::I8X4 ? 4 : 0; }
 
 struct Operand {
   const void* ptr = nullptr;   // element 0 of batch (0,0)
@@ -92,10 +95,10 @@ enum {
   kBf16TileAstat64x64 = 9
 };
 void set_bf16_tiles(int mode);
-// Variant of a Compute::I8X5 launch that runs the int8 emulation; the FP64
-// kernel variants (8, 16, 83, 84) run the launch on the FP64 kernel instead
-// (both FP64-accurate).  Built-in choice: the emulation at the Sigma P.Y1
-// product (the one site where it is faster on the benchmark shapes), the
+// Variant of a Compute::I8X5 / I8X4 launch that runs the int8 emulation; the
+// FP64 kernel variants (8, 16, 83, 84) run the launch on the FP64 kernel
+// instead.  Built-in choice: the emulation at the Sigma P.Y1 product and the
+// P0 contraction (the sites where it is faster on the benchmark shapes), the
 // FP64 kernel everywhere else; per-site variants override it.
 enum { kVariantInt8 = 200 };
 // Variant of the FP64 / FP32 kernel: 0 picks per call site; 8 or 16 forces
@@ -107,13 +110,21 @@ void set_mfma_bk(int variant);
 // an LDS k-depth / waves-per-SIMD value of set_mfma_bk for FP64 and FP32,
 // a kBf16Tile* value for split-bf16.  The global switches above win.
 void set_site_variants(const int* table);
-// Device workspace of the Compute::I8X5 launches of the calling thread
+// Device workspace of the Compute::I8X5 / I8X4 launches of the calling thread
 // (nullptr: none; such launches then fail with an error).
 void set_gemm_workspace(void* ptr, size_t bytes);
 // Variants worth timing for a compute mode (the autotuner's candidates);
 // returns their count, writes at most cap of them.
 int gemm_variant_candidates(Compute c, int* out, int cap);
 bool gemm_variant_valid(Compute c, int variant);
+// Whether the autotuner may pick the int8 emulation at this call site: any
+// site with five slices; with four slices only the P0 contraction and the
+// Sigma P.Y1 product (four-slice emulation at all six contraction sites
+// leaves ~1.1e-6 Ha in Sigma of the nk = 216, nao = 104, NQ = 600 benchmark
+// q, two sites ~3.5e-7).  An explicit per-site variant may still pick it.
+inline bool int8_site_tunable(Compute c, int site) {
+  return c == Compute::I8X5 || (c == Compute::I8X4 && (site == kSiteP0 || site == kSiteSigY2));
+}
 
 struct GemmDesc {
   int site = kSiteOther;

@@ -48,13 +48,18 @@ typedef enum {
   GGPU_PRECISION_FP64 = 0,    /* reference "double" path (gw_gpu_kernel.cpp:278-284, _sp = false) */
   GGPU_PRECISION_FP32 = 1,    /* complex64 contractions (reference _sp = P_sp && Sigma_sp)        */
   GGPU_PRECISION_BF16X3 = 2,  /* complex64 data, split-bf16 x3 products, FP32 accumulation        */
-  GGPU_PRECISION_INT8X5 = 3   /* complex128 data; FP64-accurate products, each call site either
+  GGPU_PRECISION_INT8X5 = 3,  /* complex128 data; FP64-accurate products, each call site either
                                  emulated on the int8 matrix cores (five signed 7-bit slices per
                                  element, one power-of-two scale per row / column, exact int32
                                  slice products up to slice diagonal 5, FP64 recombination; ~1e-10
                                  relative per product) or run on the FP64 kernel, whichever the
                                  autotuner (or ggpu_*_set_variants) picks; built-in: emulation at
-                                 the Sigma P.Y1 product only.  No counterpart in the reference    */
+                                 the Sigma P.Y1 product and the P0 contraction.  No counterpart in
+                                 the reference                                                   */
+  GGPU_PRECISION_INT8X4 = 4   /* as INT8X5 with four slices per element (slice diagonals up to 4,
+                                 13 slice products instead of 19): ~2^-27 of the operand scale
+                                 per product, fewer int8 products and slice bytes.  No
+                                 counterpart in the reference                                    */
 } ggpu_precision;
 
 /* Where a plan keeps the stored integrals.  HOST: in pinned host memory,
@@ -141,9 +146,9 @@ int ggpu_hf_set_integral_residence(ggpu_hf_plan* plan, int residence);
 int ggpu_hf_set_precision(ggpu_hf_plan* plan, int precision);
 /* Launch variant of the exchange products (11 entries indexed by call site
  * as for ggpu_gw_site_stats; only site 9, HF, may be set, -1 elsewhere): in
- * the GGPU_PRECISION_INT8X5 mode 200 runs them on the int8 emulation and 83
- * (the built-in choice) on the FP64 kernel; both are FP64-accurate.  No
- * counterpart in the reference. */
+ * the GGPU_PRECISION_INT8X5 / INT8X4 modes 200 runs them on the int8
+ * emulation and 83 (the built-in choice) on the FP64 kernel.  No counterpart
+ * in the reference. */
 int ggpu_hf_set_variants(ggpu_hf_plan* plan, const int* variants);
 /* This device's share of the IBZ k-points: exchange and solve then compute
  * the rows ik = rank, rank + size, ... < ink only and leave the other rows
