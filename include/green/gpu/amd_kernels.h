@@ -55,14 +55,16 @@ namespace green::gpu {
     }
 
     // Arithmetic of the contractions: the "gpu_precision" parameter (fp64,
-    // fp32 or bf16x3) when set, else `fallback`.
+    // fp32, bf16x3, i8x5 or i8x4) when set, else `fallback`.
     inline int precision_param(const params::params& p, int fallback) {
       const std::string s = p["gpu_precision"].as<std::string>();
       if (s.empty()) return fallback;
       if (s == "fp64") return GGPU_PRECISION_FP64;
       if (s == "fp32") return GGPU_PRECISION_FP32;
       if (s == "bf16x3") return GGPU_PRECISION_BF16X3;
-      throw std::runtime_error("gpu_precision must be fp64, fp32 or bf16x3, not \"" + s + "\"");
+      if (s == "i8x5") return GGPU_PRECISION_INT8X5;
+      if (s == "i8x4") return GGPU_PRECISION_INT8X4;
+      throw std::runtime_error("gpu_precision must be fp64, fp32, bf16x3, i8x5 or i8x4, not \"" + s + "\"");
     }
 
     // Owned copies of every table ggpu_symmetry points to

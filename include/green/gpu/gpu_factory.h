@@ -65,7 +65,8 @@ namespace green::gpu {
     // no counterpart in the reference: arithmetic, launch-variant tuning and
     // graph replay of this implementation
     p.define<std::string>("gpu_precision",
-                          "contraction arithmetic: fp64, fp32 or bf16x3 (split-bf16 x3 on the matrix cores); "
+                          "contraction arithmetic: fp64, fp32, bf16x3 (split-bf16 x3 on the matrix cores), i8x5 or "
+                          "i8x4 (FP64 operands sliced onto the int8 matrix cores, five or four slices); "
                           "empty: fp32 for GW when P_sp and Sigma_sp, else fp64",
                           "");
     p.define<bool>("gpu_autotune", "time every GEMM launch variant on the first GW iteration, keep the fastest", false);
