@@ -6,8 +6,8 @@
  * they translate green-symmetry / green-grids objects into the plain tables
  * of ggpu_symmetry, read the density-fitted integrals from disk once, keep
  * one device plan per kernel object, and rethrow ABI status codes as
- * std::runtime_error (std::domain_error for a non-positive-definite Dyson
- * matrix).
+ * std::runtime_error (std::domain_error for a Dyson matrix that is not
+ * positive definite under Cholesky or singular under LU).
  *
  * Threading and MPI: like the reference, every rank of a node calls solve();
  * the node's rank 0 does the device work on shared objects, the others wait
@@ -50,7 +50,7 @@ namespace green::gpu {
     inline void check(int rc) {
       if (rc == GGPU_OK) return;
       const std::string msg = ggpu_last_error();
-      if (rc == GGPU_ERR_NOT_POSITIVE_DEFINITE) throw std::domain_error(msg);
+      if (rc == GGPU_ERR_NOT_POSITIVE_DEFINITE || rc == GGPU_ERR_SINGULAR) throw std::domain_error(msg);
       throw std::runtime_error(msg);
     }
 

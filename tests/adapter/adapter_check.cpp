@@ -7,9 +7,16 @@
 //   3. registers the kernel parameters;
 //   4. creates the HF kernel through custom_hf_kernel: without a GPU this
 //      must fail with the library's device error, with one it solves and
-//      writes the Fock matrices to <dir>/F.bin.
+//      writes the Fock matrices to <dir>/F.bin;
+//   5. when GW inputs are present (gw_sizes.txt), writes the GW integrals in
+//      the chunked format, builds a transformer_t from the given bosonic
+//      transforms and runs custom_gw_kernel's callback on G (LU solver, as
+//      green-mbpt's default): without a GPU it must fail with the device
+//      error, with one it writes Sigma(tau) to <dir>/Sigma.bin.
 // Input (written by tests/test_adapter.py into <dir>): sizes.txt, tables.txt,
-// V.bin (stored pairs, complex128), dm.bin and S.bin (ns, ink, nso, nso).
+// V.bin (stored pairs, complex128), dm.bin and S.bin (ns, ink, nso, nso);
+// for GW: gw_sizes.txt (nts, nw_b, chunk), T_tw.bin (nts, nw_b), T_wt.bin
+// (nw_b, nts - 2), gw_V.bin (stored pairs), gw_G.bin (nts, ns, ink, nso, nso).
 // An optional second argument names an integral directory written by
 // green-mbpt; its pairs are read and written to <dir>/V_ref.bin.
 #include <green/gpu/gpu_factory.h>
@@ -178,6 +185,61 @@ int main(int argc, char** argv) {
   } catch (const std::runtime_error& e) {
     std::cout << "hf kernel: " << e.what() << std::endl;
     EXPECT(ndev == 0, std::string("kernel failed with a device present: ") + e.what());
+  }
+  // 5. GW kernel through the factory
+  std::ifstream gws(dir + "/gw_sizes.txt");
+  if (gws) {
+    std::map<std::string, double> gz;
+    {
+      std::string k;
+      double      v;
+      while (gws >> k >> v) gz[k] = v;
+    }
+    const size_t nts = gz["nts"], nw_b = gz["nw_b"], gchunk = gz["chunk"];
+    green::grids::transformer_t ft(nts, nw_b);
+    {
+      const auto tw = read_bin(dir + "/T_tw.bin", nts * nw_b), wt = read_bin(dir + "/T_wt.bin", nw_b * (nts - 2));
+      std::memcpy(ft.Ttn_FB().data(), tw.data(), tw.size() * sizeof(cd));
+      std::memcpy(ft.Tnt_BF().data(), wt.data(), wt.size() * sizeof(cd));
+    }
+    const auto        Vg  = read_bin(dir + "/gw_V.bin", nstored * pe);
+    const std::string dfg = dir + "/df_gw";
+    {
+      hid_t f = H5Fcreate((dfg + "/meta.h5").c_str(), H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
+      long  cs = static_cast<long>(gchunk);
+      H5LTmake_dataset_long(f, "chunk_size", 0, nullptr, &cs);
+      H5Fclose(f);
+      for (size_t c = 0; c < nstored; c += gchunk) {
+        const size_t      cnt     = std::min(gchunk, nstored - c);
+        hsize_t           dims[4] = {cnt, NQ, nao, 2 * nao};
+        const std::string name    = dfg + "/VQ_" + std::to_string(c) + ".h5";
+        hid_t             g       = H5Fcreate(name.c_str(), H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
+        H5LTmake_dataset_double(g, ("/" + std::to_string(c)).c_str(), 4, dims,
+                                reinterpret_cast<const double*>(Vg.data() + c * pe));
+        H5Fclose(g);
+      }
+    }
+    p.set("dfintegral_file", dfg);
+    p.set("cuda_linear_solver", "LU");
+    using G_type = green::gpu::G_type;
+    green::gpu::ztensor<5> G0(nts, ns, ink, nso, nso), S0(nts, ns, ink, nso, nso);
+    {
+      const auto g = read_bin(dir + "/gw_G.bin", G0.size());
+      std::memcpy(G0.data(), g.data(), g.size() * sizeof(cd));
+    }
+    G_type g_sh(std::move(G0)), s_sh(std::move(S0));
+    try {
+      auto [kernel, solve] = green::gpu::custom_gw_kernel(nso != nao, p, nao, nso, ns, NQ, ft, bz, S_k);
+      solve(g_sh, s_sh);
+      const auto&   S = s_sh.object();
+      std::ofstream out(dir + "/Sigma.bin", std::ios::binary);
+      out.write(reinterpret_cast<const char*>(S.data()), S.size() * sizeof(cd));
+      std::cout << "gw kernel: solved on the device" << std::endl;
+      EXPECT(ndev > 0, "gw kernel ran without a device");
+    } catch (const std::runtime_error& e) {
+      std::cout << "gw kernel: " << e.what() << std::endl;
+      EXPECT(ndev == 0, std::string("gw kernel failed with a device present: ") + e.what());
+    }
   }
   // optional: a directory written by green-mbpt itself
   if (argc > 2) {

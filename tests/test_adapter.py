@@ -1,10 +1,13 @@
 """The C++ adapter for green-mbpt (include/green/gpu/gpu_factory.h): compiled
 against the green-* stand-ins in tests/adapter/stubs and HDF5, then run on
-the reference's symmetric HF fixture system (27 k, 4 IBZ, conj/trans pair
-storage).  It must read green-mbpt's chunked integral files back bit for bit,
-hand the symmetry tables to the C ABI unchanged, and - without a GPU - stop
-with the library's device error (with one, its Fock matrices are checked
-against the reference's Sigma1)."""
+the reference's symmetric HF and GW fixture system (27 k, 4 IBZ, conj/trans
+pair storage).  It must read green-mbpt's chunked integral files back bit for
+bit, hand the symmetry tables to the C ABI unchanged, and - without a GPU -
+stop with the library's device error in both kernels (with one, its Fock
+matrices are checked against the reference's Sigma1 and the GW kernel's
+Sigma(tau), through custom_gw_kernel with a transformer_t carrying the
+Lambda = 1e4 transforms on the grid recovered from the fixture, against the
+reference's Sigma_tau)."""
 import os
 import shutil
 import subprocess
@@ -45,6 +48,16 @@ def write_inputs(d, fx, oracle, chunk=37):
     np.ascontiguousarray(fx["S_k"][:, sym.k_reduced_to_full]).tofile(os.path.join(d, "S.bin"))
     np.ascontiguousarray(fx["hf_G_tau"][-1]).tofile(os.path.join(d, "dm.bin"))
     os.makedirs(os.path.join(d, "df"), exist_ok=True)
+    # GW: the fixture's G, integrals and the bosonic transforms on its grid
+    g = np.load(os.path.join(ROOT, "tests", "golden", "fixtures", "gw_grid_beta100_lambda1e4.npz"))
+    T_tw, T_wt = np.ascontiguousarray(g["T_tw"], np.complex128), np.ascontiguousarray(g["T_wt"], np.complex128)
+    with open(os.path.join(d, "gw_sizes.txt"), "w") as f:
+        f.write(f"nts {T_tw.shape[0]}\nnw_b {T_tw.shape[1]}\nchunk {int(fx['gw_chunk_size'])}\n")
+    T_tw.tofile(os.path.join(d, "T_tw.bin"))
+    T_wt.tofile(os.path.join(d, "T_wt.bin"))
+    np.ascontiguousarray(fx["gw_VQ_0"]).tofile(os.path.join(d, "gw_V.bin"))
+    np.ascontiguousarray(fx["gw_G_tau"]).tofile(os.path.join(d, "gw_G.bin"))
+    os.makedirs(os.path.join(d, "df_gw"), exist_ok=True)
     return sym
 
 
@@ -96,9 +109,20 @@ def test_adapter_on_hf_fixture(tmp_path, fx, oracle):
 
 @pytest.mark.gpu
 def test_adapter_hf_fixture_on_device(tmp_path, fx, oracle):
-    """With a device, custom_hf_kernel solves the fixture: -F == Sigma1."""
+    """With a device, custom_hf_kernel solves the fixture: -F == Sigma1, and
+    custom_gw_kernel's callback (LU, FP64) reproduces the reference's GW
+    Sigma_tau (its bar 1e-6; the FP64 path is within ~1e-10) and the
+    oracle."""
     d = str(tmp_path)
     out = run_adapter(d, fx, oracle)
-    assert "solved on the device" in out
+    assert "hf kernel: solved on the device" in out and "gw kernel: solved on the device" in out
     F = np.fromfile(os.path.join(d, "F.bin"), dtype=np.complex128).reshape(fx["hf_Sigma1"].shape)
     assert max_abs(-F, fx["hf_Sigma1"]) < 1e-8
+    S = np.fromfile(os.path.join(d, "Sigma.bin"), dtype=np.complex128).reshape(fx["gw_Sigma_tau"].shape)
+    g = np.load(os.path.join(ROOT, "tests", "golden", "fixtures", "gw_grid_beta100_lambda1e4.npz"))
+    NQ = int(fx["params_NQ"])
+    S_or = oracle.gw_solve(fx["gw_G_tau"], fx["gw_VQ_0"], oracle.symmetry_from_fixture(fx, NQ), g["T_wt"],
+                           g["T_tw"], solver="lu")
+    err_ref, err_or = max_abs(S, fx["gw_Sigma_tau"]), max_abs(S, S_or)
+    print(f"adapter GW: vs Sigma_tau {err_ref:.2e}, vs oracle {err_or:.2e}", flush=True)
+    assert err_ref < 1e-6 and err_or < 1e-9
