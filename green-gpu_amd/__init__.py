@@ -24,7 +24,9 @@ LIB_PATH = os.path.join(HERE, "libgreen_gpu_amd.so")
 
 PRECISION = {"fp64": 0, "fp32": 1, "bf16x3": 2, "i8x5": 3, "i8x4": 4}
 SOLVER = {"cholesky": 0, "lu": 1}
-RESIDENCE = {"device": 0, "host": 1, "generated": 2}
+RESIDENCE = {"device": 0, "host": 1, "generated": 2, "reader": 3}
+# ggpu_pair_reader: int (*)(void* user, int64_t pos, double* dst)
+PAIR_READER = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int64, C.POINTER(C.c_double))
 
 
 class GgpuError(RuntimeError):
@@ -54,6 +56,7 @@ def lib():
         L.ggpu_hf_set_variants.argtypes = [vp, C.POINTER(C.c_int)]
         L.ggpu_hf_set_precision.argtypes = [vp, i32]
         L.ggpu_hf_load_integrals.argtypes = [vp, pd]
+        L.ggpu_hf_set_integral_reader.argtypes = [vp, PAIR_READER, vp]
         L.ggpu_hf_synth_integrals.argtypes = [vp, u64, f64]
         L.ggpu_hf_exchange.argtypes = [vp, pd, pd]
         L.ggpu_hf_solve.argtypes = [vp, pd, pd, f64, pd]
@@ -90,6 +93,7 @@ def lib():
         L.ggpu_gw_autotune_device.argtypes = [vp, vp, i32, i32]
         L.ggpu_gw_autotune_report.argtypes = [vp, i32, C.POINTER(C.c_int), C.POINTER(C.c_int), pd, pd]
         L.ggpu_gw_get_variants.argtypes = [vp, C.POINTER(C.c_int)]
+        L.ggpu_gw_set_integral_reader.argtypes = [vp, PAIR_READER, vp]
         L.ggpu_gw_set_variants.argtypes = [vp, C.POINTER(C.c_int)]
         L.ggpu_gw_plan_destroy.argtypes = [vp]
         L.ggpu_gw_plan_destroy.restype = None
@@ -100,13 +104,13 @@ def lib():
 EXPORTED = ["ggpu_last_error", "ggpu_device_count", "ggpu_device_memory",
             "ggpu_hf_plan_create", "ggpu_hf_set_integral_residence", "ggpu_hf_set_k_share", "ggpu_hf_set_variants",
             "ggpu_hf_set_precision",
-            "ggpu_hf_load_integrals",
+            "ggpu_hf_load_integrals", "ggpu_hf_set_integral_reader",
             "ggpu_hf_synth_integrals",
             "ggpu_hf_exchange", "ggpu_hf_exchange_device_noalloc", "ggpu_hf_solve", "ggpu_hf_energy",
             "ggpu_hf_plan_destroy",
             "ggpu_gw_plan_create", "ggpu_gw_set_integral_residence", "ggpu_gw_streamed_bytes",
             "ggpu_gw_generated_bytes", "ggpu_gw_staging_ms", "ggpu_gw_set_q_share",
-            "ggpu_gw_load_integrals", "ggpu_gw_synth_integrals",
+            "ggpu_gw_load_integrals", "ggpu_gw_synth_integrals", "ggpu_gw_set_integral_reader",
             "ggpu_gw_iterate", "ggpu_gw_iterate_device", "ggpu_gw_iterate_many", "ggpu_gw_polarization",
             "ggpu_gw_last_polarization", "ggpu_gw_set_pair_batch_limit",
             "ggpu_gw_last_timings", "ggpu_gw_set_profiling", "ggpu_gw_site_stats",
@@ -407,6 +411,20 @@ def device_memory():
     return f.value, t.value
 
 
+
+def _pair_reader(read_pair, pair_elems):
+    """ctypes ggpu_pair_reader around read_pair(pos) -> NQ * nao^2 complex
+    values of stored pair pos (a Python exception becomes status 1)."""
+    def cb(user, pos, dst):
+        try:
+            out = np.ctypeslib.as_array(dst, shape=(2 * pair_elems,)).view(np.complex128)
+            out[:] = np.asarray(read_pair(int(pos)), dtype=np.complex128).reshape(-1)
+            return 0
+        except Exception:
+            return 1
+    return PAIR_READER(cb)
+
+
 # ---------------------------------------------------------------------------
 # Kernels
 # ---------------------------------------------------------------------------
@@ -434,6 +452,12 @@ class HfKernel:
     def load_integrals(self, V_stored):
         V = _c128(V_stored)
         _check(lib().ggpu_hf_load_integrals(self._h, _dptr(V)))
+
+    def set_integral_reader(self, read_pair):
+        """READER residence: stored pair pos comes from read_pair(pos) (NQ x
+        nao x nao complex) whenever a pair batch needs it."""
+        self._reader = _pair_reader(read_pair, self.NQ * self.nao * self.nao)
+        _check(lib().ggpu_hf_set_integral_reader(self._h, self._reader, None))
 
     def synth_integrals(self, seed, sigma=0.1):
         _check(lib().ggpu_hf_synth_integrals(self._h, int(seed), float(sigma)))
@@ -507,6 +531,12 @@ class GwKernel:
     def load_integrals(self, V_stored):
         V = _c128(V_stored)
         _check(lib().ggpu_gw_load_integrals(self._h, _dptr(V)))
+
+    def set_integral_reader(self, read_pair):
+        """READER residence: stored pair pos comes from read_pair(pos) (NQ x
+        nao x nao complex) whenever a pair batch needs it."""
+        self._reader = _pair_reader(read_pair, self.NQ * self.nao * self.nao)
+        _check(lib().ggpu_gw_set_integral_reader(self._h, self._reader, None))
 
     def synth_integrals(self, seed, sigma=0.1):
         _check(lib().ggpu_gw_synth_integrals(self._h, int(seed), float(sigma)))

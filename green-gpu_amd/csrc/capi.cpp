@@ -115,6 +115,15 @@ int ggpu_hf_load_integrals(ggpu_hf_plan* plan, const double* V_stored) {
   });
 }
 
+int ggpu_hf_set_integral_reader(ggpu_hf_plan* plan, ggpu_pair_reader reader, void* user) {
+  return guarded([&] {
+    ggpu::require(plan && plan->impl && reader, "NULL argument");
+    auto* p = plan->impl;
+    p->store.set_residence(ggpu::IntegralStore::kResReader, p->kb, (int64_t)p->NQ * p->nao * p->nao);
+    p->store.set_reader(reader, user, p->sym.num_stored);
+  });
+}
+
 int ggpu_hf_synth_integrals(ggpu_hf_plan* plan, uint64_t seed, double sigma) {
   return guarded([&] {
     ggpu::require(plan && plan->impl, "NULL plan");
@@ -220,6 +229,16 @@ int ggpu_gw_load_integrals(ggpu_gw_plan* plan, const double* V_stored) {
     auto* p = plan->impl;
     p->invalidate_graph();
     p->store.load_host(V_stored, p->sym.num_stored * (int64_t)p->NQ * p->nao * p->nao, p->stream);
+  });
+}
+
+int ggpu_gw_set_integral_reader(ggpu_gw_plan* plan, ggpu_pair_reader reader, void* user) {
+  return guarded([&] {
+    ggpu::require(plan && plan->impl && reader, "NULL argument");
+    auto* p = plan->impl;
+    p->invalidate_graph();
+    p->store.set_residence(ggpu::IntegralStore::kResReader, p->kb, (int64_t)p->NQ * p->nao * p->nao);
+    p->store.set_reader(reader, user, p->sym.num_stored);
   });
 }
 

@@ -149,11 +149,14 @@ IntegralStore::~IntegralStore() {
   }
   if (copy_) (void)hipStreamDestroy(copy_);
   if (host_) (void)hipHostFree(host_);
+  for (auto h : hstage_)
+    if (h) (void)hipHostFree(h);
 }
 
 void IntegralStore::set_residence(int mode, int slots, int64_t pair_elems) {
   require(!loaded_, "integrals: choose the residence before loading integrals", kErrState);
-  require(mode == kResDevice || mode == kResHost || mode == kResGenerated, "integrals: unknown residence");
+  require(mode == kResDevice || mode == kResHost || mode == kResGenerated || mode == kResReader,
+          "integrals: unknown residence");
   mode_ = mode;
   if (mode == kResDevice) return;
   require(slots >= 1 && pair_elems > 0, "integrals: bad staging size");
@@ -180,9 +183,23 @@ void IntegralStore::alloc_host(int64_t n_elems) {
   GGPU_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_), (size_t)n_elems * sizeof(z64), hipHostMallocDefault));
 }
 
+void IntegralStore::set_reader(Reader fn, void* user, int64_t num_stored) {
+  require(fn != nullptr, "integrals: NULL pair reader");
+  require(mode_ == kResReader, "integrals: a pair reader needs the READER residence", kErrState);
+  for (int b = 0; b < 2; ++b)
+    if (!hstage_[b])
+      GGPU_HIP(hipHostMalloc(reinterpret_cast<void**>(&hstage_[b]), (size_t)slots_ * pair_elems_ * sizeof(z64),
+                             hipHostMallocDefault));
+  reader_ = fn;
+  reader_user_ = user;
+  n_elems_ = num_stored * pair_elems_;
+  loaded_ = true;
+}
+
 void IntegralStore::load_host(const double* V_stored, int64_t n_elems, hipStream_t s) {
   require(V_stored != nullptr, "integrals: NULL pointer");
   require(mode_ != kResGenerated, "integrals: a generated store takes no loaded integrals", kErrState);
+  require(mode_ != kResReader, "integrals: a reader store takes no loaded integrals", kErrState);
   if (mode_ == kResHost) {
     alloc_host(n_elems);
     std::memcpy(host_, V_stored, (size_t)n_elems * sizeof(z64));
@@ -196,6 +213,7 @@ void IntegralStore::load_host(const double* V_stored, int64_t n_elems, hipStream
 }
 
 void IntegralStore::synth(const SymTables& t, int NQ, int nao, uint64_t seed, double sigma, hipStream_t s) {
+  require(mode_ != kResReader, "integrals: a reader store takes no synthetic integrals", kErrState);
   // diagonal flag per stored position: find the pair index that maps to it
   std::vector<int> diag(t.num_stored, 0);
   for (int64_t k1 = 0; k1 < t.nk; ++k1) {
@@ -243,9 +261,25 @@ void IntegralStore::issue(const PairTable& tab, int i0, int n, int buf) {
     GGPU_HIP(hipEventCreate(&e));
     tpool_.push_back(e);
   }
-  GGPU_HIP(hipEventRecord(tpool_[tused_], copy_));
   const size_t bytes = (size_t)pair_elems_ * sizeof(z64);
-  for (int i = 0; i < n; ++i) {
+  if (mode_ == kResReader) {
+    // the previous copies out of this host buffer have completed before it
+    // is refilled (an event never recorded completes at once)
+    GGPU_HIP(hipEventSynchronize(ev_copy_[buf]));
+    z64* h = hstage_[buf];
+    for (int i = 0; i < n; ++i) {
+      const int64_t pos = tab.host[i0 + i].pos;
+      require(pos >= 0 && (pos + 1) * pair_elems_ <= n_elems_, "integrals: pair outside store");
+      const int rc = reader_(reader_user_, pos, reinterpret_cast<double*>(h + (int64_t)i * pair_elems_));
+      require(rc == 0, "integrals: the pair reader failed for stored pair " + std::to_string(pos) + " (status " +
+                           std::to_string(rc) + ")");
+    }
+  }
+  GGPU_HIP(hipEventRecord(tpool_[tused_], copy_));
+  if (mode_ == kResReader)
+    GGPU_HIP(hipMemcpyAsync(stage_.get() + (int64_t)buf * slots_ * pair_elems_, hstage_[buf], bytes * n,
+                            hipMemcpyHostToDevice, copy_));
+  for (int i = 0; i < n && mode_ != kResReader; ++i) {
     const int64_t pos = tab.host[i0 + i].pos;
     require(pos >= 0 && (pos + 1) * pair_elems_ <= n_elems_, "integrals: pair outside store");
     z64* dst = stage_.get() + ((int64_t)buf * slots_ + i) * pair_elems_;

@@ -45,11 +45,15 @@ class IntegralStore {
  public:
   ~IntegralStore();
   // Where the stored pairs live (before loading): kResDevice (HBM),
-  // kResHost (pinned host memory, copied per pair batch into staging) or
+  // kResHost (pinned host memory, copied per pair batch into staging),
   // kResGenerated (synthetic pairs generated per batch into staging on the
-  // device; only synth() fills such a store).
-  enum { kResDevice = 0, kResHost = 1, kResGenerated = 2 };
+  // device; only synth() fills such a store) or kResReader (each batch's
+  // pairs fetched through a caller's reader into pinned host staging, then
+  // copied like kResHost; only set_reader() fills such a store).
+  enum { kResDevice = 0, kResHost = 1, kResGenerated = 2, kResReader = 3 };
+  typedef int (*Reader)(void* user, int64_t pos, double* dst);
   void set_residence(int mode, int slots, int64_t pair_elems);
+  void set_reader(Reader fn, void* user, int64_t num_stored);
   void load_host(const double* V_stored, int64_t n_elems, hipStream_t s);
   void synth(const SymTables& t, int NQ, int nao, uint64_t seed, double sigma, hipStream_t s);
   bool loaded() const { return loaded_; }
@@ -75,6 +79,11 @@ class IntegralStore {
   void issue(const PairTable& tab, int i0, int n, int buf);
   DeviceBuffer<z64> V_;
   z64* host_ = nullptr;
+  // reader residence: the callback and two pinned batch buffers (one per
+  // device staging buffer; each is refilled after its last copy completed)
+  Reader reader_ = nullptr;
+  void* reader_user_ = nullptr;
+  z64* hstage_[2] = {nullptr, nullptr};
   int mode_ = kResDevice;
   bool loaded_ = false;
   // generated residence: the synth() parameters and per-position diagonal flags

@@ -25,6 +25,7 @@
 #include <green/utils/mpi_shared.h>
 #include <green/utils/mpi_utils.h>
 
+#include <algorithm>
 #include <complex>
 #include <cstdint>
 #include <cstring>
@@ -158,6 +159,43 @@ namespace green::gpu {
       ggpu_symmetry                     s_{};
     };
 
+    // cuda_low_cpu_memory: the reference reads the chunk holding each k-pair
+    // when a (k1, k2) loop needs it, caching one chunk
+    // (df_integral_t::read_integrals, df_integral_t.h:61-84); here the plan
+    // asks for the stored pairs of each pair batch through
+    // ggpu_*_set_integral_reader, and this cache reads their chunk files.
+    class chunk_cache {
+    public:
+      chunk_cache(const std::string& path, size_t nao, size_t NQ, size_t num_stored) :
+          reader_(path, nao, NQ), num_stored_(num_stored) {}
+
+      // ggpu_pair_reader
+      static int read(void* user, int64_t pos, double* dst) noexcept {
+        try {
+          auto& c = *static_cast<chunk_cache*>(user);
+          if (pos < 0 || static_cast<size_t>(pos) >= c.num_stored_) return 2;
+          const size_t cs = c.reader_.chunk_size(), first = static_cast<size_t>(pos) / cs * cs;
+          if (static_cast<long>(first) != c.current_) {
+            const size_t count = std::min(cs, c.num_stored_ - first);
+            c.chunk_.resize(count * c.reader_.pair_elements());
+            c.reader_.read_chunk(first, count, c.chunk_.data());
+            c.current_ = static_cast<long>(first);
+          }
+          std::memcpy(dst, c.chunk_.data() + (static_cast<size_t>(pos) - first) * c.reader_.pair_elements(),
+                      c.reader_.pair_elements() * sizeof(std::complex<double>));
+          return 0;
+        } catch (...) {
+          return 1;
+        }
+      }
+
+    private:
+      df_reader                         reader_;
+      size_t                            num_stored_;
+      long                              current_ = -1;
+      std::vector<std::complex<double>> chunk_;
+    };
+
     // HF kernel: hf_gpu_kernel::solve (hf_gpu_kernel.cpp:66-104) returns the
     // Fock contribution J + K - Ewald on the IBZ; X2C when nso == 2 nao.
     class hf_kernel {
@@ -172,9 +210,15 @@ namespace green::gpu {
         plan_.reset(h);
         // the reference's exchange is FP64 (hf_gpu_kernel.h:67-84)
         check(ggpu_hf_set_precision(plan_.get(), precision_param(p, GGPU_PRECISION_FP64)));
-        df_reader reader(p["dfintegral_hf_file"].as<std::string>(), nao, NQ);
-        const auto V = reader.read_entire(tables_.num_stored());
-        check(ggpu_hf_load_integrals(plan_.get(), reinterpret_cast<const double*>(V.data())));
+        const std::string path = p["dfintegral_hf_file"].as<std::string>();
+        if (p["cuda_low_cpu_memory"].as<bool>()) {
+          chunks_ = std::make_unique<chunk_cache>(path, nao, NQ, tables_.num_stored());
+          check(ggpu_hf_set_integral_reader(plan_.get(), &chunk_cache::read, chunks_.get()));
+        } else {
+          df_reader  reader(path, nao, NQ);
+          const auto V = reader.read_entire(tables_.num_stored());
+          check(ggpu_hf_load_integrals(plan_.get(), reinterpret_cast<const double*>(V.data())));
+        }
       }
 
       ztensor<4> solve(const ztensor<4>& dm) {
@@ -187,6 +231,7 @@ namespace green::gpu {
       }
 
     private:
+      std::unique_ptr<chunk_cache> chunks_;   // outlives the plan that reads through it
       std::unique_ptr<ggpu_hf_plan, void (*)(ggpu_hf_plan*)> plan_{nullptr, ggpu_hf_plan_destroy};
       size_t            nso_, ns_;
       double            madelung_;
@@ -214,9 +259,15 @@ namespace green::gpu {
                                   reinterpret_cast<const double*>(T_wt.data()),
                                   reinterpret_cast<const double*>(T_tw.data()), prec, lin, 0));
         plan_.reset(h);
-        df_reader reader(p["dfintegral_file"].as<std::string>(), nao, NQ);
-        const auto V = reader.read_entire(tables_.num_stored());
-        check(ggpu_gw_load_integrals(plan_.get(), reinterpret_cast<const double*>(V.data())));
+        const std::string path = p["dfintegral_file"].as<std::string>();
+        if (p["cuda_low_cpu_memory"].as<bool>()) {
+          chunks_ = std::make_unique<chunk_cache>(path, nao, NQ, tables_.num_stored());
+          check(ggpu_gw_set_integral_reader(plan_.get(), &chunk_cache::read, chunks_.get()));
+        } else {
+          df_reader  reader(path, nao, NQ);
+          const auto V = reader.read_entire(tables_.num_stored());
+          check(ggpu_gw_load_integrals(plan_.get(), reinterpret_cast<const double*>(V.data())));
+        }
         if (p["gpu_graph"].as<bool>()) check(ggpu_gw_set_graph(plan_.get(), 1));
         autotune_ = p["gpu_autotune"].as<bool>();
       }
@@ -236,6 +287,7 @@ namespace green::gpu {
       }
 
     private:
+      std::unique_ptr<chunk_cache> chunks_;   // outlives the plan that reads through it
       std::unique_ptr<ggpu_gw_plan, void (*)(ggpu_gw_plan*)> plan_{nullptr, ggpu_gw_plan_destroy};
       symmetry_tables tables_;
       bool            autotune_ = false;

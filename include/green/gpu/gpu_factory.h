@@ -60,7 +60,9 @@ namespace green::gpu {
     p.define<LinearSolverType>("cuda_linear_solver", "solver of the screened-interaction equation: LU or Cholesky",
                                LinearSolverType::LU);
     p.define<bool>("cuda_low_gpu_memory", "accepted for compatibility (device batches are sized automatically)");
-    p.define<bool>("cuda_low_cpu_memory", "accepted for compatibility (integrals are read once per kernel)");
+    p.define<bool>("cuda_low_cpu_memory",
+                   "host has little memory: read the integral chunks of each pair batch on demand (one chunk "
+                   "cached) instead of holding every stored pair");
     p.define<size_t>("nt_batch", "accepted for compatibility (tau batches are sized automatically)", 0);
     // no counterpart in the reference: arithmetic, launch-variant tuning and
     // graph replay of this implementation

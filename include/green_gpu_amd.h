@@ -70,12 +70,26 @@ typedef enum {
  * GENERATED: nothing is stored; each batch of synthetic pairs (the values
  * *_synth_integrals defines) is generated on the device into the staging area
  * when it is needed, so systems whose integrals fit neither HBM nor host
- * memory can be run; only *_synth_integrals fills such a plan. */
+ * memory can be run; only *_synth_integrals fills such a plan.
+ * READER: nothing is stored either; each batch's stored pairs are fetched
+ * through the caller's ggpu_pair_reader (*_set_integral_reader) into pinned
+ * host staging and copied like HOST batches, so real integrals larger than
+ * host memory stream from disk: the reference's low-CPU-memory "chunks"
+ * mode, df_integral_t::read_integrals (df_integral_t.h:61-84) reading the
+ * chunk of each (k1, k2) pair on demand (gpu_kernel.h:47-51). */
 typedef enum {
   GGPU_INTEGRALS_DEVICE = 0,
   GGPU_INTEGRALS_HOST = 1,
-  GGPU_INTEGRALS_GENERATED = 2
+  GGPU_INTEGRALS_GENERATED = 2,
+  GGPU_INTEGRALS_READER = 3
 } ggpu_integral_residence;
+
+/* Fills one stored k-pair (stored position pos, 0 <= pos < num_kpair_stored)
+ * into dst: NQ * nao * nao complex128 as interleaved doubles, the layout of
+ * one entry of *_load_integrals' array.  Returns 0 on success; any other
+ * value fails the calling iterate / exchange with GGPU_ERR_ARGUMENT.  Called
+ * on the thread that runs the plan, one pair batch ahead of the device. */
+typedef int (*ggpu_pair_reader)(void* user, int64_t pos, double* dst);
 
 /*
  * Symmetry and integral-storage tables: the plain data green-symmetry
@@ -162,6 +176,10 @@ int ggpu_hf_set_k_share(ggpu_hf_plan* plan, int rank, int size);
  * the buffer df_integral_t::read_entire fills, df_integral_t.h:96-129) to the
  * device.  The host array is not referenced after the call. */
 int ggpu_hf_load_integrals(ggpu_hf_plan* plan, const double* V_stored);
+/* READER residence: the plan's integrals come from `reader` (with `user`
+ * passed back) per pair batch; replaces *_load_integrals.  Sets the residence
+ * to GGPU_INTEGRALS_READER. */
+int ggpu_hf_set_integral_reader(ggpu_hf_plan* plan, ggpu_pair_reader reader, void* user);
 
 /* Fill the device integral store with the synthetic counter-based generator
  * (Re, Im ~ N(0, sigma^2); diagonal pairs Hermitised).  Benchmarks only. */
@@ -252,6 +270,8 @@ int ggpu_gw_staging_ms(ggpu_gw_plan* plan, double* ms);
 int ggpu_gw_set_q_share(ggpu_gw_plan* plan, int rank, int size);
 int ggpu_gw_load_integrals(ggpu_gw_plan* plan, const double* V_stored);
 int ggpu_gw_synth_integrals(ggpu_gw_plan* plan, uint64_t seed, double sigma);
+/* READER residence for a GW plan (as ggpu_hf_set_integral_reader). */
+int ggpu_gw_set_integral_reader(ggpu_gw_plan* plan, ggpu_pair_reader reader, void* user);
 
 /* One GW iteration: Sigma(tau) from G(tau).
  * G, Sigma_out: (nts, ns, ink, nso, nso) complex, Sigma_out overwritten

@@ -11,8 +11,10 @@
 //   5. when GW inputs are present (gw_sizes.txt), writes the GW integrals in
 //      the chunked format, builds a transformer_t from the given bosonic
 //      transforms and runs custom_gw_kernel's callback on G (LU solver, as
-//      green-mbpt's default): without a GPU it must fail with the device
-//      error, with one it writes Sigma(tau) to <dir>/Sigma.bin.
+//      green-mbpt's default) twice, with the integrals read whole and read
+//      chunk by chunk per pair batch (cuda_low_cpu_memory): without a GPU it
+//      must fail with the device error, with one it writes Sigma(tau) to
+//      <dir>/Sigma.bin and <dir>/Sigma_chunks.bin.
 // Input (written by tests/test_adapter.py into <dir>): sizes.txt, tables.txt,
 // V.bin (stored pairs, complex128), dm.bin and S.bin (ns, ink, nso, nso);
 // for GW: gw_sizes.txt (nts, nw_b, chunk), T_tw.bin (nts, nw_b), T_wt.bin
@@ -222,24 +224,31 @@ int main(int argc, char** argv) {
     p.set("dfintegral_file", dfg);
     p.set("cuda_linear_solver", "LU");
     using G_type = green::gpu::G_type;
-    green::gpu::ztensor<5> G0(nts, ns, ink, nso, nso), S0(nts, ns, ink, nso, nso);
+    green::gpu::ztensor<5> G0(nts, ns, ink, nso, nso);
     {
       const auto g = read_bin(dir + "/gw_G.bin", G0.size());
       std::memcpy(G0.data(), g.data(), g.size() * sizeof(cd));
     }
-    G_type g_sh(std::move(G0)), s_sh(std::move(S0));
-    try {
-      auto [kernel, solve] = green::gpu::custom_gw_kernel(nso != nao, p, nao, nso, ns, NQ, ft, bz, S_k);
-      solve(g_sh, s_sh);
-      const auto&   S = s_sh.object();
-      std::ofstream out(dir + "/Sigma.bin", std::ios::binary);
-      out.write(reinterpret_cast<const char*>(S.data()), S.size() * sizeof(cd));
-      std::cout << "gw kernel: solved on the device" << std::endl;
-      EXPECT(ndev > 0, "gw kernel ran without a device");
-    } catch (const std::runtime_error& e) {
-      std::cout << "gw kernel: " << e.what() << std::endl;
-      EXPECT(ndev == 0, std::string("gw kernel failed with a device present: ") + e.what());
+    G_type g_sh(std::move(G0));
+    // the integrals read whole, then chunk by chunk per pair batch (cuda_low_cpu_memory)
+    for (const char* low : {"false", "true"}) {
+      p.set("cuda_low_cpu_memory", low);
+      const std::string tag = std::string(low) == "true" ? " (chunked reads)" : "";
+      G_type            s_sh(green::gpu::ztensor<5>(nts, ns, ink, nso, nso));
+      try {
+        auto [kernel, solve] = green::gpu::custom_gw_kernel(nso != nao, p, nao, nso, ns, NQ, ft, bz, S_k);
+        solve(g_sh, s_sh);
+        const auto&   S = s_sh.object();
+        std::ofstream out(dir + (tag.empty() ? "/Sigma.bin" : "/Sigma_chunks.bin"), std::ios::binary);
+        out.write(reinterpret_cast<const char*>(S.data()), S.size() * sizeof(cd));
+        std::cout << "gw kernel" << tag << ": solved on the device" << std::endl;
+        EXPECT(ndev > 0, "gw kernel ran without a device");
+      } catch (const std::runtime_error& e) {
+        std::cout << "gw kernel" << tag << ": " << e.what() << std::endl;
+        EXPECT(ndev == 0, std::string("gw kernel failed with a device present: ") + e.what());
+      }
     }
+    p.set("cuda_low_cpu_memory", "false");
   }
   // optional: a directory written by green-mbpt itself
   if (argc > 2) {

@@ -7,7 +7,8 @@ stop with the library's device error in both kernels (with one, its Fock
 matrices are checked against the reference's Sigma1 and the GW kernel's
 Sigma(tau), through custom_gw_kernel with a transformer_t carrying the
 Lambda = 1e4 transforms on the grid recovered from the fixture, against the
-reference's Sigma_tau)."""
+reference's Sigma_tau, with the integrals read whole and read chunk by chunk
+per pair batch under cuda_low_cpu_memory)."""
 import os
 import shutil
 import subprocess
@@ -116,6 +117,7 @@ def test_adapter_hf_fixture_on_device(tmp_path, fx, oracle):
     d = str(tmp_path)
     out = run_adapter(d, fx, oracle)
     assert "hf kernel: solved on the device" in out and "gw kernel: solved on the device" in out
+    assert "gw kernel (chunked reads): solved on the device" in out
     F = np.fromfile(os.path.join(d, "F.bin"), dtype=np.complex128).reshape(fx["hf_Sigma1"].shape)
     assert max_abs(-F, fx["hf_Sigma1"]) < 1e-8
     S = np.fromfile(os.path.join(d, "Sigma.bin"), dtype=np.complex128).reshape(fx["gw_Sigma_tau"].shape)
@@ -126,3 +128,6 @@ def test_adapter_hf_fixture_on_device(tmp_path, fx, oracle):
     err_ref, err_or = max_abs(S, fx["gw_Sigma_tau"]), max_abs(S, S_or)
     print(f"adapter GW: vs Sigma_tau {err_ref:.2e}, vs oracle {err_or:.2e}", flush=True)
     assert err_ref < 1e-6 and err_or < 1e-9
+    # cuda_low_cpu_memory: the same Sigma with the pairs read chunk by chunk
+    S_chunks = np.fromfile(os.path.join(d, "Sigma_chunks.bin"), dtype=np.complex128).reshape(S.shape)
+    assert np.array_equal(S_chunks, S)

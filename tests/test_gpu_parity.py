@@ -222,6 +222,48 @@ def test_integrals_streamed_from_host(ggpu, synth, oracle):
     assert np.array_equal(hk["device"].exchange(D), hk["host"].exchange(D))
 
 
+def test_integrals_from_a_pair_reader(ggpu, synth, oracle):
+    """READER residence (the low-host-memory mode): every pair batch is fetched
+    through the caller's reader into pinned staging.  Identical results to
+    HBM-resident integrals for GW (small batches, so both staging buffers
+    are reused) and HF, every stored pair read at least once, and a failing
+    reader turns into an error status, not a wrong result."""
+    mesh, nao, NQ, ns = (2, 2, 1), 13, 40, 2
+    sym, tau, T_wt, T_tw, G, D, V = gw_system(ggpu, synth, mesh, nao, NQ, ns, 50.0, 1e3, 43)
+    pair_bytes = NQ * nao * nao * 16
+    work = 3 * (2 + 2 * ns * tau.size) * pair_bytes
+    reads = []
+
+    def read_pair(pos):
+        reads.append(pos)
+        return V[pos]
+
+    print("GW with integrals from a pair reader", flush=True)
+    ref = ggpu.GwKernel(nao, ns, NQ, T_wt, T_tw, sym, solver="cholesky", work_bytes=work)
+    ref.load_integrals(V)
+    S_ref = ref.solve(G)
+    k = ggpu.GwKernel(nao, ns, NQ, T_wt, T_tw, sym, solver="cholesky", work_bytes=work, integrals="reader")
+    k.set_integral_reader(read_pair)
+    S = k.solve(G)
+    assert np.array_equal(S, S_ref)
+    assert sorted(set(reads)) == list(range(V.shape[0]))
+    assert k.streamed_bytes() == pair_bytes * len(reads)
+    assert np.array_equal(k.solve(G), S_ref)
+    h_ref = ggpu.HfKernel(nao, ns, NQ, sym)
+    h_ref.load_integrals(V)
+    h = ggpu.HfKernel(nao, ns, NQ, sym, integrals="reader")
+    h.set_integral_reader(lambda pos: V[pos])
+    assert np.array_equal(h.exchange(D), h_ref.exchange(D))
+
+    def broken(pos):
+        raise IOError("chunk file missing")
+
+    kb = ggpu.GwKernel(nao, ns, NQ, T_wt, T_tw, sym, solver="cholesky", work_bytes=work, integrals="reader")
+    kb.set_integral_reader(broken)
+    with pytest.raises(ggpu.GgpuError, match="pair reader failed"):
+        kb.solve(G)
+
+
 def test_generated_integrals_and_q_shares(ggpu, synth, oracle):
     """GENERATED residence (synthetic pairs generated per batch on the device,
     nothing stored) gives Sigma and K bit for bit equal to the same synthetic
