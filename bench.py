@@ -220,7 +220,12 @@ def run_hf(a, w, pkg, sym, G, modes, seed, sigma, deadline, t_start):
            "e_hf": h64["e_hf"], "precision": "fp64",
            "includes": "K + J + E_HF, host-to-host",
            "e_hf_definition": "(1/nk) sum_{s, k in IBZ} |star(k)| Re Tr[D_s(k) F_s(k)] / 2",
-           "k_share": f"{k_rank}/{k_size}", "ik_built": n_ik, "modes": res}
+           "k_share": f"{k_rank}/{k_size}", "ik_built": n_ik, "modes": res,
+           # the FP64 build against the FP64 matrix peak (exchange FLOPs over
+           # the whole host-to-host build, so staging, J and the reductions
+           # count against it)
+           "roofline": {"bound": "mfma", "achieved": h64["exchange_tflops"], "peak": PEAK["f64"],
+                        "unit": "TFLOP/s", "frac": round(h64["exchange_tflops"] / PEAK["f64"], 4)}}
     return out
 
 

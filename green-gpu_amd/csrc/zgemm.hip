@@ -286,7 +286,6 @@ __global__ __launch_bounds__(NTHREADS) __attribute__((amdgpu_waves_per_eu(WPE)))
 // contiguous per row, so every MFMA fragment is one 16-byte LDS read.
 // ---------------------------------------------------------------------------
 constexpr int XBK = 32;          // k per MFMA step
-int g_bf16_tiles = -1;           // split-bf16 tile choice: -1 automatic, else a fixed kBf16Tile* value
 constexpr int XPAD = 8;          // bf16 elements of row padding
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef unsigned int v4u __attribute__((ext_vector_type(4)));
@@ -849,7 +848,7 @@ inline double tile_use(int n, int t) { return (double)n / (double)(((n + t - 1) 
 
 template <class TO, int SITE>
 void launch_bf16x3_site(const GemmDesc& d, hipStream_t stream, int variant) {
-  int mode = g_bf16_tiles >= 0 ? g_bf16_tiles : variant;
+  int mode = variant;
   if (mode < 0) {
     // Two or three waves per SIMD (6- to 10-block wave tiles) beat bigger
     // tiles.  Large shapes take the tile measured fastest for their call site
@@ -931,10 +930,6 @@ void launch_site_bk(const GemmDesc& d, hipStream_t stream) {
   GGPU_HIP(hipGetLastError());
 }
 
-// 0: per-site choice; 8 or 16 forces the LDS k-depth, 83 / 84 k-depth 8 with
-// registers allocated for 3 / 4 waves per SIMD
-int g_mfma_bk = 0;
-
 template <class T, class TO, bool GAUSS, int SITE>
 void launch_site(const GemmDesc& d, hipStream_t stream, int variant) {
   // Variants measured on MI355X (nao = 52, NQ = 300, nts = 110; medians of
@@ -945,7 +940,7 @@ void launch_site(const GemmDesc& d, hipStream_t stream, int variant) {
   // All variants give the same sums bit for bit.
   constexpr bool f64 = std::is_same<T, double>::value;
   constexpr int pref = f64 ? 83 : 8;
-  switch (g_mfma_bk ? g_mfma_bk : (variant > 0 ? variant : pref)) {
+  switch (variant > 0 ? variant : pref) {
     case 8: return launch_site_bk<T, TO, GAUSS, SITE, 8, 1>(d, stream);
     case 83: return launch_site_bk<T, TO, GAUSS, SITE, 8, 3>(d, stream);
     case 84: return launch_site_bk<T, TO, GAUSS, SITE, 8, 4>(d, stream);
@@ -1057,8 +1052,6 @@ bool gemm_variant_valid(Compute c, int v) {
   const int n = gemm_variant_candidates(c, cand, 8);
   return std::find(cand, cand + n, v) != cand + n;
 }
-void set_bf16_tiles(int mode) { g_bf16_tiles = mode; }
-void set_mfma_bk(int bk) { g_mfma_bk = bk; }
 
 void gemm(const GemmDesc& dd, hipStream_t stream) {
   gemm_check_bounds(dd);

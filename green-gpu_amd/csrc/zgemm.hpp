@@ -94,21 +94,18 @@ enum {
   // launches take the 64 x 64 three-wave tile)
   kBf16TileAstat64x64 = 9
 };
-void set_bf16_tiles(int mode);
 // Variant of a Compute::I8X5 / I8X4 launch that runs the int8 emulation; the
 // FP64 kernel variants (8, 16, 83, 84) run the launch on the FP64 kernel
 // instead.  Built-in choice: the emulation at the Sigma P.Y1 product and the
 // P0 contraction (the sites where it is faster on the benchmark shapes), the
 // FP64 kernel everywhere else; per-site variants override it.
 enum { kVariantInt8 = 200 };
-// Variant of the FP64 / FP32 kernel: 0 picks per call site; 8 or 16 forces
-// the LDS k-depth; 83 or 84 forces k-depth 8 with registers allocated for 3 or
-// 4 waves per SIMD.  All variants give the same sums.
-void set_mfma_bk(int variant);
 // Launch variant per call site for the launches of the calling thread
 // (table of kNumSites entries, -1: the built-in choice; nullptr: none):
-// an LDS k-depth / waves-per-SIMD value of set_mfma_bk for FP64 and FP32,
-// a kBf16Tile* value for split-bf16.  The global switches above win.
+// for FP64 / FP32 8 or 16 forces the LDS k-depth, 83 or 84 k-depth 8 with
+// registers allocated for 3 or 4 waves per SIMD (all give the same sums);
+// for split-bf16 a kBf16Tile* value; in the int8 modes kVariantInt8 or an
+// FP64 kernel variant.
 void set_site_variants(const int* table);
 // Device workspace of the Compute::I8X5 / I8X4 launches of the calling thread
 // (nullptr: none; such launches then fail with an error).
