@@ -25,9 +25,11 @@
 // of every slice (2 NSL).
 //
 // A launch runs the slicing of both operands (per-row / per-column maximum
-// binary exponent, then the digits; one fused pass when the summed k extent
-// is short, an exponent pass and a digit pass otherwise) and the int8 GEMM
-// with the FP64 epilogue (alpha, beta, strided or complex64 output).
+// binary exponent, then the digits: from an LDS-resident copy of each 16-row
+// block when its k extent fits (one HBM read), in one fused pass reading the
+// rows twice when it is short, else an exponent pass and a digit pass) and
+// the int8 GEMM with the FP64 epilogue (alpha, beta, strided or complex64
+// output).
 #include "ozaki.hpp"
 
 #include <algorithm>
@@ -178,6 +180,90 @@ __global__ __launch_bounds__(NT) void oz_slice_kernel(const SliceView v) {
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
       double xr = ldexp(x[j].x, sg), xi = ldexp(v.conj ? -x[j].y : x[j].y, sg);
+      const int sh = 8 * (j & 3);
+#pragma unroll
+      for (int s = 0; s < NSL; ++s) {
+        const double dr = trunc(xr), di = trunc(xi);
+        xr = (xr - dr) * 128.0;
+        xi = (xi - di) * 128.0;
+        wre[s][j >> 2] |= (unsigned int)((int)dr & 0xff) << sh;
+        wim[s][j >> 2] |= (unsigned int)((int)di & 0xff) << sh;
+      }
+    }
+    int8_t* dst = v.out + (((int64_t)b * v.nkb + kb) * rblocks + rblk) * ((int64_t)NP * FRAG) + 16 * lane;
+#pragma unroll
+    for (int s = 0; s < NSL; ++s) {
+      *reinterpret_cast<uint4*>(dst + (2 * s) * FRAG) = make_uint4(wre[s][0], wre[s][1], wre[s][2], wre[s][3]);
+      *reinterpret_cast<uint4*>(dst + (2 * s + 1) * FRAG) = make_uint4(wim[s][0], wim[s][1], wim[s][2], wim[s][3]);
+    }
+  }
+}
+
+// Exponents and digits of one block of 16 rows of a row-contiguous operand
+// whose whole k extent fits in LDS (K * KB <= LDS_KT): the workgroup stages
+// the block once (16 rows x K * KB complex doubles, k-major: the 16 rows of
+// one k are one 256-byte run), finds the rows' exponents from LDS and writes
+// the digits from LDS -- one HBM read of the operand.  Grid: (1, Rp / 16,
+// batches); one workgroup per CU (the staging fills the LDS).
+constexpr int LDS_KT = 632;
+
+template <int NSL>
+__global__ __launch_bounds__(NT) void oz_slice_lds_kernel(const SliceView v) {
+  constexpr int NP = 2 * NSL;
+  __shared__ double2 T[LDS_KT][RB];
+  __shared__ int red[NT];
+  __shared__ int rowexp[RB];
+  const int64_t b = blockIdx.z;
+  const int rblk = blockIdx.y, r0 = rblk * RB, t = threadIdx.x;
+  const double2* base = batch_base(v, b);
+  const int kt = v.K * v.KB;
+  // stage (rows contiguous only): one load-to-LDS copy moves 4 k x 16 rows
+  // (64 lanes x 16 bytes, lane l -> T[k0 + l / 16][l % 16]); every copy of the
+  // block is in flight at once.  Lanes past the operand read a valid element
+  // (row 0 / the last k) and are masked where T is used.
+  {
+    const int lane = t & 63, w = t >> 6, r = lane % RB;
+    const int row = (r0 + r < v.rows) ? r0 + r : 0;
+    for (int k0 = 4 * w; k0 < kt; k0 += 16) {
+      const int kk = min(k0 + lane / RB, kt - 1);
+      const int q = kk / v.K, k = kk - q * v.K;
+      const double2* src = base + (int64_t)row * v.row_s + (int64_t)k * v.k_s + (int64_t)q * v.kb_s;
+      __builtin_amdgcn_global_load_lds((const void*)src, (__attribute__((address_space(3))) void*)&T[k0][0], 16, 0,
+                                       0);
+    }
+    __builtin_amdgcn_s_waitcnt(0x3f70);   // vmcnt(0): this wave's copies have landed
+  }
+  __syncthreads();
+  {
+    const int rr = t % RB, kc = t / RB;
+    int e = INT_MIN;
+    if (r0 + rr < v.rows)
+      for (int k = kc; k < kt; k += RB) e = max(e, max_exponent(T[k][rr]));
+    red[rr * RB + kc] = e;
+  }
+  __syncthreads();
+  if (t < RB) {
+    int e = INT_MIN;
+    for (int j = 0; j < RB; ++j) e = max(e, red[t * RB + j]);
+    if (r0 + t < v.rows) v.E[b * v.Rp + r0 + t] = e;
+    rowexp[t] = e;
+  }
+  __syncthreads();
+  const int lane = t & 63, w = t >> 6, lr = lane & 15, lk = lane >> 4;
+  const int row = r0 + lr;
+  const int sg = row < v.rows ? sigma_of(rowexp[lr]) : 0;
+  const int rblocks = v.Rp / RB;
+  for (int kb = w; kb < v.nkb; kb += 4) {
+    unsigned int wre[NSL][4], wim[NSL][4];
+#pragma unroll
+    for (int s = 0; s < NSL; ++s)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) { wre[s][c] = 0u; wim[s][c] = 0u; }
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const int k = kb * KBLK + 16 * lk + j;
+      const double2 x = (k < kt && row < v.rows) ? T[k][lr] : make_double2(0.0, 0.0);
+      double xr = ldexp(x.x, sg), xi = ldexp(v.conj ? -x.y : x.y, sg);
       const int sh = 8 * (j & 3);
 #pragma unroll
       for (int s = 0; s < NSL; ++s) {
@@ -466,6 +552,11 @@ template <int NSL>
 void slice_operand(SliceView v, int64_t nb, hipStream_t s) {
   require(nb >= 1 && nb <= 65535 && v.Rp / RB <= 65535 && v.nkb >= 1, "int8 emulation: operand too large");
   const int64_t kt = (int64_t)v.K * v.KB;
+  if (kt <= LDS_KT && v.row_s == 1 && v.k_s != 1) {
+    hipLaunchKernelGGL((oz_slice_lds_kernel<NSL>), dim3(1, (unsigned)(v.Rp / RB), (unsigned)nb), dim3(NT), 0, s, v);
+    GGPU_HIP(hipGetLastError());
+    return;
+  }
   if (kt <= FUSE_KT) {
     hipLaunchKernelGGL((oz_slice_kernel<true, NSL>), dim3(1, (unsigned)(v.Rp / RB), (unsigned)nb), dim3(NT), 0, s, v);
     GGPU_HIP(hipGetLastError());
