@@ -618,7 +618,8 @@ class GwKernel:
 
     def set_graph(self, on=True):
         """Capture the iteration into a HIP graph on the next solve and replay
-        it afterwards (integrals resident in HBM only)."""
+        it afterwards (integrals in HBM, streamed from host memory or
+        generated on the device; not with a pair reader)."""
         _check(lib().ggpu_gw_set_graph(self._h, 1 if on else 0))
 
     def graph_captures(self):

@@ -146,6 +146,7 @@ IntegralStore::~IntegralStore() {
   for (int b = 0; b < 2; ++b) {
     if (ev_copy_[b]) (void)hipEventDestroy(ev_copy_[b]);
     if (ev_used_[b]) (void)hipEventDestroy(ev_used_[b]);
+    if (ev_join_[b]) (void)hipEventDestroy(ev_join_[b]);
   }
   if (copy_) (void)hipStreamDestroy(copy_);
   if (host_) (void)hipHostFree(host_);
@@ -167,6 +168,7 @@ void IntegralStore::set_residence(int mode, int slots, int64_t pair_elems) {
     for (int b = 0; b < 2; ++b) {
       GGPU_HIP(hipEventCreateWithFlags(&ev_copy_[b], hipEventDisableTiming));
       GGPU_HIP(hipEventCreateWithFlags(&ev_used_[b], hipEventDisableTiming));
+      GGPU_HIP(hipEventCreateWithFlags(&ev_join_[b], hipEventDisableTiming));
     }
   }
   stage_.resize((size_t)2 * slots * pair_elems);
@@ -255,8 +257,10 @@ void IntegralStore::synth(const SymTables& t, int NQ, int nao, uint64_t seed, do
 // reader of that buffer is done.
 void IntegralStore::issue(const PairTable& tab, int i0, int n, int buf) {
   require(n <= slots_, "integrals: batch larger than the staging area");
-  GGPU_HIP(hipStreamWaitEvent(copy_, ev_used_[buf], 0));
-  while (tpool_.size() < tused_ + 2) {
+  // inside a capture only events recorded in it may be waited on; the copy
+  // stream joined the capture behind everything captured before
+  if (!capturing_ || used_in_capture_[buf]) GGPU_HIP(hipStreamWaitEvent(copy_, ev_used_[buf], 0));
+  while (!capturing_ && tpool_.size() < tused_ + 2) {
     hipEvent_t e;
     GGPU_HIP(hipEventCreate(&e));
     tpool_.push_back(e);
@@ -275,7 +279,7 @@ void IntegralStore::issue(const PairTable& tab, int i0, int n, int buf) {
                            std::to_string(rc) + ")");
     }
   }
-  GGPU_HIP(hipEventRecord(tpool_[tused_], copy_));
+  if (!capturing_) GGPU_HIP(hipEventRecord(tpool_[tused_], copy_));
   if (mode_ == kResReader)
     GGPU_HIP(hipMemcpyAsync(stage_.get() + (int64_t)buf * slots_ * pair_elems_, hstage_[buf], bytes * n,
                             hipMemcpyHostToDevice, copy_));
@@ -289,8 +293,10 @@ void IntegralStore::issue(const PairTable& tab, int i0, int n, int buf) {
       GGPU_HIP(hipMemcpyAsync(dst, host_ + pos * pair_elems_, bytes, hipMemcpyHostToDevice, copy_));
   }
   GGPU_HIP(hipEventRecord(ev_copy_[buf], copy_));
-  GGPU_HIP(hipEventRecord(tpool_[tused_ + 1], copy_));
-  tused_ += 2;
+  if (!capturing_) {
+    GGPU_HIP(hipEventRecord(tpool_[tused_ + 1], copy_));
+    tused_ += 2;
+  }
   (mode_ == kResGenerated ? generated_ : streamed_) += (double)bytes * n;
 }
 
@@ -339,7 +345,29 @@ double IntegralStore::collect_staging_ms() {
 void IntegralStore::done(hipStream_t s) {
   if (mode_ == kResDevice || cur_ < 0) return;
   GGPU_HIP(hipEventRecord(ev_used_[cur_], s));
+  if (capturing_) used_in_capture_[cur_] = true;
   cur_ = -1;
+}
+
+void IntegralStore::begin_capture(hipStream_t s) {
+  if (mode_ == kResDevice) return;
+  require(mode_ != kResReader, "integrals: graph capture cannot replay a pair reader", kErrState);
+  require(cur_ < 0, "integrals: a staged batch is still in use", kErrState);
+  GGPU_HIP(hipEventRecord(ev_join_[0], s));
+  GGPU_HIP(hipStreamWaitEvent(copy_, ev_join_[0], 0));
+  capturing_ = true;
+  used_in_capture_[0] = used_in_capture_[1] = false;
+  pre_ = Pending{};
+  next_ = 0;
+}
+
+void IntegralStore::end_capture(hipStream_t s) {
+  if (mode_ == kResDevice || !capturing_) return;
+  capturing_ = false;
+  GGPU_HIP(hipEventRecord(ev_join_[1], copy_));
+  GGPU_HIP(hipStreamWaitEvent(s, ev_join_[1], 0));
+  // a prefetched batch of the capture is not valid outside it
+  pre_ = Pending{};
 }
 
 void PairTable::build(const std::vector<PairView>& views, int64_t stride, hipStream_t s) {
@@ -1213,8 +1241,11 @@ void GwPlan::capture(const z64* G_dev, z64* S_dev) {
   GGPU_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
   hipGraph_t graph = nullptr;
   try {
+    store.begin_capture(stream);
     enqueue_iteration(G_dev, S_dev, false);
+    store.end_capture(stream);
   } catch (...) {
+    store.abort_capture();
     (void)hipStreamEndCapture(stream, &graph);
     if (graph) (void)hipGraphDestroy(graph);
     throw;
@@ -1239,7 +1270,7 @@ void GwPlan::iterate_device(const z64* G_dev, z64* S_dev) {
     qev.push_back(e);
   }
   if (use_graph) {
-    require(!store.staged(), "gw: graph replay needs the integrals resident in HBM", kErrState);
+    require(!store.reader(), "gw: graph replay cannot replay a pair reader", kErrState);
     require(!profiling, "gw: per-site profiling does not apply to graph replay", kErrState);
     if (!gexec || graph_G != G_dev || graph_S != S_dev) {
       VariantGuard vguard(variants, &ozwork);

@@ -58,6 +58,7 @@ class IntegralStore {
   void synth(const SymTables& t, int NQ, int nao, uint64_t seed, double sigma, hipStream_t s);
   bool loaded() const { return loaded_; }
   bool staged() const { return mode_ != kResDevice; }
+  bool reader() const { return mode_ == kResReader; }
   int64_t elems() const { return n_elems_; }
   double bytes_streamed() const { return streamed_; }
   double bytes_generated() const { return generated_; }
@@ -71,6 +72,14 @@ class IntegralStore {
   // batches since the last call, from an event pair around every batch;
   // call after the streams have drained.
   double collect_staging_ms();
+  // Graph capture of an iteration on stream s: begin_capture() makes the
+  // copy stream join s's capture (and forgets buffer state recorded outside
+  // it), end_capture() joins it back; the staging copies or generation
+  // kernels then become nodes of the graph.  Not for the reader residence
+  // (its host-side reads cannot be captured).
+  void begin_capture(hipStream_t s);
+  void end_capture(hipStream_t s);
+  void abort_capture() { capturing_ = false; pre_ = Pending{}; cur_ = -1; }
 
  private:
   void alloc_host(int64_t n_elems);
@@ -99,6 +108,9 @@ class IntegralStore {
   DeviceBuffer<int64_t> stage_off_;        // [2][slots]: (buf * slots + b) * pair_elems
   struct Pending { const PairTable* tab = nullptr; int i0 = -1, n = 0, buf = -1; } pre_;
   int cur_ = -1, next_ = 0;
+  bool capturing_ = false;
+  bool used_in_capture_[2] = {false, false};   // ev_used_[b] recorded inside the capture
+  hipEvent_t ev_join_[2] = {nullptr, nullptr};
   double streamed_ = 0, generated_ = 0;
 };
 

@@ -347,10 +347,12 @@ int ggpu_gw_site_stats(ggpu_gw_plan* plan, int site, double* ms, double* flops, 
  * with one launch.  A new capture is made when G or Sigma change address
  * (ggpu_gw_iterate always uses the plan's own device copies) or after
  * ggpu_gw_set_q_share, *_set_variants, *_autotune, *_load_integrals,
- * *_synth_integrals or *_set_integral_residence.  Needs the integrals
- * resident in HBM (GGPU_INTEGRALS_DEVICE; GGPU_ERR_STATE otherwise) and no
- * per-site profiling; the results are those of the launches it replays, and
- * ggpu_gw_last_timings reports only the total.  No counterpart in the
+ * *_synth_integrals or *_set_integral_residence.  With staged integrals
+ * (HOST, GENERATED) the plan's copy stream joins the capture, so the staging
+ * copies or generation kernels of every pair batch are nodes of the graph and
+ * are replayed with it; the READER residence (host-side reads) and per-site
+ * profiling are refused (GGPU_ERR_STATE).  The results are those of the
+ * launches it replays, and ggpu_gw_last_timings reports only the total.  No counterpart in the
  * reference, which enqueues every launch from the host each iteration
  * (cugw_utils.cu:103-202). */
 int ggpu_gw_set_graph(ggpu_gw_plan* plan, int enable);

@@ -467,7 +467,7 @@ def test_graph_replay_matches_eager(ggpu, synth, oracle):
     """The iteration captured into one HIP graph and replayed: every replay
     equals the eager iteration bit for bit, with eager work and
     synchronisations between replays; a new G / Sigma address, a q share or a
-    new tuning makes a new capture; graph replay refuses streamed integrals."""
+    new tuning makes a new capture; graph replay refuses a pair reader."""
     mesh, nao, NQ, ns = (2, 2, 1), 13, 40, 2
     sym, tau, T_wt, T_tw, G, D, V = gw_system(ggpu, synth, mesh, nao, NQ, ns, 50.0, 1e3, 53)
     print("graph replay mesh=(2,2,1) nao=13 NQ=40", flush=True)
@@ -505,12 +505,46 @@ def test_graph_replay_matches_eager(ggpu, synth, oracle):
     Gd.free()
     Sd.free()
     import green_gpu_amd as g
-    host = ggpu.GwKernel(nao, ns, NQ, T_wt, T_tw, sym, solver="cholesky", integrals="host")
-    host.load_integrals(V)
-    host.set_graph(True)
+    rd = ggpu.GwKernel(nao, ns, NQ, T_wt, T_tw, sym, solver="cholesky", integrals="reader")
+    rd.set_integral_reader(lambda pos: V[pos])
+    rd.set_graph(True)
     with pytest.raises(g.GgpuError) as e:
-        host.solve(G)
-    assert e.value.code == 5 and "resident in HBM" in str(e.value)
+        rd.solve(G)
+    assert e.value.code == 5 and "pair reader" in str(e.value)
+
+
+@pytest.mark.parametrize("where", ["host", "generated"])
+def test_graph_replay_with_staged_integrals(ggpu, synth, oracle, where):
+    """Graph capture of an iteration whose integrals are staged per pair batch
+    on the plan's copy stream (host: pinned host memory copied; generated:
+    synthesised on the device): the copy stream joins the capture, so every
+    replay re-stages the batches and equals the eager iteration bit for bit,
+    with eager iterations of the same plan between replays.  Batches of a few
+    pairs, so both staging buffers are reused inside one replay."""
+    mesh, nao, NQ, ns = (2, 2, 1), 13, 40, 2
+    sym, tau, T_wt, T_tw, G, D, V = gw_system(ggpu, synth, mesh, nao, NQ, ns, 50.0, 1e3, 55)
+    pair_bytes = NQ * nao * nao * 16
+    work = 3 * (2 + 2 * ns * tau.size) * pair_bytes
+    print(f"graph replay, integrals {where}", flush=True)
+    k = ggpu.GwKernel(nao, ns, NQ, T_wt, T_tw, sym, solver="cholesky", work_bytes=work, integrals=where)
+    if where == "host":
+        k.load_integrals(V)
+    else:
+        k.synth_integrals(77, 0.1)
+    ref = k.solve(G)                                  # eager
+    G2 = synth.make_green(sym.nk, nao, ns, 50.0, tau, 56)[0]
+    ref2 = k.solve(G2)
+    k.set_graph(True)
+    for rep in range(3):
+        assert np.array_equal(k.solve(G), ref), f"replay {rep}"
+        assert np.array_equal(k.solve(G2), ref2), f"replay {rep}, second G"
+        k.set_graph(False)
+        assert np.array_equal(k.solve(G), ref)        # eager between replays
+        k.set_graph(True)
+    assert k.graph_captures() >= 1
+    if where == "host":
+        osym = oracle.symmetry_none(mesh, nao, NQ)
+        assert max_abs(ref, oracle.gw_solve(G, V, osym, T_wt, T_tw, solver="cholesky")) < TOL
 
 
 @pytest.mark.parametrize("precision,rtol", [("fp32", 2e-5), ("bf16x3", 2e-4)])
