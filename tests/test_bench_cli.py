@@ -26,12 +26,24 @@ def test_unknown_workload_is_rejected():
     assert "invalid choice" in r.stderr
 
 
-def test_graph_replay_needs_resident_integrals():
-    """A captured iteration replays fixed device addresses, so --graph is
-    refused for integrals streamed from host memory (the workload's default)."""
-    r = _run("--graph")
-    assert r.returncode != 0
-    assert "--integrals device" in r.stderr
+def test_int8_modes_price_each_site_on_its_engine():
+    """In the int8 modes a call site runs either the int8 emulation (priced
+    at 19 or 13 int8 ops per algorithmic FLOP against the 5 POP/s int8 peak)
+    or the FP64 kernel (3/4 against 78.6 TFLOP/s): the autotuner's pick when
+    it ran, else the built-in sites (P0 contraction, Sigma P.Y1)."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(ROOT, "bench.py"))
+    b = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(b)
+    assert b.site_engine("i8x4", "sig_py", None) == (13.0, 5000.0, "int8 emulation")
+    assert b.site_engine("i8x5", "p0_contract", None) == (19.0, 5000.0, "int8 emulation")
+    assert b.site_engine("i8x4", "sig_vg", None) == (0.75, 78.6, "fp64 kernel")
+    tune = {"chosen": {"sig_py": "bk8_w3", "sig_vg": "int8"}}
+    assert b.site_engine("i8x5", "sig_py", tune) == (0.75, 78.6, "fp64 kernel")
+    assert b.site_engine("i8x5", "sig_vg", tune) == (19.0, 5000.0, "int8 emulation")
+    assert b.site_engine("fp64", "sig_py", tune) == (0.75, 78.6, "fp64")
+    r = b.mode_roofline("i8x4", {"sig_py": (2000.0, 2.0e14, 10), "p0_gv": (10.0, 1.0e12, 10)})
+    assert r["engine"] == "int8 emulation" and abs(r["frac_mfma"] - 100.0 * 13 / 5000.0) < 1e-9
 
 
 def test_mode_roofline_counts_matrix_core_flops():
