@@ -387,28 +387,30 @@ __device__ inline z64 gauss_c(uint64_t seed, uint64_t e, double sigma) {
   const double u2 = (double)(h2 >> 11) * 0x1.0p-53;
   const double rad = sigma * sqrt(-2.0 * log(u1));
   const double ang = 6.283185307179586 * u2;
-  return z64{rad * cos(ang), rad * sin(ang)};
+  double sn, cs;
+  sincos(ang, &sn, &cs);
+  return z64{rad * cs, rad * sn};
 }
 
-__global__ void synth_integrals_kernel(z64* V, int64_t p0, int64_t np, const int* diag, int NQ, int nao,
-                                       uint64_t seed, double sigma) {
-  const int64_t per_pair = (int64_t)NQ * nao * nao;
-  const int64_t total = np * per_pair;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t lp = e / per_pair;
-    const int64_t rem = e - lp * per_pair;
-    const int64_t Q = rem / ((int64_t)nao * nao);
-    const int ij = (int)(rem - Q * nao * nao);
+// One pair per grid row (blockIdx.y), the pair's NQ * nao^2 elements along
+// x: no 64-bit index division per element (they cost more than the Gaussian
+// itself).  Counter of element (Q, i, j) of stored pair p:
+// (p * NQ + Q) * nao^2 + i * nao + j.
+__global__ void synth_integrals_kernel(z64* V, int64_t p0, const int* diag, int NQ, int nao, uint64_t seed,
+                                       double sigma) {
+  const int nn = nao * nao;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;   // element of the pair
+  if (e >= NQ * nn) return;
+  const int64_t lp = blockIdx.y;
+  const int Q = e / nn, ij = e - Q * nn;
+  const uint64_t base = ((uint64_t)(p0 + lp) * NQ + Q) * nn;
+  z64 v = gauss_c(seed, base + ij, sigma);
+  if (diag[lp]) {
     const int i = ij / nao, j = ij - i * nao;
-    const int64_t p = p0 + lp;
-    const int64_t base = (p * NQ + Q) * nao * nao;
-    z64 v = gauss_c(seed, (uint64_t)(base + ij), sigma);
-    if (diag[lp]) {
-      const z64 t = gauss_c(seed, (uint64_t)(base + j * nao + i), sigma);
-      v = z64{0.5 * (v.x + t.x), 0.5 * (v.y - t.y)};
-    }
-    V[e] = v;
+    const z64 t = gauss_c(seed, base + j * nao + i, sigma);
+    v = z64{0.5 * (v.x + t.x), 0.5 * (v.y - t.y)};
   }
+  V[lp * NQ * nn + e] = v;
 }
 
 inline int grid_for(int64_t n, int block) {
@@ -546,9 +548,16 @@ void trsm_upper_cols(const z64* A, z64* B, int n, int nw, int i0, int nb, int bc
 void synth_integrals(z64* V, int64_t p0, int64_t np, const int* pair_is_diag, int NQ, int nao, uint64_t seed,
                      double sigma, hipStream_t s) {
   if (np <= 0) return;
-  hipLaunchKernelGGL(synth_integrals_kernel, dim3(grid_for(np * NQ * (int64_t)nao * nao, 256)), dim3(256), 0, s,
-                     V, p0, np, pair_is_diag, NQ, nao, seed, sigma);
-  GGPU_HIP(hipGetLastError());
+  const int64_t per_pair = (int64_t)NQ * nao * nao;
+  require(per_pair < ((int64_t)1 << 31), "synth_integrals: NQ * nao^2 above 2^31");
+  const unsigned bx = (unsigned)((per_pair + 255) / 256);
+  // grid.y is limited to 65535: launch the pairs in chunks
+  for (int64_t c = 0; c < np; c += 65535) {
+    const int64_t n = std::min<int64_t>(65535, np - c);
+    hipLaunchKernelGGL(synth_integrals_kernel, dim3(bx, (unsigned)n), dim3(256), 0, s, V + c * per_pair, p0 + c,
+                       pair_is_diag + c, NQ, nao, seed, sigma);
+    GGPU_HIP(hipGetLastError());
+  }
 }
 
 }  // namespace ggpu
