@@ -86,14 +86,16 @@ def gw_flops(nk, nao, NQ, ns, nts, nw_b):
 def site_engine(mode, site, tune):
     """(matrix-core ops per algorithmic FLOP, peak TFLOP/s, engine) of one
     call site in one precision mode: the int8 modes run each site either on
-    the int8 emulation or on the FP64 kernel (autotuner's pick, else the
-    built-in choice)."""
+    the int8 emulation (the mode's slice count, or five slices where a
+    four-slice plan's autotuner picked them) or on the FP64 kernel
+    (autotuner's pick, else the built-in choice)."""
     if mode in INT8_MODES:
         chosen = (tune or {}).get("chosen", {}).get(site)
-        emulated = chosen == "int8" if chosen is not None else site in INT8_DEFAULT_SITES
+        emulated = chosen in ("int8", "int8x5") if chosen is not None else site in INT8_DEFAULT_SITES
         if not emulated:
             return MFMA_PER_ALG["fp64"], PEAK["f64"], "fp64 kernel"
-        return MFMA_PER_ALG[mode], PEAK[DTYPE[mode]], "int8 emulation"
+        # "int8x5": five slices in a four-slice plan
+        return MFMA_PER_ALG["i8x5" if chosen == "int8x5" else mode], PEAK[DTYPE[mode]], "int8 emulation"
     return MFMA_PER_ALG[mode], PEAK[DTYPE[mode]], mode
 
 

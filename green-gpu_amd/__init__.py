@@ -122,8 +122,9 @@ EXPORTED = ["ggpu_last_error", "ggpu_device_count", "ggpu_device_memory",
 
 # Launch variants of the contraction GEMMs (ids as the library numbers them):
 # FP64 / FP32 LDS k-depth and waves-per-SIMD register allocation, split-bf16
-# workgroup tiles.
-VARIANTS_MFMA = {8: "bk8", 16: "bk16", 83: "bk8_w3", 84: "bk8_w4", 200: "int8"}
+# workgroup tiles; in the int8 modes the emulation with the mode's slices
+# ("int8") or with five slices ("int8x5", offered to four-slice plans).
+VARIANTS_MFMA = {8: "bk8", 16: "bk16", 83: "bk8_w3", 84: "bk8_w4", 200: "int8", 201: "int8x5"}
 VARIANTS_BF16 = {0: "64x64", 1: "128x64", 4: "64x128", 5: "160x64", 6: "96x64_w3", 7: "64x96_w3", 8: "64x64_w3",
                  9: "64x64_astat"}
 
@@ -468,8 +469,9 @@ class HfKernel:
         _check(lib().ggpu_hf_set_k_share(self._h, int(rank), int(size)))
 
     def set_variants(self, variants):
-        """Launch variant per call site (only site "hf"; 200: int8 emulation,
-        83: FP64 kernel, -1: built-in) for the exchange products."""
+        """Launch variant per call site (only site "hf"; 200: int8 emulation
+        with the mode's slices, 201: with five slices, 83: FP64 kernel,
+        -1: built-in) for the exchange products."""
         v = (C.c_int * len(SITES))(*[int(x) for x in variants])
         _check(lib().ggpu_hf_set_variants(self._h, v))
 

@@ -1164,7 +1164,8 @@ void GwPlan::autotune(const z64* G_dev, int rounds, int batches) {
   int trial[kNumSites];
   // the int8 emulation is a candidate only where int8_site_tunable allows it
   auto tried = [&](int c, int s) {
-    return tunable_site(s) && (cand[c] != kVariantInt8 || int8_site_tunable(compute, s));
+    const bool i8 = cand[c] == kVariantInt8 || cand[c] == kVariantInt8x5;
+    return tunable_site(s) && (!i8 || int8_site_tunable(compute, s, cand[c]));
   };
   for (int r = 0; r < rounds; ++r)
     for (int c = 0; c < nc; ++c) {

@@ -1036,9 +1036,12 @@ int gemm_variant_candidates(Compute c, int* out, int cap) {
   static const int bf[] = {kBf16Tile128x64, kBf16Tile64x128, kBf16Tile160x64, kBf16Tile96x64W3,
                            kBf16Tile64x96W3, kBf16Tile64x64W3, kBf16Tile64x64, kBf16TileAstat64x64};
   if (int8_slices(c)) {                  // int8 emulation or the FP64 kernel
-    static const int i8[] = {kVariantInt8, 83};
-    for (int i = 0; i < 2 && i < cap; ++i) out[i] = i8[i];
-    return 2;
+    // four-slice plans may also take five slices where four are over budget
+    static const int i8x5[] = {kVariantInt8, 83}, i8x4[] = {kVariantInt8, kVariantInt8x5, 83};
+    const int* v = c == Compute::I8X4 ? i8x4 : i8x5;
+    const int n = c == Compute::I8X4 ? 3 : 2;
+    for (int i = 0; i < n && i < cap; ++i) out[i] = v[i];
+    return n;
   }
   const int* v = c == Compute::F64 ? f64 : c == Compute::F32 ? f32 : bf;
   const int n = c == Compute::F64 ? 3 : c == Compute::F32 ? 3 : 8;
@@ -1075,14 +1078,16 @@ void gemm(const GemmDesc& dd, hipStream_t stream) {
       break;
     case Compute::I8X5:
     case Compute::I8X4: {
-      const bool emulate = v == kVariantInt8 || (v == -1 && (d.site == kSiteSigY2 || d.site == kSiteP0));
+      const bool emulate =
+          v == kVariantInt8 || v == kVariantInt8x5 || (v == -1 && (d.site == kSiteSigY2 || d.site == kSiteP0));
       if (emulate) {
-        gemm_int8_emulated(d, int8_slices(d.compute), g_workspace, g_workspace_bytes, stream);
+        gemm_int8_emulated(d, v == kVariantInt8x5 ? 5 : int8_slices(d.compute), g_workspace, g_workspace_bytes,
+                           stream);
       } else {
         require(!d.out_f32, "gemm: FP64 kernel writes complex128");
         GemmDesc f = d;
         f.compute = Compute::F64;
-        launch_typed<double, double2>(f, stream, v == kVariantInt8 ? -1 : v);
+        launch_typed<double, double2>(f, stream, v);
       }
       break;
     }

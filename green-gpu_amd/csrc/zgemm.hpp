@@ -94,12 +94,14 @@ enum {
   // launches take the 64 x 64 three-wave tile)
   kBf16TileAstat64x64 = 9
 };
-// Variant of a Compute::I8X5 / I8X4 launch that runs the int8 emulation; the
-// FP64 kernel variants (8, 16, 83, 84) run the launch on the FP64 kernel
-// instead.  Built-in choice: the emulation at the Sigma P.Y1 product and the
-// P0 contraction (the sites where it is faster on the benchmark shapes), the
-// FP64 kernel everywhere else; per-site variants override it.
-enum { kVariantInt8 = 200 };
+// Variant of a Compute::I8X5 / I8X4 launch that runs the int8 emulation with
+// the mode's slice count; kVariantInt8x5 runs it with five slices in either
+// mode (FP64-accurate products, so any site of a four-slice plan may take
+// it); the FP64 kernel variants (8, 16, 83, 84) run the launch on the FP64
+// kernel instead.  Built-in choice: the emulation at the Sigma P.Y1 product
+// and the P0 contraction (the sites where it is faster on the benchmark
+// shapes), the FP64 kernel everywhere else; per-site variants override it.
+enum { kVariantInt8 = 200, kVariantInt8x5 = 201 };
 // Launch variant per call site for the launches of the calling thread
 // (table of kNumSites entries, -1: the built-in choice; nullptr: none):
 // for FP64 / FP32 8 or 16 forces the LDS k-depth, 83 or 84 k-depth 8 with
@@ -114,13 +116,15 @@ void set_gemm_workspace(void* ptr, size_t bytes);
 // returns their count, writes at most cap of them.
 int gemm_variant_candidates(Compute c, int* out, int cap);
 bool gemm_variant_valid(Compute c, int variant);
-// Whether the autotuner may pick the int8 emulation at this call site: any
-// site with five slices; with four slices only the P0 contraction and the
-// Sigma P.Y1 product (four-slice emulation at all six contraction sites
-// leaves ~1.1e-6 Ha in Sigma of the nk = 216, nao = 104, NQ = 600 benchmark
-// q, two sites ~3.5e-7).  An explicit per-site variant may still pick it.
-inline bool int8_site_tunable(Compute c, int site) {
-  return c == Compute::I8X5 || (c == Compute::I8X4 && (site == kSiteP0 || site == kSiteSigY2));
+// Whether the autotuner may pick the int8 emulation variant `variant`
+// (kVariantInt8 or kVariantInt8x5) at this call site: five slices anywhere;
+// four slices only at the P0 contraction and the Sigma P.Y1 product
+// (four-slice emulation at all six contraction sites leaves ~1.1e-6 Ha in
+// Sigma of the nk = 216, nao = 104, NQ = 600 benchmark q, two sites
+// ~3.5e-7).  An explicit per-site variant may still pick it.
+inline bool int8_site_tunable(Compute c, int site, int variant = kVariantInt8) {
+  if (c == Compute::I8X5 || variant == kVariantInt8x5) return true;
+  return c == Compute::I8X4 && (site == kSiteP0 || site == kSiteSigY2);
 }
 
 struct GemmDesc {

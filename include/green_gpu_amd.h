@@ -58,8 +58,10 @@ typedef enum {
                                  the reference                                                   */
   GGPU_PRECISION_INT8X4 = 4   /* as INT8X5 with four slices per element (slice diagonals up to 4,
                                  13 slice products instead of 19): ~2^-27 of the operand scale
-                                 per product, fewer int8 products and slice bytes.  No
-                                 counterpart in the reference                                    */
+                                 per product, fewer int8 products and slice bytes; the autotuner
+                                 offers four slices at the P0 contraction and the Sigma P.Y1
+                                 product only, five slices (variant 201) and the FP64 kernel at
+                                 every site.  No counterpart in the reference                    */
 } ggpu_precision;
 
 /* Where a plan keeps the stored integrals.  HOST: in pinned host memory,
@@ -161,7 +163,8 @@ int ggpu_hf_set_precision(ggpu_hf_plan* plan, int precision);
 /* Launch variant of the exchange products (11 entries indexed by call site
  * as for ggpu_gw_site_stats; only site 9, HF, may be set, -1 elsewhere): in
  * the GGPU_PRECISION_INT8X5 / INT8X4 modes 200 runs them on the int8
- * emulation and 83 (the built-in choice) on the FP64 kernel.  No counterpart
+ * emulation with the mode's slice count, 201 on the emulation with five
+ * slices, and 83 (the built-in choice) on the FP64 kernel.  No counterpart
  * in the reference. */
 int ggpu_hf_set_variants(ggpu_hf_plan* plan, const int* variants);
 /* This device's share of the IBZ k-points: exchange and solve then compute

@@ -42,6 +42,8 @@ def test_int8_modes_price_each_site_on_its_engine():
     assert b.site_engine("i8x5", "sig_py", tune) == (0.75, 78.6, "fp64 kernel")
     assert b.site_engine("i8x5", "sig_vg", tune) == (19.0, 5000.0, "int8 emulation")
     assert b.site_engine("fp64", "sig_py", tune) == (0.75, 78.6, "fp64")
+    # five slices picked in a four-slice plan are priced at 19 ops
+    assert b.site_engine("i8x4", "sig_vg", {"chosen": {"sig_vg": "int8x5"}}) == (19.0, 5000.0, "int8 emulation")
     r = b.mode_roofline("i8x4", {"sig_py": (2000.0, 2.0e14, 10), "p0_gv": (10.0, 1.0e12, 10)})
     assert r["engine"] == "int8 emulation" and abs(r["frac_mfma"] - 100.0 * 13 / 5000.0) < 1e-9
 

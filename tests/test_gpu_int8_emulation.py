@@ -175,9 +175,10 @@ def test_int8_autotune_chooses_per_site(ggpu, synth, oracle):
 
 
 def test_int8x4_autotune_candidates(ggpu, synth, oracle):
-    """With four slices the autotuner offers the int8 emulation only at the P0
-    contraction and the Sigma P.Y1 product (the error budget of the other
-    sites), and the tuned iteration stays within RTOL["i8x4"]."""
+    """With four slices the autotuner offers four-slice emulation only at the
+    P0 contraction and the Sigma P.Y1 product (the error budget of the other
+    sites), five-slice emulation (FP64-accurate) and the FP64 kernel at every
+    site, and the tuned iteration stays within RTOL["i8x4"]."""
     mesh, nao, NQ, ns = (2, 2, 1), 13, 40, 2
     sym, T_wt, T_tw, G, D, V = gw_system(ggpu, synth, mesh, nao, NQ, ns, 95)
     k = ggpu.GwKernel(nao, ns, NQ, T_wt, T_tw, sym, precision="i8x4", solver="cholesky")
@@ -185,7 +186,9 @@ def test_int8x4_autotune_candidates(ggpu, synth, oracle):
     rep = k.autotune(G, rounds=1)
     for name in CONTRACTION_SITES:
         cands = sorted(rep["sites"][name]["ms"])
-        assert cands == (["bk8_w3", "int8"] if name in ("p0_contract", "sig_py") else ["bk8_w3"]), (name, cands)
+        want = ["bk8_w3", "int8", "int8x5"] if name in ("p0_contract", "sig_py") else ["bk8_w3", "int8x5"]
+        assert cands == want, (name, cands)
+        assert rep["sites"][name]["chosen"] == min(rep["sites"][name]["ms"], key=rep["sites"][name]["ms"].get)
     Sig = k.solve(G)
     ref = oracle.gw_solve(G, V, oracle.symmetry_none(mesh, nao, NQ), T_wt, T_tw, solver="cholesky")
     assert max_abs(Sig, ref) < RTOL["i8x4"] * np.abs(ref).max()
