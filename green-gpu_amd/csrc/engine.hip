@@ -706,9 +706,10 @@ GwPlan::GwPlan(int nao_, int nso_, int ns_, int NQ_, int nts_, int nw_b_, const 
   kb = (int)std::max<int64_t>(1, std::min<int64_t>(budget / per_pair, std::max(sym.nk, sym.ink)));
   kb = std::min(kb, 4096);
   work.resize(((size_t)kb * per_pair + sizeof(z64) - 1) / sizeof(z64));
-  if (int8_slices(compute)) {
-    // int8 slices of the operands of one contraction launch (launches whose
-    // slices do not fit are split over their batches)
+  if (int8_slices(compute) || compute == Compute::BF16X3) {
+    // int8 slices (or pre-split bf16 planes) of the operands of one
+    // contraction launch (launches whose slices do not fit are split over
+    // their batches)
     size_t fr = 0, tot = 0;
     GGPU_HIP(hipMemGetInfo(&fr, &tot));
     ozwork.resize(std::min<size_t>((size_t)24 << 30, fr / 4));

@@ -76,7 +76,7 @@ int main(int argc, char** argv) {
   if (argc < 2) {
     std::fprintf(stderr,
                  "usage: %s <fp64|fp32|bf16x3|i8x5|i8x4> [--nao N] [--nq N] [--nts N] [--kb N] [--p0-split S] "
-                 "[--reps R] [--sites a,b] [--calibrate-copy]\n",
+                 "[--reps R] [--sites a,b] [--variant V] [--calibrate-copy]\n",
                  argv[0]);
     return 2;
   }
@@ -88,7 +88,7 @@ int main(int argc, char** argv) {
   const Compute comp = mode == "fp64" ? Compute::F64 : mode == "fp32" ? Compute::F32
                        : mode == "i8x5" ? Compute::I8X5
                        : mode == "i8x4" ? Compute::I8X4 : Compute::BF16X3;
-  int reps = 10, nao = 104, NQ = 600, nts = 110, kb = 1, S = 8;
+  int reps = 10, nao = 104, NQ = 600, nts = 110, kb = 1, S = 8, variant = -2;
   std::string only;
   bool calibrate = false;
   for (int i = 2; i < argc; ++i) {
@@ -101,6 +101,7 @@ int main(int argc, char** argv) {
     else if (a == "--kb") kb = next();
     else if (a == "--p0-split") S = next();
     else if (a == "--reps") reps = std::max(1, next());
+    else if (a == "--variant") variant = next();
     else if (a == "--sites" && i + 1 < argc) only = argv[++i];
     else {
       std::fprintf(stderr, "unknown argument %s\n", a.c_str());
@@ -139,19 +140,25 @@ int main(int argc, char** argv) {
       {"sig_contract", kSiteSigma, nts * nao, nao, nao, NQ, kb, ns, nao, 1, (int64_t)nts * n2, ns * ys, ys, 1, nao,
        n2, stride, 0, nao, 1, (int64_t)ns * nts * n2, (int64_t)nts * n2, 2, 0, -1.0, 1.0},
   };
-  // the int8 emulation: a workspace and the emulated variant at every site
+  // a workspace as the plans have one (int8 slices, pre-split bf16 planes)
+  // and the launch variant at every site: --variant, else the int8
+  // emulation in the int8 modes and the built-in choice otherwise
   void* work = nullptr;
   size_t work_bytes = 0;
   int variants[kNumSites];
-  std::fill(variants, variants + kNumSites, int8_slices(comp) ? (int)kVariantInt8 : -1);
-  if (int8_slices(comp)) {
+  if (variant != -2 && !gemm_variant_valid(comp, variant)) {
+    std::fprintf(stderr, "variant %d not valid in mode %s\n", variant, mode.c_str());
+    return 2;
+  }
+  std::fill(variants, variants + kNumSites, variant != -2 ? variant : int8_slices(comp) ? (int)kVariantInt8 : -1);
+  if (int8_slices(comp) || comp == Compute::BF16X3) {
     size_t fr = 0, tot = 0;
     check(hipMemGetInfo(&fr, &tot), "meminfo");
     work_bytes = std::min<size_t>((size_t)24 << 30, fr / 4);
     check(hipMalloc(&work, work_bytes), "malloc workspace");
     set_gemm_workspace(work, work_bytes);
-    set_site_variants(variants);
   }
+  set_site_variants(variants);
 
   std::printf("GEMM call sites, nao=%d NQ=%d ns=%d nts=%d, %d pairs per batch, %d P0 slices, mode %s, %d repetitions\n",
               nao, NQ, ns, nts, kb, S, mode.c_str(), reps);

@@ -800,6 +800,289 @@ inline bool astat_fits(const GemmDesc& d) {
          (int64_t)d.nb0 * d.nb1 >= 2;
 }
 
+thread_local GemmProfiler* g_profiler = nullptr;
+thread_local void* g_workspace = nullptr;
+thread_local size_t g_workspace_bytes = 0;
+thread_local const int* g_site_variants = nullptr;   // [kNumSites], -1: built-in choice
+
+// ---------------------------------------------------------------------------
+// Split-bf16 x3 from pre-split, fragment-native operands.  A split kernel
+// turns each complex64 operand into bf16 planes (re_hi, im_hi, re_lo, im_lo:
+// hi = bf16(x), lo = bf16(x - hi), as split2) stored as 1 KiB fragments, one
+// per (batch, k block of 32, 16-row block, plane): lane l holds row l % 16
+// and the 8 k values 8 (l / 16) .. + 7 -- the operand layout of
+// v_mfma_f32_16x16x32_bf16 -- so the GEMM moves every fragment HBM -> LDS
+// with one load-to-LDS wave instruction and LDS -> registers with one
+// conflict-free 16-byte read per lane, and splits nothing.  Each K-batch
+// entry's K is padded to a multiple of 32 on its own (the k blocks of the
+// other split-bf16 tiles), and every output sums its products in their
+// order, so all split-bf16 tiles give the same sums.
+// ---------------------------------------------------------------------------
+constexpr int FB_KB = 32;        // k per fragment / MFMA step
+constexpr int FB_TILE = 128;     // workgroup tile (both dimensions)
+constexpr int FB_FRAG = 1024;
+constexpr int FB_NP = 4;         // planes per element
+constexpr int FB_STAGE = 2 * (FB_TILE / 16) * FB_NP * FB_FRAG;   // both operands, one k block: 64 KiB
+
+struct FbView {
+  const float2* src = nullptr;   // element (row 0, k 0, kb 0, batch 0)
+  int64_t row_s = 0, k_s = 0, kb_s = 0, bs0 = 0, bs1 = 0;
+  const int64_t* off1 = nullptr;
+  int rows = 0, K = 0, KB = 1, nb1 = 1, conj = 0;
+  int Rp = 0, kpb = 0;           // padded rows; k blocks per K-batch entry
+  int8_t* out = nullptr;
+};
+
+// One wave per (16-row block, k block): lane l splits row l % 16, k values
+// 8 (l / 16) .. + 7 and writes its 16 bytes of each plane's fragment.
+__global__ __launch_bounds__(256) void fb_split_kernel(const FbView v, int64_t nkb) {
+  const int lane = threadIdx.x & 63;
+  const int64_t kblk = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (kblk >= nkb) return;
+  const int rb = blockIdx.y;
+  const int64_t b = blockIdx.z;
+  const int64_t b0 = b / v.nb1, b1 = b - b0 * v.nb1;
+  const float2* base = v.src + b0 * v.bs0 + (v.off1 ? v.off1[b1] : b1 * v.bs1);
+  const int row = rb * 16 + (lane & 15);
+  const int kb = (int)(kblk / v.kpb);
+  const int k0 = (int)(kblk - (int64_t)kb * v.kpb) * FB_KB + 8 * (lane >> 4);
+  const float s = v.conj ? -1.f : 1.f;
+  float2 x[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int k = k0 + j;
+    x[j] = (row < v.rows && k < v.K) ? base[kb * v.kb_s + row * v.row_s + k * v.k_s] : float2{0.f, 0.f};
+  }
+  unsigned hr[4], hi[4], lr_[4], li[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    split2(x[2 * q].x, x[2 * q + 1].x, hr[q], lr_[q]);
+    split2(s * x[2 * q].y, s * x[2 * q + 1].y, hi[q], li[q]);
+  }
+  int8_t* o = v.out + ((b * nkb + kblk) * (v.Rp / 16) + rb) * (int64_t)FB_NP * FB_FRAG + 16 * lane;
+  *reinterpret_cast<v4u*>(o + 0 * FB_FRAG) = v4u{hr[0], hr[1], hr[2], hr[3]};
+  *reinterpret_cast<v4u*>(o + 1 * FB_FRAG) = v4u{hi[0], hi[1], hi[2], hi[3]};
+  *reinterpret_cast<v4u*>(o + 2 * FB_FRAG) = v4u{lr_[0], lr_[1], lr_[2], lr_[3]};
+  *reinterpret_cast<v4u*>(o + 3 * FB_FRAG) = v4u{li[0], li[1], li[2], li[3]};
+}
+
+struct FbGemm {
+  const int8_t* Aq = nullptr;
+  const int8_t* Bq = nullptr;
+  int M = 0, N = 0, Mp = 0, Np = 0;
+  int64_t nkb = 0;
+  int nb0 = 1, nb1 = 1, a_shared = 0, b_shared = 0, batch_major = 0;
+  void* C = nullptr;
+  int64_t c_rs = 0, c_cs = 0, c_bs0 = 0, c_bs1 = 0;
+  const int64_t* c_off1 = nullptr;
+  double alpha_re = 1.0, alpha_im = 0.0, beta_re = 0.0, beta_im = 0.0;
+};
+
+// 128 x 128 tile per workgroup, four waves of 64 x 64 (4 x 4 blocks of the
+// 16 x 16 MFMA output, 128 FP32 accumulator registers).  A stage holds both
+// operands' fragments of one k block (64 KiB); two stages, one barrier per
+// k block; a wave keeps its 16 B fragments of the k block in registers and
+// walks its 4 row blocks, reading the next row block's A fragments ahead.
+template <class TO>
+__global__ __launch_bounds__(256) void gemm_bf16x3_frag_kernel(const FbGemm g) {
+  constexpr int STAGE_A = FB_STAGE / 2;
+  __shared__ __attribute__((aligned(1024))) int8_t lds[2 * FB_STAGE];
+  const int tiles_m = g.Mp / FB_TILE, tiles_n = g.Np / FB_TILE;
+  int64_t L = xcd_logical(blockIdx.x, gridDim.x);
+  int mt, nt;
+  int64_t b0, b1;
+  if (g.batch_major) {
+    const int64_t nb = (int64_t)g.nb0 * g.nb1, bb = L % nb;
+    L /= nb;
+    mt = (int)(L % tiles_m);
+    nt = (int)(L / tiles_m);
+    b1 = bb % g.nb1;
+    b0 = bb / g.nb1;
+  } else {
+    mt = (int)(L % tiles_m);
+    L /= tiles_m;
+    nt = (int)(L % tiles_n);
+    L /= tiles_n;
+    b1 = L % g.nb1;
+    b0 = L / g.nb1;
+  }
+  const int m0 = mt * FB_TILE, n0 = nt * FB_TILE;
+  const int64_t bA = g.a_shared ? 0 : b0 * g.nb1 + b1, bB = g.b_shared ? 0 : b0 * g.nb1 + b1;
+  const int64_t kstrideA = (int64_t)(g.Mp / 16) * FB_NP * FB_FRAG, kstrideB = (int64_t)(g.Np / 16) * FB_NP * FB_FRAG;
+  const int8_t* gA = g.Aq + bA * g.nkb * kstrideA + (int64_t)(m0 / 16) * FB_NP * FB_FRAG;
+  const int8_t* gB = g.Bq + bB * g.nkb * kstrideB + (int64_t)(n0 / 16) * FB_NP * FB_FRAG;
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+  const int wm = (w >> 1) * 64, wn = (w & 1) * 64;
+  const int lr = lane & 15, lk = lane >> 4;
+
+  // wave w copies fragments w, w + 4, .. of each operand's half of the stage
+  auto copy_stage = [&](int64_t kb, int st) {
+    const int8_t* srcA = gA + kb * kstrideA + 16 * lane;
+    const int8_t* srcB = gB + kb * kstrideB + 16 * lane;
+    int8_t* dst = lds + st * FB_STAGE;
+#pragma unroll
+    for (int it = 0; it < STAGE_A / FB_FRAG / 4; ++it) {
+      const int f = w + 4 * it;
+      __builtin_amdgcn_global_load_lds((const void*)(srcA + f * FB_FRAG),
+                                       (__attribute__((address_space(3))) void*)(dst + f * FB_FRAG), 16, 0, 0);
+    }
+#pragma unroll
+    for (int it = 0; it < STAGE_A / FB_FRAG / 4; ++it) {
+      const int f = w + 4 * it;
+      __builtin_amdgcn_global_load_lds((const void*)(srcB + f * FB_FRAG),
+                                       (__attribute__((address_space(3))) void*)(dst + STAGE_A + f * FB_FRAG), 16,
+                                       0, 0);
+    }
+  };
+  auto frag = [](const int8_t* p) { return *reinterpret_cast<const v4u*>(p); };
+
+  v4f cr[4][4], ci[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) { cr[i][j] = v4f{0, 0, 0, 0}; ci[i][j] = v4f{0, 0, 0, 0}; }
+  const v4u sign = v4u{0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
+
+  copy_stage(0, 0);
+  for (int64_t kb = 0; kb < g.nkb; ++kb) {
+    const int st = (int)(kb & 1);
+    // stage st has landed for every wave, and every wave is done with the
+    // stage the next copy overwrites
+    __syncthreads();
+    if (kb + 1 < g.nkb) copy_stage(kb + 1, st ^ 1);
+    const int8_t* sA = lds + st * FB_STAGE + (wm / 16) * FB_NP * FB_FRAG + 16 * lane;
+    const int8_t* sB = lds + st * FB_STAGE + STAGE_A + (wn / 16) * FB_NP * FB_FRAG + 16 * lane;
+    v8bf brh[4], bih[4], brl[4], bil[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      brh[j] = __builtin_bit_cast(v8bf, frag(sB + (j * FB_NP + 0) * FB_FRAG));
+      bih[j] = __builtin_bit_cast(v8bf, frag(sB + (j * FB_NP + 1) * FB_FRAG));
+      brl[j] = __builtin_bit_cast(v8bf, frag(sB + (j * FB_NP + 2) * FB_FRAG));
+      bil[j] = __builtin_bit_cast(v8bf, frag(sB + (j * FB_NP + 3) * FB_FRAG));
+    }
+    v4u a0 = frag(sA + 0 * FB_FRAG), a1 = frag(sA + 1 * FB_FRAG), a2 = frag(sA + 2 * FB_FRAG),
+        a3 = frag(sA + 3 * FB_FRAG);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      v4u n0_ = a0, n1_ = a1, n2_ = a2, n3_ = a3;
+      if (i + 1 < 4) {
+        n0_ = frag(sA + ((i + 1) * FB_NP + 0) * FB_FRAG);
+        n1_ = frag(sA + ((i + 1) * FB_NP + 1) * FB_FRAG);
+        n2_ = frag(sA + ((i + 1) * FB_NP + 2) * FB_FRAG);
+        n3_ = frag(sA + ((i + 1) * FB_NP + 3) * FB_FRAG);
+      }
+      const v8bf arh = __builtin_bit_cast(v8bf, a0), aih = __builtin_bit_cast(v8bf, a1);
+      const v8bf arl = __builtin_bit_cast(v8bf, a2), ail = __builtin_bit_cast(v8bf, a3);
+      const v8bf naih = __builtin_bit_cast(v8bf, a1 ^ sign), nail = __builtin_bit_cast(v8bf, a3 ^ sign);
+      // per accumulator the order of gemm_bf16x3_big_kernel
+#define GGPU_FB_STEP(acc, a, b)                                                        \
+  _Pragma("unroll") for (int j = 0; j < 4; ++j) acc[i][j] =                            \
+      __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j], acc[i][j], 0, 0, 0);
+      GGPU_FB_STEP(cr, arh, brh) GGPU_FB_STEP(ci, arh, bih)
+      GGPU_FB_STEP(cr, arh, brl) GGPU_FB_STEP(ci, arh, bil)
+      GGPU_FB_STEP(cr, arl, brh) GGPU_FB_STEP(ci, arl, bih)
+      GGPU_FB_STEP(cr, naih, bih) GGPU_FB_STEP(ci, aih, brh)
+      GGPU_FB_STEP(cr, naih, bil) GGPU_FB_STEP(ci, aih, brl)
+      GGPU_FB_STEP(cr, nail, bih) GGPU_FB_STEP(ci, ail, brh)
+#undef GGPU_FB_STEP
+      a0 = n0_; a1 = n1_; a2 = n2_; a3 = n3_;
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+
+  const bool use_beta = (g.beta_re != 0.0 || g.beta_im != 0.0);
+  TO* C = static_cast<TO*>(g.C) + b0 * g.c_bs0 + (g.c_off1 ? g.c_off1[b1] : b1 * g.c_bs1);
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm + 16 * i + 4 * lk + r;
+        const int col = n0 + wn + 16 * j + lr;
+        if (row < g.M && col < g.N) {
+          const double xr = cr[i][j][r], xi = ci[i][j][r];
+          double vr = g.alpha_re * xr - g.alpha_im * xi;
+          double vi = g.alpha_re * xi + g.alpha_im * xr;
+          TO* p = C + row * g.c_rs + col * g.c_cs;
+          if (use_beta) {
+            const TO o = *p;
+            vr += g.beta_re * o.x - g.beta_im * o.y;
+            vi += g.beta_re * o.y + g.beta_im * o.x;
+          }
+          using TS = decltype(TO{}.x);
+          *p = TO{static_cast<TS>(vr), static_cast<TS>(vi)};
+        }
+      }
+}
+
+inline int64_t fb_round(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+// Bytes of split planes one launch over nb0 batches needs.
+inline size_t fb_bytes(const GemmDesc& d, int nb0, bool a_shared, bool b_shared) {
+  const int64_t kpb = (d.K + FB_KB - 1) / FB_KB, nkb = kpb * d.KB;
+  const int64_t nbA = a_shared ? 1 : (int64_t)nb0 * d.nb1, nbB = b_shared ? 1 : (int64_t)nb0 * d.nb1;
+  const int64_t per_row_block = nkb * FB_NP * FB_FRAG;
+  return (size_t)(nbA * (fb_round(d.M, FB_TILE) / 16) + nbB * (fb_round(d.N, FB_TILE) / 16)) * per_row_block;
+}
+
+// Whether the pre-split tile can run this launch with the workspace it has
+// (one batch of its planes must fit).
+inline bool fb_fits(const GemmDesc& d) {
+  const bool a_shared = d.A.bs0 == 0 && d.A.bs1 == 0 && d.A.off1 == nullptr;
+  const bool b_shared = d.B.bs0 == 0 && d.B.bs1 == 0 && d.B.off1 == nullptr;
+  return g_workspace != nullptr && fb_bytes(d, 1, a_shared, b_shared) <= g_workspace_bytes;
+}
+
+template <class TO>
+void launch_bf16x3_frag(const GemmDesc& d, hipStream_t stream) {
+  const bool a_shared = d.A.bs0 == 0 && d.A.bs1 == 0 && d.A.off1 == nullptr;
+  const bool b_shared = d.B.bs0 == 0 && d.B.bs1 == 0 && d.B.off1 == nullptr;
+  const int kpb = (d.K + FB_KB - 1) / FB_KB;
+  const int64_t nkb = (int64_t)kpb * d.KB;
+  const int Mp = (int)fb_round(d.M, FB_TILE), Np = (int)fb_round(d.N, FB_TILE);
+  int nb0c = d.nb0;   // batches per part: their planes fit the workspace
+  while (nb0c > 1 && fb_bytes(d, nb0c, a_shared, b_shared) > g_workspace_bytes) nb0c = (nb0c + 1) / 2;
+  require(fb_bytes(d, nb0c, a_shared, b_shared) <= g_workspace_bytes, "gemm: split-bf16 workspace too small");
+  require(Mp / 16 <= 65535 && Np / 16 <= 65535, "gemm: split-bf16 operand too large");
+  const size_t elem = d.out_f32 ? sizeof(float2) : sizeof(double2);
+  for (int b0s = 0; b0s < d.nb0; b0s += nb0c) {
+    const int nb = std::min(nb0c, d.nb0 - b0s);
+    const int64_t nbA = a_shared ? 1 : (int64_t)nb * d.nb1, nbB = b_shared ? 1 : (int64_t)nb * d.nb1;
+    int8_t* Aq = static_cast<int8_t*>(g_workspace);
+    int8_t* Bq = Aq + (size_t)nbA * (Mp / 16) * nkb * FB_NP * FB_FRAG;
+    for (int op = 0; op < 2; ++op) {
+      const Operand& o = op == 0 ? d.A : d.B;
+      FbView v;
+      v.src = static_cast<const float2*>(o.ptr) + (int64_t)b0s * o.bs0;
+      // A(m, k) = A[m rs + k cs]; B(k, n) = B[k rs + n cs]: rows are m / n
+      v.row_s = op == 0 ? o.rs : o.cs;
+      v.k_s = op == 0 ? o.cs : o.rs;
+      v.kb_s = o.kbs; v.bs0 = o.bs0; v.bs1 = o.bs1; v.off1 = o.off1;
+      v.rows = op == 0 ? d.M : d.N; v.K = d.K; v.KB = d.KB;
+      v.nb1 = (op == 0 ? a_shared : b_shared) ? 1 : d.nb1;
+      v.conj = o.conj; v.Rp = op == 0 ? Mp : Np; v.kpb = kpb;
+      v.out = op == 0 ? Aq : Bq;
+      const int64_t nbat = op == 0 ? nbA : nbB;
+      require(nbat <= 65535 && (nkb + 3) / 4 < ((int64_t)1 << 31), "gemm: split-bf16 operand too large");
+      hipLaunchKernelGGL(fb_split_kernel, dim3((unsigned)((nkb + 3) / 4), (unsigned)(v.Rp / 16), (unsigned)nbat),
+                         dim3(256), 0, stream, v, nkb);
+      GGPU_HIP(hipGetLastError());
+    }
+    FbGemm g;
+    g.Aq = Aq; g.Bq = Bq; g.M = d.M; g.N = d.N; g.Mp = Mp; g.Np = Np; g.nkb = nkb;
+    g.nb0 = nb; g.nb1 = d.nb1; g.a_shared = a_shared; g.b_shared = b_shared;
+    g.batch_major = (nb * (int64_t)d.nb1 > 1) && a_shared;
+    g.C = static_cast<char*>(d.C.ptr) + (int64_t)b0s * d.C.bs0 * (int64_t)elem;
+    g.c_rs = d.C.rs; g.c_cs = d.C.cs; g.c_bs0 = d.C.bs0; g.c_bs1 = d.C.bs1; g.c_off1 = d.C.off1;
+    g.alpha_re = d.alpha_re; g.alpha_im = d.alpha_im; g.beta_re = d.beta_re; g.beta_im = d.beta_im;
+    const int64_t tiles = (int64_t)(Mp / FB_TILE) * (Np / FB_TILE) * nb * d.nb1;
+    require(tiles < ((int64_t)1 << 31), "gemm: grid too large");
+    hipLaunchKernelGGL((gemm_bf16x3_frag_kernel<TO>), dim3((unsigned)tiles), dim3(256), 0, stream, g);
+    GGPU_HIP(hipGetLastError());
+  }
+}
+
 template <class TO, int SITE>
 void launch_bf16x3_astat(const GemmDesc& d, hipStream_t stream) {
   constexpr int TM = 64, TN = 64;
@@ -887,6 +1170,9 @@ void launch_bf16x3_site(const GemmDesc& d, hipStream_t stream, int variant) {
     case kBf16TileAstat64x64:
       if (astat_fits(d)) return launch_bf16x3_astat<TO, SITE>(d, stream);
       return launch_bf16x3_big<TO, 2, 2, SITE, 3>(d, stream);   // a 64 x 64 tile, A staged per batch
+    case kBf16TileFrag128x128:
+      if (fb_fits(d)) return launch_bf16x3_frag<TO>(d, stream);
+      return launch_bf16x3_big<TO, 5, 2, SITE>(d, stream);   // no workspace: 160 x 64
     default: break;
   }
   const int64_t tiles = (int64_t)((d.M + BM - 1) / BM) * ((d.N + BN - 1) / BN);
@@ -965,10 +1251,6 @@ void launch_typed(const GemmDesc& d, hipStream_t stream, int v) {
   }
 }
 
-thread_local GemmProfiler* g_profiler = nullptr;
-thread_local void* g_workspace = nullptr;
-thread_local size_t g_workspace_bytes = 0;
-thread_local const int* g_site_variants = nullptr;   // [kNumSites], -1: built-in choice
 
 // Range of element offsets (relative to ptr) touched by a 2-D view with the
 // given extents and strides, over all batch indices.
@@ -1034,7 +1316,8 @@ int gemm_variant_candidates(Compute c, int* out, int cap) {
   static const int f64[] = {83, 8, 16};
   static const int f32[] = {8, 16, 83};
   static const int bf[] = {kBf16Tile128x64, kBf16Tile64x128, kBf16Tile160x64, kBf16Tile96x64W3,
-                           kBf16Tile64x96W3, kBf16Tile64x64W3, kBf16Tile64x64, kBf16TileAstat64x64};
+                           kBf16Tile64x96W3, kBf16Tile64x64W3, kBf16Tile64x64, kBf16TileAstat64x64,
+                           kBf16TileFrag128x128};
   if (int8_slices(c)) {                  // int8 emulation or the FP64 kernel
     // four-slice plans may also take five slices where four are over budget
     static const int i8x5[] = {kVariantInt8, 83}, i8x4[] = {kVariantInt8, kVariantInt8x5, 83};
@@ -1044,15 +1327,15 @@ int gemm_variant_candidates(Compute c, int* out, int cap) {
     return n;
   }
   const int* v = c == Compute::F64 ? f64 : c == Compute::F32 ? f32 : bf;
-  const int n = c == Compute::F64 ? 3 : c == Compute::F32 ? 3 : 8;
+  const int n = c == Compute::F64 ? 3 : c == Compute::F32 ? 3 : 9;
   for (int i = 0; i < n && i < cap; ++i) out[i] = v[i];
   return n;
 }
 
 bool gemm_variant_valid(Compute c, int v) {
   if (v == -1) return true;
-  int cand[8];
-  const int n = gemm_variant_candidates(c, cand, 8);
+  int cand[16];
+  const int n = gemm_variant_candidates(c, cand, 16);
   return std::find(cand, cand + n, v) != cand + n;
 }
 

@@ -92,7 +92,10 @@ enum {
   // 64 x 64, A split once and kept in registers while the workgroup walks
   // several batches (one A for every batch, K <= 64, no K-batch; other
   // launches take the 64 x 64 three-wave tile)
-  kBf16TileAstat64x64 = 9
+  kBf16TileAstat64x64 = 9,
+  // 128 x 128 from operands split once into fragment-native bf16 planes in
+  // the plan's workspace (no workspace: the 160 x 64 tile)
+  kBf16TileFrag128x128 = 10
 };
 // Variant of a Compute::I8X5 / I8X4 launch that runs the int8 emulation with
 // the mode's slice count; kVariantInt8x5 runs it with five slices in either
