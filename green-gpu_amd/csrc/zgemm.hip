@@ -878,13 +878,15 @@ struct FbGemm {
   double alpha_re = 1.0, alpha_im = 0.0, beta_re = 0.0, beta_im = 0.0;
 };
 
-// 128 x 128 tile per workgroup, four waves of 64 x 64 (4 x 4 blocks of the
-// 16 x 16 MFMA output, 128 FP32 accumulator registers).  A stage holds both
+// 128 x 128 tile per workgroup of WMW x WNW waves, each wave RI x RJ blocks
+// of the 16 x 16 MFMA output: 2 x 2 waves of 64 x 64 (one wave per SIMD),
+// 4 x 2 of 32 x 64 (two), 4 x 4 of 32 x 32 (four).  A stage holds both
 // operands' fragments of one k block (64 KiB); two stages, one barrier per
-// k block; a wave keeps its 16 B fragments of the k block in registers and
-// walks its 4 row blocks, reading the next row block's A fragments ahead.
-template <class TO>
-__global__ __launch_bounds__(256) void gemm_bf16x3_frag_kernel(const FbGemm g) {
+// k block; a wave keeps its B fragments of the k block in registers and
+// walks its row blocks, reading the next row block's A fragments ahead.
+template <class TO, int WMW, int WNW>
+__global__ __launch_bounds__(64 * WMW * WNW) void gemm_bf16x3_frag_kernel(const FbGemm g) {
+  constexpr int NW = WMW * WNW, RI = 8 / WMW, RJ = 8 / WNW;
   constexpr int STAGE_A = FB_STAGE / 2;
   __shared__ __attribute__((aligned(1024))) int8_t lds[2 * FB_STAGE];
   const int tiles_m = g.Mp / FB_TILE, tiles_n = g.Np / FB_TILE;
@@ -912,23 +914,23 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_frag_kernel(const FbGemm g) {
   const int8_t* gA = g.Aq + bA * g.nkb * kstrideA + (int64_t)(m0 / 16) * FB_NP * FB_FRAG;
   const int8_t* gB = g.Bq + bB * g.nkb * kstrideB + (int64_t)(n0 / 16) * FB_NP * FB_FRAG;
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
-  const int wm = (w >> 1) * 64, wn = (w & 1) * 64;
+  const int wm = (w / WNW) * 16 * RI, wn = (w % WNW) * 16 * RJ;
   const int lr = lane & 15, lk = lane >> 4;
 
-  // wave w copies fragments w, w + 4, .. of each operand's half of the stage
+  // wave w copies fragments w, w + NW, .. of each operand's half of the stage
   auto copy_stage = [&](int64_t kb, int st) {
     const int8_t* srcA = gA + kb * kstrideA + 16 * lane;
     const int8_t* srcB = gB + kb * kstrideB + 16 * lane;
     int8_t* dst = lds + st * FB_STAGE;
 #pragma unroll
-    for (int it = 0; it < STAGE_A / FB_FRAG / 4; ++it) {
-      const int f = w + 4 * it;
+    for (int it = 0; it < STAGE_A / FB_FRAG / NW; ++it) {
+      const int f = w + NW * it;
       __builtin_amdgcn_global_load_lds((const void*)(srcA + f * FB_FRAG),
                                        (__attribute__((address_space(3))) void*)(dst + f * FB_FRAG), 16, 0, 0);
     }
 #pragma unroll
-    for (int it = 0; it < STAGE_A / FB_FRAG / 4; ++it) {
-      const int f = w + 4 * it;
+    for (int it = 0; it < STAGE_A / FB_FRAG / NW; ++it) {
+      const int f = w + NW * it;
       __builtin_amdgcn_global_load_lds((const void*)(srcB + f * FB_FRAG),
                                        (__attribute__((address_space(3))) void*)(dst + STAGE_A + f * FB_FRAG), 16,
                                        0, 0);
@@ -936,11 +938,11 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_frag_kernel(const FbGemm g) {
   };
   auto frag = [](const int8_t* p) { return *reinterpret_cast<const v4u*>(p); };
 
-  v4f cr[4][4], ci[4][4];
+  v4f cr[RI][RJ], ci[RI][RJ];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < RI; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) { cr[i][j] = v4f{0, 0, 0, 0}; ci[i][j] = v4f{0, 0, 0, 0}; }
+    for (int j = 0; j < RJ; ++j) { cr[i][j] = v4f{0, 0, 0, 0}; ci[i][j] = v4f{0, 0, 0, 0}; }
   const v4u sign = v4u{0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
 
   copy_stage(0, 0);
@@ -952,9 +954,9 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_frag_kernel(const FbGemm g) {
     if (kb + 1 < g.nkb) copy_stage(kb + 1, st ^ 1);
     const int8_t* sA = lds + st * FB_STAGE + (wm / 16) * FB_NP * FB_FRAG + 16 * lane;
     const int8_t* sB = lds + st * FB_STAGE + STAGE_A + (wn / 16) * FB_NP * FB_FRAG + 16 * lane;
-    v8bf brh[4], bih[4], brl[4], bil[4];
+    v8bf brh[RJ], bih[RJ], brl[RJ], bil[RJ];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < RJ; ++j) {
       brh[j] = __builtin_bit_cast(v8bf, frag(sB + (j * FB_NP + 0) * FB_FRAG));
       bih[j] = __builtin_bit_cast(v8bf, frag(sB + (j * FB_NP + 1) * FB_FRAG));
       brl[j] = __builtin_bit_cast(v8bf, frag(sB + (j * FB_NP + 2) * FB_FRAG));
@@ -963,9 +965,9 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_frag_kernel(const FbGemm g) {
     v4u a0 = frag(sA + 0 * FB_FRAG), a1 = frag(sA + 1 * FB_FRAG), a2 = frag(sA + 2 * FB_FRAG),
         a3 = frag(sA + 3 * FB_FRAG);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < RI; ++i) {
       v4u n0_ = a0, n1_ = a1, n2_ = a2, n3_ = a3;
-      if (i + 1 < 4) {
+      if (i + 1 < RI) {
         n0_ = frag(sA + ((i + 1) * FB_NP + 0) * FB_FRAG);
         n1_ = frag(sA + ((i + 1) * FB_NP + 1) * FB_FRAG);
         n2_ = frag(sA + ((i + 1) * FB_NP + 2) * FB_FRAG);
@@ -976,7 +978,7 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_frag_kernel(const FbGemm g) {
       const v8bf naih = __builtin_bit_cast(v8bf, a1 ^ sign), nail = __builtin_bit_cast(v8bf, a3 ^ sign);
       // per accumulator the order of gemm_bf16x3_big_kernel
 #define GGPU_FB_STEP(acc, a, b)                                                        \
-  _Pragma("unroll") for (int j = 0; j < 4; ++j) acc[i][j] =                            \
+  _Pragma("unroll") for (int j = 0; j < RJ; ++j) acc[i][j] =                           \
       __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j], acc[i][j], 0, 0, 0);
       GGPU_FB_STEP(cr, arh, brh) GGPU_FB_STEP(ci, arh, bih)
       GGPU_FB_STEP(cr, arh, brl) GGPU_FB_STEP(ci, arh, bil)
@@ -993,9 +995,9 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_frag_kernel(const FbGemm g) {
   const bool use_beta = (g.beta_re != 0.0 || g.beta_im != 0.0);
   TO* C = static_cast<TO*>(g.C) + b0 * g.c_bs0 + (g.c_off1 ? g.c_off1[b1] : b1 * g.c_bs1);
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < RI; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < RJ; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = m0 + wm + 16 * i + 4 * lk + r;
@@ -1034,7 +1036,7 @@ inline bool fb_fits(const GemmDesc& d) {
   return g_workspace != nullptr && fb_bytes(d, 1, a_shared, b_shared) <= g_workspace_bytes;
 }
 
-template <class TO>
+template <class TO, int WMW, int WNW>
 void launch_bf16x3_frag(const GemmDesc& d, hipStream_t stream) {
   const bool a_shared = d.A.bs0 == 0 && d.A.bs1 == 0 && d.A.off1 == nullptr;
   const bool b_shared = d.B.bs0 == 0 && d.B.bs1 == 0 && d.B.off1 == nullptr;
@@ -1078,7 +1080,8 @@ void launch_bf16x3_frag(const GemmDesc& d, hipStream_t stream) {
     g.alpha_re = d.alpha_re; g.alpha_im = d.alpha_im; g.beta_re = d.beta_re; g.beta_im = d.beta_im;
     const int64_t tiles = (int64_t)(Mp / FB_TILE) * (Np / FB_TILE) * nb * d.nb1;
     require(tiles < ((int64_t)1 << 31), "gemm: grid too large");
-    hipLaunchKernelGGL((gemm_bf16x3_frag_kernel<TO>), dim3((unsigned)tiles), dim3(256), 0, stream, g);
+    hipLaunchKernelGGL((gemm_bf16x3_frag_kernel<TO, WMW, WNW>), dim3((unsigned)tiles), dim3(64 * WMW * WNW), 0,
+                       stream, g);
     GGPU_HIP(hipGetLastError());
   }
 }
@@ -1171,8 +1174,14 @@ void launch_bf16x3_site(const GemmDesc& d, hipStream_t stream, int variant) {
       if (astat_fits(d)) return launch_bf16x3_astat<TO, SITE>(d, stream);
       return launch_bf16x3_big<TO, 2, 2, SITE, 3>(d, stream);   // a 64 x 64 tile, A staged per batch
     case kBf16TileFrag128x128:
-      if (fb_fits(d)) return launch_bf16x3_frag<TO>(d, stream);
+      if (fb_fits(d)) return launch_bf16x3_frag<TO, 2, 2>(d, stream);
       return launch_bf16x3_big<TO, 5, 2, SITE>(d, stream);   // no workspace: 160 x 64
+    case kBf16TileFrag128x128W8:
+      if (fb_fits(d)) return launch_bf16x3_frag<TO, 4, 2>(d, stream);
+      return launch_bf16x3_big<TO, 5, 2, SITE>(d, stream);
+    case kBf16TileFrag128x128W16:
+      if (fb_fits(d)) return launch_bf16x3_frag<TO, 4, 4>(d, stream);
+      return launch_bf16x3_big<TO, 5, 2, SITE>(d, stream);
     default: break;
   }
   const int64_t tiles = (int64_t)((d.M + BM - 1) / BM) * ((d.N + BN - 1) / BN);
@@ -1317,7 +1326,7 @@ int gemm_variant_candidates(Compute c, int* out, int cap) {
   static const int f32[] = {8, 16, 83};
   static const int bf[] = {kBf16Tile128x64, kBf16Tile64x128, kBf16Tile160x64, kBf16Tile96x64W3,
                            kBf16Tile64x96W3, kBf16Tile64x64W3, kBf16Tile64x64, kBf16TileAstat64x64,
-                           kBf16TileFrag128x128};
+                           kBf16TileFrag128x128, kBf16TileFrag128x128W8, kBf16TileFrag128x128W16};
   if (int8_slices(c)) {                  // int8 emulation or the FP64 kernel
     // four-slice plans may also take five slices where four are over budget
     static const int i8x5[] = {kVariantInt8, 83}, i8x4[] = {kVariantInt8, kVariantInt8x5, 83};
@@ -1327,7 +1336,7 @@ int gemm_variant_candidates(Compute c, int* out, int cap) {
     return n;
   }
   const int* v = c == Compute::F64 ? f64 : c == Compute::F32 ? f32 : bf;
-  const int n = c == Compute::F64 ? 3 : c == Compute::F32 ? 3 : 9;
+  const int n = c == Compute::F64 ? 3 : c == Compute::F32 ? 3 : 11;
   for (int i = 0; i < n && i < cap; ++i) out[i] = v[i];
   return n;
 }

@@ -95,7 +95,9 @@ enum {
   kBf16TileAstat64x64 = 9,
   // 128 x 128 from operands split once into fragment-native bf16 planes in
   // the plan's workspace (no workspace: the 160 x 64 tile)
-  kBf16TileFrag128x128 = 10
+  kBf16TileFrag128x128 = 10,
+  kBf16TileFrag128x128W8 = 11,   // the same with eight waves (two per SIMD)
+  kBf16TileFrag128x128W16 = 12   // and with sixteen (four per SIMD)
 };
 // Variant of a Compute::I8X5 / I8X4 launch that runs the int8 emulation with
 // the mode's slice count; kVariantInt8x5 runs it with five slices in either
