@@ -335,10 +335,12 @@ __device__ inline v4i neg_bytes(v4i x) {
 
 __device__ inline v4i lds_frag(const int8_t* p) { return *reinterpret_cast<const v4i*>(p); }
 
-// One 64 x 64 output tile per workgroup, four waves of 32 x 32 (2 x 2
-// blocks of the 16 x 16 MFMA output).  Both operands' fragments of a k block
-// (80 KiB) are copied HBM -> LDS by the load-to-LDS path into one of two
-// stages while the other stage is multiplied, one barrier per k block.
+// One 64 x 64 output tile per workgroup, 2 x WNW waves of 32 x (64 / WNW)
+// (2 x 4 / WNW blocks of the 16 x 16 MFMA output): four waves of 32 x 32,
+// one per SIMD, or eight of 32 x 16, two per SIMD.  Both operands'
+// fragments of a k block (80 KiB) are copied HBM -> LDS by the load-to-LDS
+// path into one of two stages while the other stage is multiplied, one
+// barrier per k block.
 // Within a k block every wave walks its A slices from the last to the first,
 // so the B fragments it needs grow by one slice per step and are read from
 // LDS one step ahead of their first use.
@@ -350,9 +352,10 @@ __device__ inline v4i lds_frag(const int8_t* p) { return *reinterpret_cast<const
 // in its L2 while the group sweeps the columns) and ~32 / GM column panels of B.
 constexpr int GM = 4;
 
-template <class TO, int NSL>
-__global__ __launch_bounds__(NT) void oz_gemm_kernel(const OzGemm g) {
+template <class TO, int NSL, int WNW>
+__global__ __launch_bounds__(128 * WNW) void oz_gemm_kernel(const OzGemm g) {
   constexpr int DMAX = NSL, NP = 2 * NSL, STAGE = stage_bytes<NSL>(), STAGE_A = STAGE / 2;
+  constexpr int NW = 2 * WNW, RJ = 4 / WNW;   // waves; column blocks per wave (2 row blocks each)
   __shared__ __attribute__((aligned(1024))) int8_t lds[2 * STAGE];
   const int tiles_m = (g.M + TILE - 1) / TILE, tiles_n = (g.N + TILE - 1) / TILE;
   int64_t L = oz_xcd_logical(blockIdx.x, gridDim.x);
@@ -385,7 +388,7 @@ __global__ __launch_bounds__(NT) void oz_gemm_kernel(const OzGemm g) {
   const int8_t* gA = g.Aq + bA * g.nkb * kstrideA + (int64_t)(m0 / RB) * NP * FRAG;
   const int8_t* gB = g.Bq + bB * g.nkb * kstrideB + (int64_t)(n0 / RB) * NP * FRAG;
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
-  const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+  const int wm = (w / WNW) * 32, wn = (w % WNW) * 16 * RJ;
   const int lr = lane & 15, lk = lane >> 4;
 
   // wave w copies fragments w, w + 4, .. of the stage: NSL of A, then NSL of B
@@ -394,14 +397,14 @@ __global__ __launch_bounds__(NT) void oz_gemm_kernel(const OzGemm g) {
     const int8_t* srcB = gB + (int64_t)kb * kstrideB + 16 * lane;
     int8_t* dst = lds + st * STAGE;
 #pragma unroll
-    for (int it = 0; it < STAGE_A / FRAG / 4; ++it) {
-      const int f = w + 4 * it;
+    for (int it = 0; it < STAGE_A / FRAG / NW; ++it) {
+      const int f = w + NW * it;
       __builtin_amdgcn_global_load_lds((const void*)(srcA + f * FRAG),
                                        (__attribute__((address_space(3))) void*)(dst + f * FRAG), 16, 0, 0);
     }
 #pragma unroll
-    for (int it = 0; it < STAGE_A / FRAG / 4; ++it) {
-      const int f = w + 4 * it;
+    for (int it = 0; it < STAGE_A / FRAG / NW; ++it) {
+      const int f = w + NW * it;
       __builtin_amdgcn_global_load_lds((const void*)(srcB + f * FRAG),
                                        (__attribute__((address_space(3))) void*)(dst + STAGE_A + f * FRAG), 16, 0,
                                        0);
@@ -410,7 +413,7 @@ __global__ __launch_bounds__(NT) void oz_gemm_kernel(const OzGemm g) {
 
   // exponents of the rows / columns this lane stores (fetched with the first
   // k block)
-  int ea[2][4], eb[2];
+  int ea[2][4], eb[RJ];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -419,16 +422,16 @@ __global__ __launch_bounds__(NT) void oz_gemm_kernel(const OzGemm g) {
       ea[i][r] = row < g.M ? g.EA[bA * g.Mp + row] : INT_MIN;
     }
 #pragma unroll
-  for (int j = 0; j < 2; ++j) {
+  for (int j = 0; j < RJ; ++j) {
     const int col = n0 + wn + 16 * j + lr;
     eb[j] = col < g.N ? g.EB[bB * g.Np + col] : INT_MIN;
   }
 
-  v4i re[2][2][DMAX + 1], im[2][2][DMAX + 1];
+  v4i re[2][RJ][DMAX + 1], im[2][RJ][DMAX + 1];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < RJ; ++j)
 #pragma unroll
       for (int d = 0; d <= DMAX; ++d) { re[i][j][d] = v4i{0, 0, 0, 0}; im[i][j][d] = v4i{0, 0, 0, 0}; }
 
@@ -442,11 +445,11 @@ __global__ __launch_bounds__(NT) void oz_gemm_kernel(const OzGemm g) {
     const int8_t* sA = lds + st * STAGE + (wm / RB) * NP * FRAG + 16 * lane;
     const int8_t* sB = lds + st * STAGE + STAGE_A + (wn / RB) * NP * FRAG + 16 * lane;
     // B fragments of slice u: bf[j][2u] (re), bf[j][2u + 1] (im), block column j
-    v4i bf[2][NP];
+    v4i bf[RJ][NP];
 #pragma unroll
     for (int u = 0; u < NSL && u <= DMAX - (NSL - 1); ++u)
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
+      for (int j = 0; j < RJ; ++j) {
         bf[j][2 * u] = lds_frag(sB + (j * NP + 2 * u) * FRAG);
         bf[j][2 * u + 1] = lds_frag(sB + (j * NP + 2 * u + 1) * FRAG);
       }
@@ -464,7 +467,7 @@ __global__ __launch_bounds__(NT) void oz_gemm_kernel(const OzGemm g) {
       if (i == 0 && s >= 1 && DMAX - s + 1 < NSL) {
         const int u = DMAX - s + 1;
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < RJ; ++j) {
           bf[j][2 * u] = lds_frag(sB + (j * NP + 2 * u) * FRAG);
           bf[j][2 * u + 1] = lds_frag(sB + (j * NP + 2 * u + 1) * FRAG);
         }
@@ -475,7 +478,7 @@ __global__ __launch_bounds__(NT) void oz_gemm_kernel(const OzGemm g) {
       for (int u = 0; u < NSL && u + s <= DMAX; ++u) {
         const int d = s + u;
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < RJ; ++j) {
           re[i][j][d] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ar, bf[j][2 * u], re[i][j][d], 0, 0, 0);
           im[i][j][d] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ar, bf[j][2 * u + 1], im[i][j][d], 0, 0, 0);
         }
@@ -484,7 +487,7 @@ __global__ __launch_bounds__(NT) void oz_gemm_kernel(const OzGemm g) {
       for (int u = 0; u < NSL && u + s <= DMAX; ++u) {
         const int d = s + u;
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < RJ; ++j) {
           re[i][j][d] = __builtin_amdgcn_mfma_i32_16x16x64_i8(nai, bf[j][2 * u + 1], re[i][j][d], 0, 0, 0);
           im[i][j][d] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ai, bf[j][2 * u], im[i][j][d], 0, 0, 0);
         }
@@ -499,7 +502,7 @@ __global__ __launch_bounds__(NT) void oz_gemm_kernel(const OzGemm g) {
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < RJ; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = m0 + wm + 16 * i + 4 * lk + r, col = n0 + wn + 16 * j + lr;
@@ -845,9 +848,10 @@ void run_part(const GemmDesc& d, int b0s, int nb0, int kb0, int KBc, bool first_
     if (d.out_f32) hipLaunchKernelGGL((oz_gemm_walk_kernel<float2, NSL>), dim3(grid), dim3(NT), 0, s, g, tiles);
     else hipLaunchKernelGGL((oz_gemm_walk_kernel<double2, NSL>), dim3(grid), dim3(NT), 0, s, g, tiles);
   } else if (d.out_f32) {
-    hipLaunchKernelGGL((oz_gemm_kernel<float2, NSL>), dim3((unsigned)tiles), dim3(NT), 0, s, g);
+    // eight waves (two per SIMD): 1-3 % faster than four on every long-K site
+    hipLaunchKernelGGL((oz_gemm_kernel<float2, NSL, 4>), dim3((unsigned)tiles), dim3(512), 0, s, g);
   } else {
-    hipLaunchKernelGGL((oz_gemm_kernel<double2, NSL>), dim3((unsigned)tiles), dim3(NT), 0, s, g);
+    hipLaunchKernelGGL((oz_gemm_kernel<double2, NSL, 4>), dim3((unsigned)tiles), dim3(512), 0, s, g);
   }
   GGPU_HIP(hipGetLastError());
 }
