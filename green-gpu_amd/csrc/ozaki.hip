@@ -552,17 +552,19 @@ __device__ inline OzTile oz_tile(const OzGemm& g, int64_t L, int tiles_m, int ti
   return c;
 }
 
-// Variant for short k extents (a few k blocks per tile), persistent: each
-// workgroup walks several tiles (workgroup w, on XCD w % 8,
+// Variant for short k extents (a few k blocks per tile), persistent (waves
+// laid out as in oz_gemm_kernel): each workgroup walks several tiles
+// (workgroup w, on XCD w % 8,
 // takes every nx-th tile of that XCD's contiguous range, nx = the XCD's
 // workgroups), so the tiles an XCD runs at once are neighbours in the tile
 // order.  The copy of a tile's first k block is issued with the previous
 // tile's last k block, so the epilogue of one tile runs while the next
 // tile's first stage lands; the exponents of the rows and columns a lane
 // stores are fetched with the tile's last k block.
-template <class TO, int NSL>
-__global__ __launch_bounds__(NT) void oz_gemm_walk_kernel(const OzGemm g, int64_t ntiles) {
+template <class TO, int NSL, int WNW>
+__global__ __launch_bounds__(128 * WNW) void oz_gemm_walk_kernel(const OzGemm g, int64_t ntiles) {
   constexpr int DMAX = NSL, NP = 2 * NSL, STAGE = stage_bytes<NSL>(), STAGE_A = STAGE / 2;
+  constexpr int NW = 2 * WNW, RJ = 4 / WNW;   // waves; column blocks per wave (2 row blocks each)
   __shared__ __attribute__((aligned(1024))) int8_t lds[2 * STAGE];
   const int tiles_m = (g.M + TILE - 1) / TILE, tiles_n = (g.N + TILE - 1) / TILE;
   const int64_t P = gridDim.x, x = blockIdx.x % 8, slot = blockIdx.x / 8;
@@ -577,7 +579,7 @@ __global__ __launch_bounds__(NT) void oz_gemm_walk_kernel(const OzGemm g, int64_
   // kstrideA / kstrideB bytes apart
   const int64_t kstrideA = (int64_t)(g.Mp / RB) * NP * FRAG, kstrideB = (int64_t)(g.Np / RB) * NP * FRAG;
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
-  const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+  const int wm = (w / WNW) * 32, wn = (w % WNW) * 16 * RJ;
   const int lr = lane & 15, lk = lane >> 4;
 
   // wave w copies fragments w, w + 4, .. of the stage: NSL of A, then NSL of B
@@ -588,21 +590,21 @@ __global__ __launch_bounds__(NT) void oz_gemm_walk_kernel(const OzGemm g, int64_
                          (int64_t)kb * kstrideB + 16 * lane;
     int8_t* dst = lds + st * STAGE;
 #pragma unroll
-    for (int it = 0; it < STAGE_A / FRAG / 4; ++it) {
-      const int f = w + 4 * it;
+    for (int it = 0; it < STAGE_A / FRAG / NW; ++it) {
+      const int f = w + NW * it;
       __builtin_amdgcn_global_load_lds((const void*)(srcA + f * FRAG),
                                        (__attribute__((address_space(3))) void*)(dst + f * FRAG), 16, 0, 0);
     }
 #pragma unroll
-    for (int it = 0; it < STAGE_A / FRAG / 4; ++it) {
-      const int f = w + 4 * it;
+    for (int it = 0; it < STAGE_A / FRAG / NW; ++it) {
+      const int f = w + NW * it;
       __builtin_amdgcn_global_load_lds((const void*)(srcB + f * FRAG),
                                        (__attribute__((address_space(3))) void*)(dst + STAGE_A + f * FRAG), 16, 0,
                                        0);
     }
   };
   // exponents of the rows / columns this lane stores
-  auto load_exps = [&](const OzTile& c, int (&ea)[2][4], int (&eb)[2]) {
+  auto load_exps = [&](const OzTile& c, int (&ea)[2][4], int (&eb)[RJ]) {
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -611,24 +613,24 @@ __global__ __launch_bounds__(NT) void oz_gemm_walk_kernel(const OzGemm g, int64_
         ea[i][r] = row < g.M ? g.EA[c.bA * g.Mp + row] : INT_MIN;
       }
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < RJ; ++j) {
       const int col = c.n0 + wn + 16 * j + lr;
       eb[j] = col < g.N ? g.EB[c.bB * g.Np + col] : INT_MIN;
     }
   };
 
   OzTile cur = oz_tile(g, L, tiles_m, tiles_n);
-  v4i re[2][2][DMAX + 1], im[2][2][DMAX + 1];
+  v4i re[2][RJ][DMAX + 1], im[2][RJ][DMAX + 1];
   // the products of one k block from stage st
   auto compute = [&](int st) {
     const int8_t* sA = lds + st * STAGE + (wm / RB) * NP * FRAG + 16 * lane;
     const int8_t* sB = lds + st * STAGE + STAGE_A + (wn / RB) * NP * FRAG + 16 * lane;
     // B fragments of slice u: bf[j][2u] (re), bf[j][2u + 1] (im), block column j
-    v4i bf[2][NP];
+    v4i bf[RJ][NP];
 #pragma unroll
     for (int u = 0; u < NSL && u <= DMAX - (NSL - 1); ++u)
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
+      for (int j = 0; j < RJ; ++j) {
         bf[j][2 * u] = lds_frag(sB + (j * NP + 2 * u) * FRAG);
         bf[j][2 * u + 1] = lds_frag(sB + (j * NP + 2 * u + 1) * FRAG);
       }
@@ -646,7 +648,7 @@ __global__ __launch_bounds__(NT) void oz_gemm_walk_kernel(const OzGemm g, int64_
       if (i == 0 && s >= 1 && DMAX - s + 1 < NSL) {
         const int u = DMAX - s + 1;
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < RJ; ++j) {
           bf[j][2 * u] = lds_frag(sB + (j * NP + 2 * u) * FRAG);
           bf[j][2 * u + 1] = lds_frag(sB + (j * NP + 2 * u + 1) * FRAG);
         }
@@ -657,7 +659,7 @@ __global__ __launch_bounds__(NT) void oz_gemm_walk_kernel(const OzGemm g, int64_
       for (int u = 0; u < NSL && u + s <= DMAX; ++u) {
         const int d = s + u;
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < RJ; ++j) {
           re[i][j][d] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ar, bf[j][2 * u], re[i][j][d], 0, 0, 0);
           im[i][j][d] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ar, bf[j][2 * u + 1], im[i][j][d], 0, 0, 0);
         }
@@ -666,7 +668,7 @@ __global__ __launch_bounds__(NT) void oz_gemm_walk_kernel(const OzGemm g, int64_
       for (int u = 0; u < NSL && u + s <= DMAX; ++u) {
         const int d = s + u;
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < RJ; ++j) {
           re[i][j][d] = __builtin_amdgcn_mfma_i32_16x16x64_i8(nai, bf[j][2 * u + 1], re[i][j][d], 0, 0, 0);
           im[i][j][d] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ai, bf[j][2 * u], im[i][j][d], 0, 0, 0);
         }
@@ -683,7 +685,7 @@ __global__ __launch_bounds__(NT) void oz_gemm_walk_kernel(const OzGemm g, int64_
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j)
+      for (int j = 0; j < RJ; ++j)
 #pragma unroll
         for (int d = 0; d <= DMAX; ++d) { re[i][j][d] = v4i{0, 0, 0, 0}; im[i][j][d] = v4i{0, 0, 0, 0}; }
     for (int kb = 0; kb + 1 < g.nkb; ++kb, st ^= 1) {
@@ -697,7 +699,7 @@ __global__ __launch_bounds__(NT) void oz_gemm_walk_kernel(const OzGemm g, int64_
     // next copy, so waiting for them leaves that copy in flight), then the
     // next tile's first k block
     __syncthreads();
-    int ea[2][4], eb[2];
+    int ea[2][4], eb[RJ];
     load_exps(cur, ea, eb);
     const bool more = L + nx < end;
     const OzTile nxt = oz_tile(g, more ? L + nx : L, tiles_m, tiles_n);
@@ -708,7 +710,7 @@ __global__ __launch_bounds__(NT) void oz_gemm_walk_kernel(const OzGemm g, int64_
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j)
+      for (int j = 0; j < RJ; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int row = cur.m0 + wm + 16 * i + 4 * lk + r, col = cur.n0 + wn + 16 * j + lr;
@@ -845,8 +847,9 @@ void run_part(const GemmDesc& d, int b0s, int nb0, int kb0, int KBc, bool first_
     // tile's last k block (K = nao = 104 launches: 5-11 % faster; on the
     // K = 600 and K = 10816 launches walking tiles cost 1-2 %)
     const unsigned grid = (unsigned)std::min<int64_t>(tiles, compute_units());
-    if (d.out_f32) hipLaunchKernelGGL((oz_gemm_walk_kernel<float2, NSL>), dim3(grid), dim3(NT), 0, s, g, tiles);
-    else hipLaunchKernelGGL((oz_gemm_walk_kernel<double2, NSL>), dim3(grid), dim3(NT), 0, s, g, tiles);
+    // eight waves (two per SIMD): 4-14 % faster than four at the K = nao sites
+    if (d.out_f32) hipLaunchKernelGGL((oz_gemm_walk_kernel<float2, NSL, 4>), dim3(grid), dim3(512), 0, s, g, tiles);
+    else hipLaunchKernelGGL((oz_gemm_walk_kernel<double2, NSL, 4>), dim3(grid), dim3(512), 0, s, g, tiles);
   } else if (d.out_f32) {
     // eight waves (two per SIMD): 1-3 % faster than four on every long-K site
     hipLaunchKernelGGL((oz_gemm_kernel<float2, NSL, 4>), dim3((unsigned)tiles), dim3(512), 0, s, g);
