@@ -367,9 +367,15 @@ int ggpu_gw_graph_captures(ggpu_gw_plan* plan, int* captures);
  * iteration's own launch sequence (operands cold in the caches as they are in
  * an iteration): the P0 and Sigma chains of the first q of this plan's share,
  * `batches` pair batches per chain (and per star member), best of `rounds`
- * (1 .. 8).  Candidates: the LDS k-depth and waves-per-SIMD register
- * allocation for FP64 / FP32, the workgroup tile for split-bf16.  The choice
- * changes no result (every variant sums in the same order); it is kept by the
+ * (1 .. 8).  Candidates (variant ids): the LDS k-depth and waves-per-SIMD
+ * register allocation for FP64 / FP32 (8, 16, 83, 84); the workgroup tile for
+ * split-bf16 (0, 1, 4 .. 9; 10 .. 12: 128 x 128 from operands split once into
+ * bf16 planes in the plan's workspace, four, eight or sixteen waves); in the
+ * int8 modes the emulation with the mode's slices (200), with five slices
+ * (201, four-slice plans) or the FP64 kernel (83).  Within FP64, FP32 and
+ * split-bf16 the choice changes no result (every variant sums in the same
+ * order); in the int8 modes it changes the rounding within the mode's
+ * bound.  The choice is kept by the
  * plan and used by every later call.  Nothing the caller sees is written:
  * the plan's own buffers hold the trial results, and the next iterate starts
  * from zero.  G / G_device as for ggpu_gw_iterate / ggpu_gw_iterate_device.
