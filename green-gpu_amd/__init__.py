@@ -127,7 +127,7 @@ EXPORTED = ["ggpu_last_error", "ggpu_device_count", "ggpu_device_memory",
 VARIANTS_MFMA = {8: "bk8", 16: "bk16", 83: "bk8_w3", 84: "bk8_w4", 200: "int8", 201: "int8x5"}
 VARIANTS_BF16 = {0: "64x64", 1: "128x64", 4: "64x128", 5: "160x64", 6: "96x64_w3", 7: "64x96_w3", 8: "64x64_w3",
                  9: "64x64_astat", 10: "128x128_presplit", 11: "128x128_presplit_w8",
-                 12: "128x128_presplit_w16"}
+                 12: "128x128_presplit_w16", 13: "128x192_presplit_w8"}
 
 SITES = ["other", "p0_gv", "p0_gvh", "p0_contract", "sig_vg", "sig_py", "sig_contract",
          "ir", "dyson", "hf", "rotate"]

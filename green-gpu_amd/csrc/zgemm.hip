@@ -819,10 +819,12 @@ thread_local const int* g_site_variants = nullptr;   // [kNumSites], -1: built-i
 // order, so all split-bf16 tiles give the same sums.
 // ---------------------------------------------------------------------------
 constexpr int FB_KB = 32;        // k per fragment / MFMA step
-constexpr int FB_TILE = 128;     // workgroup tile (both dimensions)
+constexpr int FB_TILE = 128;     // workgroup tile rows (columns: 128 or 192)
 constexpr int FB_FRAG = 1024;
 constexpr int FB_NP = 4;         // planes per element
-constexpr int FB_STAGE = 2 * (FB_TILE / 16) * FB_NP * FB_FRAG;   // both operands, one k block: 64 KiB
+// both operands' fragments of one k block: 64 KiB (128 columns) or 80 KiB (192)
+template <int TN>
+constexpr int fb_stage() { return (FB_TILE / 16 + TN / 16) * FB_NP * FB_FRAG; }
 
 struct FbView {
   const float2* src = nullptr;   // element (row 0, k 0, kb 0, batch 0)
@@ -878,18 +880,20 @@ struct FbGemm {
   double alpha_re = 1.0, alpha_im = 0.0, beta_re = 0.0, beta_im = 0.0;
 };
 
-// 128 x 128 tile per workgroup of WMW x WNW waves, each wave RI x RJ blocks
-// of the 16 x 16 MFMA output: 2 x 2 waves of 64 x 64 (one wave per SIMD),
-// 4 x 2 of 32 x 64 (two), 4 x 4 of 32 x 32 (four).  A stage holds both
-// operands' fragments of one k block (64 KiB); two stages, one barrier per
-// k block; a wave keeps its B fragments of the k block in registers and
-// walks its row blocks, reading the next row block's A fragments ahead.
-template <class TO, int WMW, int WNW>
+// 128 x TN tile per workgroup of WMW x WNW waves, each wave RI x RJ blocks
+// of the 16 x 16 MFMA output: for TN = 128, 2 x 2 waves of 64 x 64 (one wave
+// per SIMD), 4 x 2 of 32 x 64 (two), 4 x 4 of 32 x 32 (four); for TN = 192,
+// 2 x 4 waves of 64 x 48 (two).  A stage holds both operands' fragments of
+// one k block; two stages, one barrier per k block; a wave keeps its B
+// fragments of the k block in registers and walks its row blocks, reading
+// the next row block's A fragments ahead.
+template <class TO, int WMW, int WNW, int TN>
 __global__ __launch_bounds__(64 * WMW * WNW) void gemm_bf16x3_frag_kernel(const FbGemm g) {
-  constexpr int NW = WMW * WNW, RI = 8 / WMW, RJ = 8 / WNW;
-  constexpr int STAGE_A = FB_STAGE / 2;
-  __shared__ __attribute__((aligned(1024))) int8_t lds[2 * FB_STAGE];
-  const int tiles_m = g.Mp / FB_TILE, tiles_n = g.Np / FB_TILE;
+  constexpr int NW = WMW * WNW, RI = 8 / WMW, RJ = TN / 16 / WNW;
+  constexpr int STAGE = fb_stage<TN>(), STAGE_A = (FB_TILE / 16) * FB_NP * FB_FRAG;
+  constexpr int STAGE_B = STAGE - STAGE_A;
+  __shared__ __attribute__((aligned(1024))) int8_t lds[2 * STAGE];
+  const int tiles_m = g.Mp / FB_TILE, tiles_n = g.Np / TN;
   int64_t L = xcd_logical(blockIdx.x, gridDim.x);
   int mt, nt;
   int64_t b0, b1;
@@ -908,7 +912,7 @@ __global__ __launch_bounds__(64 * WMW * WNW) void gemm_bf16x3_frag_kernel(const 
     b1 = L % g.nb1;
     b0 = L / g.nb1;
   }
-  const int m0 = mt * FB_TILE, n0 = nt * FB_TILE;
+  const int m0 = mt * FB_TILE, n0 = nt * TN;
   const int64_t bA = g.a_shared ? 0 : b0 * g.nb1 + b1, bB = g.b_shared ? 0 : b0 * g.nb1 + b1;
   const int64_t kstrideA = (int64_t)(g.Mp / 16) * FB_NP * FB_FRAG, kstrideB = (int64_t)(g.Np / 16) * FB_NP * FB_FRAG;
   const int8_t* gA = g.Aq + bA * g.nkb * kstrideA + (int64_t)(m0 / 16) * FB_NP * FB_FRAG;
@@ -921,7 +925,7 @@ __global__ __launch_bounds__(64 * WMW * WNW) void gemm_bf16x3_frag_kernel(const 
   auto copy_stage = [&](int64_t kb, int st) {
     const int8_t* srcA = gA + kb * kstrideA + 16 * lane;
     const int8_t* srcB = gB + kb * kstrideB + 16 * lane;
-    int8_t* dst = lds + st * FB_STAGE;
+    int8_t* dst = lds + st * STAGE;
 #pragma unroll
     for (int it = 0; it < STAGE_A / FB_FRAG / NW; ++it) {
       const int f = w + NW * it;
@@ -929,7 +933,7 @@ __global__ __launch_bounds__(64 * WMW * WNW) void gemm_bf16x3_frag_kernel(const 
                                        (__attribute__((address_space(3))) void*)(dst + f * FB_FRAG), 16, 0, 0);
     }
 #pragma unroll
-    for (int it = 0; it < STAGE_A / FB_FRAG / NW; ++it) {
+    for (int it = 0; it < STAGE_B / FB_FRAG / NW; ++it) {
       const int f = w + NW * it;
       __builtin_amdgcn_global_load_lds((const void*)(srcB + f * FB_FRAG),
                                        (__attribute__((address_space(3))) void*)(dst + STAGE_A + f * FB_FRAG), 16,
@@ -952,8 +956,8 @@ __global__ __launch_bounds__(64 * WMW * WNW) void gemm_bf16x3_frag_kernel(const 
     // stage the next copy overwrites
     __syncthreads();
     if (kb + 1 < g.nkb) copy_stage(kb + 1, st ^ 1);
-    const int8_t* sA = lds + st * FB_STAGE + (wm / 16) * FB_NP * FB_FRAG + 16 * lane;
-    const int8_t* sB = lds + st * FB_STAGE + STAGE_A + (wn / 16) * FB_NP * FB_FRAG + 16 * lane;
+    const int8_t* sA = lds + st * STAGE + (wm / 16) * FB_NP * FB_FRAG + 16 * lane;
+    const int8_t* sB = lds + st * STAGE + STAGE_A + (wn / 16) * FB_NP * FB_FRAG + 16 * lane;
     v8bf brh[RJ], bih[RJ], brl[RJ], bil[RJ];
 #pragma unroll
     for (int j = 0; j < RJ; ++j) {
@@ -1020,32 +1024,32 @@ __global__ __launch_bounds__(64 * WMW * WNW) void gemm_bf16x3_frag_kernel(const 
 
 inline int64_t fb_round(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
-// Bytes of split planes one launch over nb0 batches needs.
-inline size_t fb_bytes(const GemmDesc& d, int nb0, bool a_shared, bool b_shared) {
+// Bytes of split planes one launch over nb0 batches needs (B padded to TN).
+inline size_t fb_bytes(const GemmDesc& d, int nb0, bool a_shared, bool b_shared, int TN) {
   const int64_t kpb = (d.K + FB_KB - 1) / FB_KB, nkb = kpb * d.KB;
   const int64_t nbA = a_shared ? 1 : (int64_t)nb0 * d.nb1, nbB = b_shared ? 1 : (int64_t)nb0 * d.nb1;
   const int64_t per_row_block = nkb * FB_NP * FB_FRAG;
-  return (size_t)(nbA * (fb_round(d.M, FB_TILE) / 16) + nbB * (fb_round(d.N, FB_TILE) / 16)) * per_row_block;
+  return (size_t)(nbA * (fb_round(d.M, FB_TILE) / 16) + nbB * (fb_round(d.N, TN) / 16)) * per_row_block;
 }
 
 // Whether the pre-split tile can run this launch with the workspace it has
 // (one batch of its planes must fit).
-inline bool fb_fits(const GemmDesc& d) {
+inline bool fb_fits(const GemmDesc& d, int TN) {
   const bool a_shared = d.A.bs0 == 0 && d.A.bs1 == 0 && d.A.off1 == nullptr;
   const bool b_shared = d.B.bs0 == 0 && d.B.bs1 == 0 && d.B.off1 == nullptr;
-  return g_workspace != nullptr && fb_bytes(d, 1, a_shared, b_shared) <= g_workspace_bytes;
+  return g_workspace != nullptr && fb_bytes(d, 1, a_shared, b_shared, TN) <= g_workspace_bytes;
 }
 
-template <class TO, int WMW, int WNW>
+template <class TO, int WMW, int WNW, int TN = 128>
 void launch_bf16x3_frag(const GemmDesc& d, hipStream_t stream) {
   const bool a_shared = d.A.bs0 == 0 && d.A.bs1 == 0 && d.A.off1 == nullptr;
   const bool b_shared = d.B.bs0 == 0 && d.B.bs1 == 0 && d.B.off1 == nullptr;
   const int kpb = (d.K + FB_KB - 1) / FB_KB;
   const int64_t nkb = (int64_t)kpb * d.KB;
-  const int Mp = (int)fb_round(d.M, FB_TILE), Np = (int)fb_round(d.N, FB_TILE);
+  const int Mp = (int)fb_round(d.M, FB_TILE), Np = (int)fb_round(d.N, TN);
   int nb0c = d.nb0;   // batches per part: their planes fit the workspace
-  while (nb0c > 1 && fb_bytes(d, nb0c, a_shared, b_shared) > g_workspace_bytes) nb0c = (nb0c + 1) / 2;
-  require(fb_bytes(d, nb0c, a_shared, b_shared) <= g_workspace_bytes, "gemm: split-bf16 workspace too small");
+  while (nb0c > 1 && fb_bytes(d, nb0c, a_shared, b_shared, TN) > g_workspace_bytes) nb0c = (nb0c + 1) / 2;
+  require(fb_bytes(d, nb0c, a_shared, b_shared, TN) <= g_workspace_bytes, "gemm: split-bf16 workspace too small");
   require(Mp / 16 <= 65535 && Np / 16 <= 65535, "gemm: split-bf16 operand too large");
   const size_t elem = d.out_f32 ? sizeof(float2) : sizeof(double2);
   for (int b0s = 0; b0s < d.nb0; b0s += nb0c) {
@@ -1078,9 +1082,9 @@ void launch_bf16x3_frag(const GemmDesc& d, hipStream_t stream) {
     g.C = static_cast<char*>(d.C.ptr) + (int64_t)b0s * d.C.bs0 * (int64_t)elem;
     g.c_rs = d.C.rs; g.c_cs = d.C.cs; g.c_bs0 = d.C.bs0; g.c_bs1 = d.C.bs1; g.c_off1 = d.C.off1;
     g.alpha_re = d.alpha_re; g.alpha_im = d.alpha_im; g.beta_re = d.beta_re; g.beta_im = d.beta_im;
-    const int64_t tiles = (int64_t)(Mp / FB_TILE) * (Np / FB_TILE) * nb * d.nb1;
+    const int64_t tiles = (int64_t)(Mp / FB_TILE) * (Np / TN) * nb * d.nb1;
     require(tiles < ((int64_t)1 << 31), "gemm: grid too large");
-    hipLaunchKernelGGL((gemm_bf16x3_frag_kernel<TO, WMW, WNW>), dim3((unsigned)tiles), dim3(64 * WMW * WNW), 0,
+    hipLaunchKernelGGL((gemm_bf16x3_frag_kernel<TO, WMW, WNW, TN>), dim3((unsigned)tiles), dim3(64 * WMW * WNW), 0,
                        stream, g);
     GGPU_HIP(hipGetLastError());
   }
@@ -1174,13 +1178,16 @@ void launch_bf16x3_site(const GemmDesc& d, hipStream_t stream, int variant) {
       if (astat_fits(d)) return launch_bf16x3_astat<TO, SITE>(d, stream);
       return launch_bf16x3_big<TO, 2, 2, SITE, 3>(d, stream);   // a 64 x 64 tile, A staged per batch
     case kBf16TileFrag128x128:
-      if (fb_fits(d)) return launch_bf16x3_frag<TO, 2, 2>(d, stream);
+      if (fb_fits(d, 128)) return launch_bf16x3_frag<TO, 2, 2>(d, stream);
       return launch_bf16x3_big<TO, 5, 2, SITE>(d, stream);   // no workspace: 160 x 64
     case kBf16TileFrag128x128W8:
-      if (fb_fits(d)) return launch_bf16x3_frag<TO, 4, 2>(d, stream);
+      if (fb_fits(d, 128)) return launch_bf16x3_frag<TO, 4, 2>(d, stream);
       return launch_bf16x3_big<TO, 5, 2, SITE>(d, stream);
     case kBf16TileFrag128x128W16:
-      if (fb_fits(d)) return launch_bf16x3_frag<TO, 4, 4>(d, stream);
+      if (fb_fits(d, 128)) return launch_bf16x3_frag<TO, 4, 4>(d, stream);
+      return launch_bf16x3_big<TO, 5, 2, SITE>(d, stream);
+    case kBf16TileFrag128x192W8:
+      if (fb_fits(d, 192)) return launch_bf16x3_frag<TO, 2, 4, 192>(d, stream);
       return launch_bf16x3_big<TO, 5, 2, SITE>(d, stream);
     default: break;
   }
@@ -1326,7 +1333,8 @@ int gemm_variant_candidates(Compute c, int* out, int cap) {
   static const int f32[] = {8, 16, 83};
   static const int bf[] = {kBf16Tile128x64, kBf16Tile64x128, kBf16Tile160x64, kBf16Tile96x64W3,
                            kBf16Tile64x96W3, kBf16Tile64x64W3, kBf16Tile64x64, kBf16TileAstat64x64,
-                           kBf16TileFrag128x128, kBf16TileFrag128x128W8, kBf16TileFrag128x128W16};
+                           kBf16TileFrag128x128, kBf16TileFrag128x128W8, kBf16TileFrag128x128W16,
+                           kBf16TileFrag128x192W8};
   if (int8_slices(c)) {                  // int8 emulation or the FP64 kernel
     // four-slice plans may also take five slices where four are over budget
     static const int i8x5[] = {kVariantInt8, 83}, i8x4[] = {kVariantInt8, kVariantInt8x5, 83};
@@ -1336,7 +1344,7 @@ int gemm_variant_candidates(Compute c, int* out, int cap) {
     return n;
   }
   const int* v = c == Compute::F64 ? f64 : c == Compute::F32 ? f32 : bf;
-  const int n = c == Compute::F64 ? 3 : c == Compute::F32 ? 3 : 11;
+  const int n = c == Compute::F64 ? 3 : c == Compute::F32 ? 3 : 12;
   for (int i = 0; i < n && i < cap; ++i) out[i] = v[i];
   return n;
 }

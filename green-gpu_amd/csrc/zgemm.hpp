@@ -97,7 +97,8 @@ enum {
   // the plan's workspace (no workspace: the 160 x 64 tile)
   kBf16TileFrag128x128 = 10,
   kBf16TileFrag128x128W8 = 11,   // the same with eight waves (two per SIMD)
-  kBf16TileFrag128x128W16 = 12   // and with sixteen (four per SIMD)
+  kBf16TileFrag128x128W16 = 12,  // and with sixteen (four per SIMD)
+  kBf16TileFrag128x192W8 = 13    // 128 x 192, eight waves
 };
 // Variant of a Compute::I8X5 / I8X4 launch that runs the int8 emulation with
 // the mode's slice count; kVariantInt8x5 runs it with five slices in either

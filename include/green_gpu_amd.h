@@ -370,7 +370,8 @@ int ggpu_gw_graph_captures(ggpu_gw_plan* plan, int* captures);
  * (1 .. 8).  Candidates (variant ids): the LDS k-depth and waves-per-SIMD
  * register allocation for FP64 / FP32 (8, 16, 83, 84); the workgroup tile for
  * split-bf16 (0, 1, 4 .. 9; 10 .. 12: 128 x 128 from operands split once into
- * bf16 planes in the plan's workspace, four, eight or sixteen waves); in the
+ * bf16 planes in the plan's workspace, four, eight or sixteen waves; 13:
+ * 128 x 192, eight waves); in the
  * int8 modes the emulation with the mode's slices (200), with five slices
  * (201, four-slice plans) or the FP64 kernel (83).  Within FP64, FP32 and
  * split-bf16 the choice changes no result (every variant sums in the same

@@ -427,7 +427,7 @@ def test_autotune_picks_fastest_and_keeps_results(ggpu, synth, oracle, precision
     rep = kern.autotune(G, rounds=2)
     names = ["p0_gv", "p0_gvh", "p0_contract", "sig_vg", "sig_py", "sig_contract"]
     assert sorted(rep["sites"]) == sorted(names)
-    ncand = {"fp64": 3, "fp32": 3, "bf16x3": 11}[precision]
+    ncand = {"fp64": 3, "fp32": 3, "bf16x3": 12}[precision]
     chosen = kern.get_variants()
     for si, name in enumerate(ggpu.SITES):
         if name not in names:
@@ -641,7 +641,7 @@ def test_bf16_a_stationary_tile_keeps_results(ggpu, synth, oracle, nao, NQ):
     assert max_abs(out[1], Sig_or) < 2e-4 * np.abs(Sig_or).max()
 
 
-@pytest.mark.parametrize("nao,NQ,variant", [(13, 40, 10), (40, 47, 10), (40, 47, 11), (40, 47, 12)])
+@pytest.mark.parametrize("nao,NQ,variant", [(13, 40, 10), (40, 47, 10), (40, 47, 11), (40, 47, 12), (40, 47, 13)])
 def test_bf16_presplit_tile_keeps_results(ggpu, synth, oracle, nao, NQ, variant):
     """The pre-split split-bf16 tile (operands split once into fragment-native
     bf16 planes, 128 x 128 tiles fed by load-to-LDS copies) forced on all six
@@ -660,7 +660,7 @@ def test_bf16_presplit_tile_keeps_results(ggpu, synth, oracle, nao, NQ, variant)
             for name in ("p0_gv", "p0_gvh", "p0_contract", "sig_vg", "sig_py", "sig_contract"):
                 v[ggpu.SITES.index(name)] = variant
             k.set_variants(v)
-            assert k.variant_name(variant).startswith("128x128_presplit")
+            assert "presplit" in k.variant_name(variant)
         out.append(k.solve(G))
     assert np.array_equal(out[0], out[1])
     Sig_or = oracle.gw_solve(G, V, oracle.symmetry_none(mesh, nao, NQ), T_wt, T_tw, solver="cholesky")
