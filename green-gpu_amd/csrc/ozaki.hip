@@ -208,6 +208,69 @@ __global__ __launch_bounds__(NT) void oz_slice_kernel(const SliceView v) {
 // batches); one workgroup per CU (the staging fills the LDS).
 constexpr int LDS_KT = 632;
 
+// Digit pass for k-contiguous operands with a long k extent (after the
+// exponent pass): one wave per (16-row block, k block), which first copies
+// its 16 rows x 64 k into its own LDS area with load-to-LDS copies -- one
+// instruction per row, 64 lanes reading 1 KiB contiguous -- and then reads
+// each lane's 16 k values from there (rows 1040 bytes apart: conflict-free).
+// Reading the 16 k values straight from memory instead touches 64 lines per
+// load instruction.  Lanes past the operand read a valid element and are
+// masked below; same digits as oz_slice_kernel.
+constexpr int KF_ROW = KBLK * 16 + 16;   // bytes per staged row (64 complex + pad)
+template <int NSL>
+__global__ __launch_bounds__(64) void oz_digits_kf_kernel(const SliceView v) {
+  constexpr int NP = 2 * NSL;
+  __shared__ __attribute__((aligned(16))) int8_t T[RB * KF_ROW];
+  const int64_t b = blockIdx.z;
+  const int rblk = blockIdx.y, r0 = rblk * RB, kb = blockIdx.x;
+  const int lane = threadIdx.x, lr = lane & 15, lk = lane >> 4;
+  const double2* base = batch_base(v, b);
+  const int kt = v.K * v.KB;
+  {
+    const int kk = min(kb * KBLK + lane, kt - 1);
+    const int q = kk / v.K, k = kk - q * v.K;
+    const int64_t koff = (int64_t)k * v.k_s + (int64_t)q * v.kb_s;
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+      const int row = (r0 + i < v.rows) ? r0 + i : 0;
+      __builtin_amdgcn_global_load_lds((const void*)(base + (int64_t)row * v.row_s + koff),
+                                       (__attribute__((address_space(3))) void*)(T + i * KF_ROW), 16, 0, 0);
+    }
+    __builtin_amdgcn_s_waitcnt(0x3f70);   // vmcnt(0): this wave's copies have landed (the area is its own)
+  }
+  const int row = r0 + lr;
+  const bool live = row < v.rows;
+  const int sg = live ? sigma_of(v.E[b * v.Rp + row]) : 0;
+  const int rblocks = v.Rp / RB;
+  unsigned int wre[NSL][4], wim[NSL][4];
+#pragma unroll
+  for (int s = 0; s < NSL; ++s)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) { wre[s][c] = 0u; wim[s][c] = 0u; }
+  const double2* Tl = reinterpret_cast<const double2*>(T + lr * KF_ROW) + 16 * lk;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int k = kb * KBLK + 16 * lk + j;
+    const double2 x = (live && k < kt) ? Tl[j] : make_double2(0.0, 0.0);
+    double xr = ldexp(x.x, sg), xi = ldexp(v.conj ? -x.y : x.y, sg);
+    const int sh = 8 * (j & 3);
+#pragma unroll
+    for (int s = 0; s < NSL; ++s) {
+      const double dr = trunc(xr), di = trunc(xi);
+      xr = (xr - dr) * 128.0;
+      xi = (xi - di) * 128.0;
+      wre[s][j >> 2] |= (unsigned int)((int)dr & 0xff) << sh;
+      wim[s][j >> 2] |= (unsigned int)((int)di & 0xff) << sh;
+    }
+  }
+  int8_t* dst = v.out + (((int64_t)b * v.nkb + kb) * rblocks + rblk) * ((int64_t)NP * FRAG) + 16 * lane;
+#pragma unroll
+  for (int s = 0; s < NSL; ++s) {
+    *reinterpret_cast<uint4*>(dst + (2 * s) * FRAG) = make_uint4(wre[s][0], wre[s][1], wre[s][2], wre[s][3]);
+    *reinterpret_cast<uint4*>(dst + (2 * s + 1) * FRAG) = make_uint4(wim[s][0], wim[s][1], wim[s][2], wim[s][3]);
+  }
+}
+
 template <int NSL, int NTH>
 __global__ __launch_bounds__(NTH) void oz_slice_lds_kernel(const SliceView v) {
   constexpr int NWV = NTH / 64, KC = NTH / RB;   // waves; k columns of the exponent pass
@@ -803,6 +866,12 @@ void slice_operand(SliceView v, int64_t nb, hipStream_t s) {
   hipLaunchKernelGGL(oz_exponent_kernel, dim3((unsigned)v.nkb, (unsigned)(v.Rp / TILE), (unsigned)nb), dim3(NT),
                      0, s, v);
   GGPU_HIP(hipGetLastError());
+  if (v.k_s == 1) {   // k-contiguous rows: stage each wave's block through LDS
+    hipLaunchKernelGGL((oz_digits_kf_kernel<NSL>), dim3((unsigned)v.nkb, (unsigned)(v.Rp / RB), (unsigned)nb),
+                       dim3(64), 0, s, v);
+    GGPU_HIP(hipGetLastError());
+    return;
+  }
   const int ngroups = (v.nkb + 3) / 4;
   hipLaunchKernelGGL((oz_slice_kernel<false, NSL>), dim3((unsigned)ngroups, (unsigned)(v.Rp / RB), (unsigned)nb),
                      dim3(NT), 0, s, v);
