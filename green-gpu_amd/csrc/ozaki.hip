@@ -66,6 +66,10 @@ struct SliceView {
   const double2* src = nullptr;   // element (row 0, k 0, kb 0, batch 0)
   int64_t row_s = 0, k_s = 0, kb_s = 0, bs0 = 0, bs1 = 0;
   const int64_t* off1 = nullptr;  // device: offset of batch b1 (replaces bs1)
+  // rows in groups of rgrp (0: one group), group g at g * rgrp_s: batches
+  // merged into the row index
+  int rgrp = 0;
+  int64_t rgrp_s = 0;
   int rows = 0, K = 0, KB = 1, nb1 = 1;
   int conj = 0;
   int Rp = 0, nkb = 0;            // padded rows, k blocks
@@ -78,12 +82,17 @@ __device__ inline const double2* batch_base(const SliceView& v, int64_t b) {
   return v.src + b0 * v.bs0 + (v.off1 ? v.off1[b1] : b1 * v.bs1);
 }
 
+// Offset of a row within its batch.
+__device__ inline int64_t row_off(const SliceView& v, int row) {
+  return v.rgrp ? (int64_t)(row / v.rgrp) * v.rgrp_s + (int64_t)(row % v.rgrp) * v.row_s : (int64_t)row * v.row_s;
+}
+
 // Element (row, flattened k) of one batch; zero outside the operand.  The
 // flattened k of one launch stays below 2^31 (K * KB <= 12800).
 __device__ inline double2 element(const SliceView& v, const double2* base, int row, int kk) {
   if (row >= v.rows || kk >= v.K * v.KB) return make_double2(0.0, 0.0);
   const int kb = kk / v.K, k = kk - kb * v.K;
-  return base[(int64_t)row * v.row_s + (int64_t)k * v.k_s + (int64_t)kb * v.kb_s];
+  return base[row_off(v, row) + (int64_t)k * v.k_s + (int64_t)kb * v.kb_s];
 }
 
 __device__ inline int max_exponent(double2 x) {
@@ -160,7 +169,7 @@ __global__ __launch_bounds__(NT) void oz_slice_kernel(const SliceView v) {
   const bool live = row < v.rows;
   const int sg = live ? sigma_of(FUSED ? rowexp[lr] : v.E[b * v.Rp + row]) : 0;
   const int rblocks = v.Rp / RB;
-  const double2* rowp = base + (int64_t)(live ? row : 0) * v.row_s;
+  const double2* rowp = base + row_off(v, live ? row : 0);
   const int kb_first = FUSED ? w : (int)blockIdx.x * 4 + w;
   const int kb_end = FUSED ? v.nkb : min(v.nkb, kb_first + 1);
   for (int kb = kb_first; kb < kb_end; kb += 4) {
@@ -233,7 +242,7 @@ __global__ __launch_bounds__(64) void oz_digits_kf_kernel(const SliceView v) {
 #pragma unroll
     for (int i = 0; i < RB; ++i) {
       const int row = (r0 + i < v.rows) ? r0 + i : 0;
-      __builtin_amdgcn_global_load_lds((const void*)(base + (int64_t)row * v.row_s + koff),
+      __builtin_amdgcn_global_load_lds((const void*)(base + row_off(v, row) + koff),
                                        (__attribute__((address_space(3))) void*)(T + i * KF_ROW), 16, 0, 0);
     }
     __builtin_amdgcn_s_waitcnt(0x3f70);   // vmcnt(0): this wave's copies have landed (the area is its own)
@@ -292,7 +301,7 @@ __global__ __launch_bounds__(NTH) void oz_slice_lds_kernel(const SliceView v) {
     for (int k0 = 4 * w; k0 < kt; k0 += 4 * NWV) {
       const int kk = min(k0 + lane / RB, kt - 1);
       const int q = kk / v.K, k = kk - q * v.K;
-      const double2* src = base + (int64_t)row * v.row_s + (int64_t)k * v.k_s + (int64_t)q * v.kb_s;
+      const double2* src = base + row_off(v, row) + (int64_t)k * v.k_s + (int64_t)q * v.kb_s;
       __builtin_amdgcn_global_load_lds((const void*)src, (__attribute__((address_space(3))) void*)&T[k0][0], 16, 0,
                                        0);
     }
@@ -360,6 +369,8 @@ struct OzGemm {
   int nb0 = 1, nb1 = 1, a_shared = 0, b_shared = 0, batch_major = 0;
   void* C = nullptr;
   int64_t c_rs = 0, c_cs = 0, c_bs0 = 0, c_bs1 = 0;
+  int cgrp = 0;                   // columns in groups of cgrp at cgrp_s (merged batches)
+  int64_t cgrp_s = 0;
   const int64_t* c_off1 = nullptr;
   double alpha_re = 1.0, alpha_im = 0.0, beta_re = 0.0, beta_im = 0.0;
 };
@@ -375,8 +386,9 @@ __device__ inline int64_t oz_xcd_logical(int64_t id, int64_t total) {
 
 template <class TO>
 __device__ inline void oz_store(const OzGemm& g, int64_t b0, int64_t b1, int row, int col, double vr, double vi) {
-  TO* C = static_cast<TO*>(g.C) + b0 * g.c_bs0 + (g.c_off1 ? g.c_off1[b1] : b1 * g.c_bs1) + row * g.c_rs +
-          col * g.c_cs;
+  const int64_t coff = g.cgrp ? (int64_t)(col / g.cgrp) * g.cgrp_s + (int64_t)(col % g.cgrp) * g.c_cs
+                             : (int64_t)col * g.c_cs;
+  TO* C = static_cast<TO*>(g.C) + b0 * g.c_bs0 + (g.c_off1 ? g.c_off1[b1] : b1 * g.c_bs1) + row * g.c_rs + coff;
   double cr = g.alpha_re * vr - g.alpha_im * vi, ci = g.alpha_re * vi + g.alpha_im * vr;
   if (g.beta_re != 0.0 || g.beta_im != 0.0) {
     const TO o = *C;
@@ -878,6 +890,28 @@ void slice_operand(SliceView v, int64_t nb, hipStream_t s) {
   GGPU_HIP(hipGetLastError());
 }
 
+// The int8 GEMM of one launch part: `tiles` 64 x 64 tiles of nkb k blocks.
+template <int NSL>
+void launch_gemm(const OzGemm& g, int64_t tiles, int64_t nkb, bool out_f32, hipStream_t s) {
+  require(tiles < ((int64_t)1 << 31), "int8 emulation: grid too large");
+  if (nkb <= 4) {
+    // short k extents: one workgroup per CU (the two LDS stages fill it),
+    // each walking tiles, so a tile's first copy hides behind the previous
+    // tile's last k block (K = nao = 104 launches: 5-11 % faster; on the
+    // K = 600 and K = 10816 launches walking tiles cost 1-2 %)
+    const unsigned grid = (unsigned)std::min<int64_t>(tiles, compute_units());
+    // eight waves (two per SIMD): 4-14 % faster than four at the K = nao sites
+    if (out_f32) hipLaunchKernelGGL((oz_gemm_walk_kernel<float2, NSL, 4>), dim3(grid), dim3(512), 0, s, g, tiles);
+    else hipLaunchKernelGGL((oz_gemm_walk_kernel<double2, NSL, 4>), dim3(grid), dim3(512), 0, s, g, tiles);
+  } else if (out_f32) {
+    // eight waves (two per SIMD): 1-3 % faster than four on every long-K site
+    hipLaunchKernelGGL((oz_gemm_kernel<float2, NSL, 4>), dim3((unsigned)tiles), dim3(512), 0, s, g);
+  } else {
+    hipLaunchKernelGGL((oz_gemm_kernel<double2, NSL, 4>), dim3((unsigned)tiles), dim3(512), 0, s, g);
+  }
+  GGPU_HIP(hipGetLastError());
+}
+
 // One launch over batches [b0s, b0s + nb0) and K-batch [kb0, kb0 + KBc).
 template <int NSL>
 void run_part(const GemmDesc& d, int b0s, int nb0, int kb0, int KBc, bool first_k, char* work, size_t work_bytes,
@@ -885,6 +919,44 @@ void run_part(const GemmDesc& d, int b0s, int nb0, int kb0, int KBc, bool first_
   const bool a_shared = d.A.bs0 == 0 && d.A.bs1 == 0 && d.A.off1 == nullptr;
   const bool b_shared = d.B.bs0 == 0 && d.B.bs1 == 0 && d.B.off1 == nullptr;
   const int64_t kt = (int64_t)d.K * KBc;
+  // One A for batches along b0 only (the G.V products of the P0 and Sigma
+  // chains): the batches become column groups of one product, C[:, (b, n)]
+  // = A B_b, so N = nao is not padded to the tile once per batch (p0_gvh and
+  // sig_vg at nao = 104: 11 % faster).  Same integer sums per output.
+  const bool merge = a_shared && d.nb1 == 1 && nb0 > 1 && d.KB == 1 && !b_shared && d.B.off1 == nullptr &&
+                     d.C.off1 == nullptr && (int64_t)d.N * nb0 < ((int64_t)1 << 30);
+  if (merge) {
+    const int Nm = d.N * nb0;
+    const Plan p = plan_for(d.M, Nm, kt, 1, 1, true, false, NSL);
+    require(p.total() <= work_bytes, "int8 emulation: workspace too small for one batch");
+    int* EA = reinterpret_cast<int*>(work);
+    int* EB = reinterpret_cast<int*>(work + p.bytesEA);
+    int8_t* Aq = reinterpret_cast<int8_t*>(work + p.bytesEA + p.bytesEB);
+    int8_t* Bq = Aq + p.bytesA;
+    SliceView a;
+    a.src = static_cast<const double2*>(d.A.ptr);
+    a.row_s = d.A.rs; a.k_s = d.A.cs; a.kb_s = d.A.kbs;
+    a.rows = d.M; a.K = d.K; a.KB = 1; a.nb1 = 1;
+    a.conj = d.A.conj; a.Rp = (int)p.Mp; a.nkb = (int)p.nkb; a.E = EA; a.out = Aq;
+    slice_operand<NSL>(a, 1, s);
+    SliceView b;
+    b.src = static_cast<const double2*>(d.B.ptr) + (int64_t)b0s * d.B.bs0;
+    b.row_s = d.B.cs; b.k_s = d.B.rs; b.kb_s = d.B.kbs; b.rgrp = d.N; b.rgrp_s = d.B.bs0;
+    b.rows = Nm; b.K = d.K; b.KB = 1; b.nb1 = 1;
+    b.conj = d.B.conj; b.Rp = (int)p.Np; b.nkb = (int)p.nkb; b.E = EB; b.out = Bq;
+    slice_operand<NSL>(b, 1, s);
+    OzGemm g;
+    g.Aq = Aq; g.Bq = Bq; g.EA = EA; g.EB = EB;
+    g.M = d.M; g.N = Nm; g.Mp = (int)p.Mp; g.Np = (int)p.Np; g.nkb = (int)p.nkb;
+    g.nb0 = 1; g.nb1 = 1; g.a_shared = 1; g.b_shared = 0; g.batch_major = 0;
+    g.C = static_cast<char*>(d.C.ptr) + (int64_t)b0s * d.C.bs0 * (d.out_f32 ? sizeof(float2) : sizeof(double2));
+    g.c_rs = d.C.rs; g.c_cs = d.C.cs; g.cgrp = d.N; g.cgrp_s = d.C.bs0;
+    g.alpha_re = d.alpha_re; g.alpha_im = d.alpha_im;
+    g.beta_re = first_k ? d.beta_re : 1.0;
+    g.beta_im = first_k ? d.beta_im : 0.0;
+    launch_gemm<NSL>(g, (int64_t)(p.Mp / TILE) * (p.Np / TILE), p.nkb, d.out_f32, s);
+    return;
+  }
   const Plan p = plan_for(d.M, d.N, kt, nb0, d.nb1, a_shared, b_shared, NSL);
   require(p.total() <= work_bytes, "int8 emulation: workspace too small for one batch");
   int* EA = reinterpret_cast<int*>(work);
@@ -915,24 +987,7 @@ void run_part(const GemmDesc& d, int b0s, int nb0, int kb0, int KBc, bool first_
   g.alpha_re = d.alpha_re; g.alpha_im = d.alpha_im;
   g.beta_re = first_k ? d.beta_re : 1.0;
   g.beta_im = first_k ? d.beta_im : 0.0;
-  const int64_t tiles = (int64_t)(p.Mp / TILE) * (p.Np / TILE) * nb0 * d.nb1;
-  require(tiles < ((int64_t)1 << 31), "int8 emulation: grid too large");
-  if (p.nkb <= 4) {
-    // short k extents: one workgroup per CU (the two LDS stages fill it),
-    // each walking tiles, so a tile's first copy hides behind the previous
-    // tile's last k block (K = nao = 104 launches: 5-11 % faster; on the
-    // K = 600 and K = 10816 launches walking tiles cost 1-2 %)
-    const unsigned grid = (unsigned)std::min<int64_t>(tiles, compute_units());
-    // eight waves (two per SIMD): 4-14 % faster than four at the K = nao sites
-    if (d.out_f32) hipLaunchKernelGGL((oz_gemm_walk_kernel<float2, NSL, 4>), dim3(grid), dim3(512), 0, s, g, tiles);
-    else hipLaunchKernelGGL((oz_gemm_walk_kernel<double2, NSL, 4>), dim3(grid), dim3(512), 0, s, g, tiles);
-  } else if (d.out_f32) {
-    // eight waves (two per SIMD): 1-3 % faster than four on every long-K site
-    hipLaunchKernelGGL((oz_gemm_kernel<float2, NSL, 4>), dim3((unsigned)tiles), dim3(512), 0, s, g);
-  } else {
-    hipLaunchKernelGGL((oz_gemm_kernel<double2, NSL, 4>), dim3((unsigned)tiles), dim3(512), 0, s, g);
-  }
-  GGPU_HIP(hipGetLastError());
+  launch_gemm<NSL>(g, (int64_t)(p.Mp / TILE) * (p.Np / TILE) * nb0 * d.nb1, p.nkb, d.out_f32, s);
 }
 
 }  // namespace
